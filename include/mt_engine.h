@@ -1,0 +1,166 @@
+/*
+ * mt_engine.h -- C ABI of the MI355X (gfx950) engine for MADDY's per-step
+ * Langevin hot path: monomer forces, pair-list rebuilds, the Euler-Maruyama
+ * integrator with HybridTaus noise, TEA hydrodynamics and the learned MLP-HI
+ * correction, for a batch of trajectories of uneven sizes (CSR offsets).
+ *
+ * What each entry point replaces in the reference (paths relative to the
+ * reference repository root):
+ *   mte_create / mte_destroy   initIntegration / deleteIntegration,
+ *                              src/compute_cuda.cu:977-1123, plus
+ *                              initTeaIntegrator / deleteTeaIntegrator,
+ *                              src/bdhitea.cu:13-55, and initRand,
+ *                              src/HybridTaus.cu:21-30
+ *   mte_run                    the per-step loop of compute(),
+ *                              src/compute_cuda.cu:1137-1240 (LJ_kernel,
+ *                              pairs_kernel, compute_kernel, updateTea +
+ *                              integrateTea or integrate_kernel), between two
+ *                              host events
+ *   mte_set_state/get_state    the cudaMemcpy of d_r, :1013-1014, :1177, :1204
+ *   mte_set_flags              the updater's re-uploads of extra / gtp /
+ *                              on_tubule_cur, :1157, :1190, :1202
+ *   mte_compute_energies       energy_kernel + host sums, :676-911 and
+ *                              src/updater.cpp:3-43
+ *   mte_compute_forces         compute_kernel, :32-525 (forces only)
+ *   mte_get_lists              the device lists pairs_kernel / LJ_kernel
+ *                              fill, :527-674, :913-940
+ *   mte_tea_stats              tea.h_epsilon / h_beta_ij, src/bdhitea.cu:57-118
+ *   mte_get_rng / mte_set_rng  ht.d_seeds, src/HybridTaus.cu:18-30
+ *
+ * Conventions: every call returns MTE_OK (0) or an MTE_ERR_* code and never
+ * exits the process; mte_last_error() gives the message.  The caller owns all
+ * host buffers; the engine owns device memory.  Calls on one engine are not
+ * thread-safe.  mte_run is asynchronous inside and synchronous at return.
+ */
+#ifndef MT_ENGINE_H
+#define MT_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MTE_OK 0
+#define MTE_ERR_ARG 1       /* bad argument or topology */
+#define MTE_ERR_HIP 2       /* HIP runtime error */
+#define MTE_ERR_OVERFLOW 3  /* a neighbour list exceeded its capacity */
+#define MTE_ERR_TEA 4       /* TEA capricious check (src/bdhitea.cu:89-107) */
+#define MTE_ERR_STATE 5     /* call not valid in the current state */
+#define MTE_ERR_NOISE 6     /* injected noise exhausted */
+
+/* Same layout as the reference's Coord, src/mt.h:63-71. */
+typedef struct { float x, y, z, fi, theta, psi, w; } mte_coord;
+
+/* Same layout as the reference's Energies, src/mt.h:92-100. */
+typedef struct { double U_harm, U_long, U_lat, U_psi, U_fi, U_teta, U_lj; } mte_energies;
+
+/* Derived constants the hot path reads (subset of Parameters,
+ * src/parameters.h:246-318; derivation src/preparator.cpp:124-221). */
+typedef struct {
+    float dt, temp, gammaR, gammaTheta, varR, varTheta, alpha, freeze_temp;
+    float C, B_psi, B_fi, B_theta, psi_0, fi_0, theta0_gtp, theta0_gdp;
+    float A_long, D_long, A_lat, D_lat, seam_coeff;
+    int32_t barrier;
+    float a_barr_long, r_barr_long, w_barr_long, a_barr_lat, r_barr_lat, w_barr_lat;
+    int32_t lj_on;
+    float ljscale, ljsigma6, ljpairscutoff;
+    int32_t ljpairsupdatefreq;
+    int32_t is_wall;
+    float rep_h, rep_r, rep_eps, rep_leftborder;
+    int32_t is_assembly;
+    int32_t tea_on;
+    float tea_a;
+    int32_t tea_epsilon_freq, tea_capricious;
+    float tea_epsmax;
+} mte_params;
+
+/* Topology of a CSR batch.  Trajectory t owns global beads
+ * [traj_offsets[t], traj_offsets[t+1]).  List entries are LOCAL indices in
+ * the reference's sign encodings (src/preparator.cpp:339-355 for harmonic,
+ * with -1 for "no partner"; src/compute_cuda.cu:587-592 longitudinal;
+ * :646-658 lateral, 999999 standing for j = 0).  The reference's
+ * per-monomer arrays (harmonic, fixed, mon_type) are given per bead. */
+typedef struct {
+    int32_t ntraj;
+    const int64_t *traj_offsets;        /* ntraj + 1 */
+    const int32_t *harmonic_count;      /* B, 0 or 1 */
+    const int32_t *harmonic;            /* B */
+    const int32_t *longitudinal_count;  /* B: initial lists (used as-is when is_assembly == 0) */
+    const int32_t *longitudinal;        /* B * 8 */
+    const int32_t *lateral_count;       /* B */
+    const int32_t *lateral;             /* B * 16 */
+    const uint8_t *fixed, *extra;       /* B */
+    const int32_t *mon_type, *gtp, *on_tubule; /* B */
+    const float *wall_z;                /* ntraj: Parameters.zs */
+} mte_topology;
+
+#define MTE_MODE_AUTO 0
+#define MTE_MODE_STREAMED 1  /* one thread per bead, state in HBM, per-step graph */
+#define MTE_MODE_RESIDENT 2  /* one workgroup per trajectory, state in LDS across steps */
+
+typedef struct {
+    int32_t max_lj;   /* LJ list capacity per bead (the reference uses 256) */
+    int32_t mode;     /* MTE_MODE_* */
+    int32_t rseed;    /* HybridTaus seeding via ran2 (src/HybridTaus.cu:32-48) */
+    int32_t device;   /* HIP device ordinal */
+} mte_options;
+
+typedef struct mte_engine mte_engine;
+
+int mte_create(const mte_params *params, const mte_topology *topo, const mte_options *opt, mte_engine **out);
+int mte_destroy(mte_engine *e);
+/* Message of the last failing call on e (or of the last failed mte_create
+ * when e is NULL). */
+const char *mte_last_error(const mte_engine *e);
+
+int mte_num_beads(const mte_engine *e, int64_t *nbeads, int32_t *ntraj);
+int mte_set_state(mte_engine *e, const mte_coord *r);
+int mte_get_state(mte_engine *e, mte_coord *r);
+/* Any pointer may be NULL (left unchanged). */
+int mte_set_flags(mte_engine *e, const uint8_t *extra, const int32_t *gtp, const int32_t *on_tubule);
+int mte_set_step(mte_engine *e, int64_t step);
+int mte_get_step(const mte_engine *e, int64_t *step);
+
+/* Advance nsteps steps (global step counter continues).  Lists are rebuilt
+ * at steps divisible by LJPairsUpdateFreq, TEA's epsilon/beta at steps
+ * divisible by tea_epsilon_freq, as in the reference loop. */
+int mte_run(mte_engine *e, int64_t nsteps);
+
+/* Forces (and torques) at the current state, reference compute_kernel
+ * semantics; f: B records, w = 0.  Rebuilds the lists first if the current
+ * step is a rebuild step that has not run yet. */
+int mte_compute_forces(mte_engine *e, mte_coord *f);
+/* Per-bead energies (halved, energy_kernel convention) and/or per-trajectory
+ * FP64 sums (ntraj x 7, Energies order).  Either pointer may be NULL. */
+int mte_compute_energies(mte_engine *e, mte_energies *per_bead, double *per_traj);
+int mte_rebuild_lists(mte_engine *e);
+/* Lists in the reference encodings (local indices): lon B*8, lat B*16,
+ * lj B*max_lj.  Any pointer may be NULL. */
+int mte_get_lists(mte_engine *e, int32_t *lon_count, int32_t *lon, int32_t *lat_count, int32_t *lat,
+                  int32_t *lj_count, int32_t *lj);
+
+/* Test hook: nsteps x B x 6 floats of standard normals used instead of
+ * HybridTaus (xyz = rforce(p).xyz, angles = rforce(p + B).xyz).  NULL
+ * returns to HybridTaus. */
+int mte_set_noise(mte_engine *e, const float *noise, int64_t nsteps);
+/* HybridTaus states, 2 * B uint4 (xyz stream at p, angular at p + B). */
+int mte_get_rng(mte_engine *e, uint32_t *seeds);
+int mte_set_rng(mte_engine *e, const uint32_t *seeds);
+
+/* Per-trajectory epsilon and beta of the last TEA refresh (FP64). */
+int mte_tea_stats(mte_engine *e, double *eps, double *beta);
+
+/* Kernel timing with HIP events on the engine's stream. */
+int mte_set_timing(mte_engine *e, int32_t on);
+/* name: "step" (dominant kernel of mte_run), "rebuild", "tea", "forces".
+ * Returns summed milliseconds and launch count since the last reset. */
+int mte_kernel_time(mte_engine *e, const char *name, double *total_ms, int64_t *launches);
+int mte_reset_timing(mte_engine *e);
+/* Which mode create() chose (MTE_MODE_STREAMED or MTE_MODE_RESIDENT). */
+int mte_get_mode(const mte_engine *e, int32_t *mode);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

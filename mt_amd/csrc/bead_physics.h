@@ -1,0 +1,229 @@
+// bead_physics.h -- per-bead force and energy evaluation, shared by the
+// streamed kernels (state in HBM) and the resident kernel (state in LDS).
+// An accessor type supplies neighbour data by global bead index.
+//
+// Restates compute_kernel (src/compute_cuda.cu:32-525) and energy_kernel
+// (:676-911) in attachment-point form; see device_common.h.
+#pragma once
+#include "device_common.h"
+
+namespace mte {
+
+__device__ __forceinline__ float3 scale3(float s, float3 v) { return make_float3(s * v.x, s * v.y, s * v.z); }
+
+// Own-bead quantities needed for the angular gradients.
+struct Own {
+    float3 r, n, a1, a2, uphi;
+    float2 uth;
+    float fi, theta, psi;
+};
+
+__device__ __forceinline__ Own make_own(float4 xyz, float4 ang, float4 fn, float4 fp1, float4 fp2) {
+    Own o;
+    o.r = f3(xyz);
+    o.n = f3(fn);
+    o.a1 = f3(fp1);
+    o.a2 = f3(fp2);
+    o.fi = ang.x;
+    o.theta = ang.y;
+    o.psi = ang.z;
+    float st, ct, sp, cp;
+    sincosf(ang.y, &st, &ct);
+    sincosf(ang.z, &sp, &cp);
+    o.uphi = make_float3(cp * ct, sp * ct, -st);
+    o.uth = make_float2(-sp, cp);
+    return o;
+}
+
+// Bending torques of a harmonic or longitudinal pair (:156-177, :271-295).
+__device__ __forceinline__ void bending(const mte_params &P, float rm, float th0, const Own &I, float4 aj,
+                                        float F[6]) {
+    const float psiji = aj.z - I.psi, thetaji = aj.y - I.theta, fiji = aj.x - I.fi;
+    if (rm > 0.f) {
+        F[5] += P.B_psi * sinf(psiji - P.psi_0);
+        F[3] += P.B_fi * sinf(fiji - P.fi_0);
+        F[4] += P.B_theta * sinf(thetaji - th0);
+    } else {
+        F[5] -= P.B_psi * sinf(-psiji - P.psi_0);
+        F[3] -= P.B_fi * sinf(-fiji - P.fi_0);
+        F[4] -= P.B_theta * sinf(-thetaji - th0);
+    }
+}
+
+template <class Acc>
+__device__ __forceinline__ void bead_force(const mte_params &P, const DevTopo &T, const Acc &A, int b, const Own &I,
+                                           float F[6]) {
+#pragma unroll
+    for (int q = 0; q < 6; q++) F[q] = 0.f;
+    const int nb = T.nb;
+    // harmonic (intra-dimer spring + bending), :70-182
+    const int h = T.harm[b];
+    if (h >= 0) {
+        const int j = ent_j(h);
+        const float rm = ent_f(h) ? RMON : -RMON;
+        const float3 nj = f3(A.fn(j));
+        const Grad G = gap_grad(I.r, scale3(rm, I.n), f3(A.xyz(j)), scale3(-rm, nj), I.uphi, I.uth);
+#pragma unroll
+        for (int q = 0; q < 6; q++) F[q] -= P.C * G.g[q];
+        if (G.dr < ANGLE_CUT) {
+            const float th0 = (T.gtp[b] == 1) ? P.theta0_gtp : P.theta0_gdp;
+            bending(P, rm, th0, I, A.ang(j), F);
+        }
+    }
+    // longitudinal Morse + barrier + bending, :189-299
+    const int nlon = T.lon_n[b];
+    for (int k = 0; k < nlon; k++) {
+        const int e = T.lon[k * nb + b];
+        const int j = ent_j(e);
+        const float rm = ent_f(e) ? RMON : -RMON;
+        const float4 xj = A.xyz(j);
+        const Grad G = gap_grad(I.r, scale3(rm, I.n), f3(xj), scale3(-rm, f3(A.fn(j))), I.uphi, I.uth);
+        float dUdr = (G.dr == 0.f) ? 0.f : dmorse(P.D_long, P.A_long, G.dr) / G.dr;
+        if (P.barrier && T.on_tub[b] == 0 && T.on_tub[j] == 0 && G.dr != 0.f)
+            dUdr += dbarr(P.a_barr_long, P.r_barr_long, P.w_barr_long, G.dr) / G.dr;
+#pragma unroll
+        for (int q = 0; q < 6; q++) F[q] -= dUdr * G.g[q];
+        if (G.dr < ANGLE_CUT) {
+            const int last = (I.r.z > xj.z) ? b : j;
+            const float th0 = (T.gtp[last] == 1) ? P.theta0_gtp : P.theta0_gdp;
+            bending(P, rm, th0, I, A.ang(j), F);
+        }
+    }
+    // lateral Morse + barrier with seam weakening, :304-466
+    const int nlat = T.lat_n[b];
+    for (int k = 0; k < nlat; k++) {
+        const int e = T.lat[k * nb + b];
+        const int j = ent_j(e);
+        const int sw = ent_f(e);
+        const float3 a = sw ? I.a1 : I.a2;
+        const float3 bj = f3(sw ? A.fp2(j) : A.fp1(j));
+        const Grad G = gap_grad(I.r, a, f3(A.xyz(j)), bj, I.uphi, I.uth);
+        float dUdr;
+        if (G.dr == 0.f) dUdr = 0.f;
+        else if (T.mon_type[b] != T.mon_type[j]) dUdr = dmorse(P.D_lat / P.seam_coeff, P.A_lat, G.dr) / G.dr;
+        else dUdr = dmorse(P.D_lat, P.A_lat, G.dr) / G.dr;
+        if (P.barrier && T.on_tub[b] == 0 && T.on_tub[j] == 0 && G.dr != 0.f)
+            dUdr += dbarr(P.a_barr_lat, P.r_barr_lat, P.w_barr_lat, G.dr) / G.dr;
+#pragma unroll
+        for (int q = 0; q < 6; q++) F[q] -= dUdr * G.g[q];
+    }
+    // LJ repulsion r^-6 below the hard 6 nm cutoff, :470-495
+    if (P.lj_on) {
+        const int nlj = T.lj_n[b];
+        const float s0 = P.ljscale * P.ljsigma6;
+        for (int k = 0; k < nlj; k++) {
+            const int j = T.lj[k * nb + b];
+            const float3 rj = f3(A.xyz(j));
+            const float dx = I.r.x - rj.x, dy = I.r.y - rj.y, dz = I.r.z - rj.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (sqrtf(r2) < LJ_CUT) {
+                const float inv = 1.f / r2;
+                const float df = 6.f * (inv * inv) * (inv * inv);
+                const float s = s0 * df;
+                F[0] += s * dx;
+                F[1] += s * dy;
+                F[2] += s * dz;
+            }
+        }
+    }
+    // cylindrical and end walls, :497-517
+    if (P.is_wall) {
+        const float z = I.r.z, lb = P.rep_leftborder, zt = T.zs[T.tid[b]] + lb;
+        if (z < lb) F[2] += P.rep_eps * fabsf(z - lb);
+        else if (z > zt) F[2] += -P.rep_eps * fabsf(z - zt);
+        const float rad2 = I.r.x * I.r.x + I.r.y * I.r.y;
+        if (rad2 > P.rep_r * P.rep_r) {
+            const float rad = sqrtf(rad2);
+            const float coeff = -P.rep_eps * (rad - P.rep_r);
+            F[0] += I.r.x / rad * coeff;
+            F[1] += I.r.y / rad * coeff;
+        }
+    }
+}
+
+// Energies in the energy_kernel convention (:676-911): U_harm, U_long,
+// U_lat, U_psi, U_fi, U_teta, U_lj, each halved.
+template <class Acc>
+__device__ __forceinline__ void bead_energy(const mte_params &P, const DevTopo &T, const Acc &A, int b, const Own &I,
+                                            float U[7]) {
+#pragma unroll
+    for (int q = 0; q < 7; q++) U[q] = 0.f;
+    const int nb = T.nb;
+    const int h = T.harm[b];
+    if (h >= 0) {
+        const int j = ent_j(h);
+        const float rm = ent_f(h) ? RMON : -RMON;
+        const Grad G = gap_grad(I.r, scale3(rm, I.n), f3(A.xyz(j)), scale3(-rm, f3(A.fn(j))), I.uphi, I.uth);
+        U[0] += (P.C / 2.f) * G.dr * G.dr;
+        if (G.dr < ANGLE_CUT) {
+            const float4 aj = A.ang(j);
+            const float psiij = I.psi - aj.z, fiij = I.fi - aj.x, thetaji = aj.y - I.theta;
+            U[3] += P.B_psi * (1.f - cosf(psiij - P.psi_0));
+            U[4] += P.B_fi * (1.f - cosf(fiij - P.fi_0));
+            U[5] += P.B_theta * (1.f - cosf(thetaji - ((T.gtp[b] == 1) ? P.theta0_gtp : P.theta0_gdp)));
+        }
+    }
+    const int nlon = T.lon_n[b];
+    for (int k = 0; k < nlon; k++) {
+        const int e = T.lon[k * nb + b];
+        const int j = ent_j(e);
+        const float rm = ent_f(e) ? RMON : -RMON;
+        const float4 xj = A.xyz(j);
+        const Grad G = gap_grad(I.r, scale3(rm, I.n), f3(xj), scale3(-rm, f3(A.fn(j))), I.uphi, I.uth);
+        U[1] += morse_en(P.D_long, P.A_long, G.dr);
+        if (P.barrier && T.on_tub[b] == 0 && T.on_tub[j] == 0)
+            U[1] += barr(P.a_barr_long, P.r_barr_long, P.w_barr_long, G.dr);
+        if (G.dr < ANGLE_CUT) {
+            const float4 aj = A.ang(j);
+            const float psiij = I.psi - aj.z, fiij = I.fi - aj.x, thetaij = I.theta - aj.y;
+            const int last = (I.r.z > xj.z) ? b : j;
+            const float th0 = (T.gtp[last] == 1) ? P.theta0_gtp : P.theta0_gdp;
+            U[3] += P.B_psi * (1.f - cosf(psiij - P.psi_0));
+            U[4] += P.B_fi * (1.f - cosf(fiij - P.fi_0));
+            U[5] += P.B_theta * (1.f - cosf(thetaij - th0));
+        }
+    }
+    const int nlat = T.lat_n[b];
+    for (int k = 0; k < nlat; k++) {
+        const int e = T.lat[k * nb + b];
+        const int j = ent_j(e);
+        const int sw = ent_f(e);
+        const Grad G = gap_grad(I.r, sw ? I.a1 : I.a2, f3(A.xyz(j)), f3(sw ? A.fp2(j) : A.fp1(j)), I.uphi, I.uth);
+        if (T.mon_type[b] != T.mon_type[j]) U[2] += morse_en(P.D_lat / P.seam_coeff, P.A_lat, G.dr);
+        else U[2] += morse_en(P.D_lat, P.A_lat, G.dr);
+        if (P.barrier && T.on_tub[b] == 0 && T.on_tub[j] == 0)
+            U[2] += barr(P.a_barr_lat, P.r_barr_lat, P.w_barr_lat, G.dr);
+    }
+    if (P.lj_on) {
+        const int nlj = T.lj_n[b];
+        for (int k = 0; k < nlj; k++) {
+            const int j = T.lj[k * nb + b];
+            const float3 rj = f3(A.xyz(j));
+            const float dx = I.r.x - rj.x, dy = I.r.y - rj.y, dz = I.r.z - rj.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (sqrtf(r2) < LJ_CUT) {
+                const float inv = 1.f / r2;
+                U[6] += P.ljscale * P.ljsigma6 * (inv * inv * inv);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 7; q++) U[q] *= 0.5f;
+}
+
+// Euler-Maruyama update of one mobile bead, src/compute_cuda.cu:955-961.
+__device__ __forceinline__ void em_update(const mte_params &P, const float F[6], float4 rx, float4 ra, float4 &x,
+                                          float4 &a) {
+    const float dtg = P.dt / P.gammaR;
+    const float dta = P.dt / (P.gammaTheta * P.alpha);
+    const float dtt = P.dt / P.gammaTheta;
+    const float vta = P.varTheta * sqrtf(P.freeze_temp / P.alpha);
+    x.x += dtg * F[0] + P.varR * rx.x;
+    x.y += dtg * F[1] + P.varR * rx.y;
+    x.z += dtg * F[2] + P.varR * rx.z;
+    a.x += dta * F[3] + vta * ra.x;          // fi
+    a.z += dta * F[5] + vta * ra.y;          // psi
+    a.y += dtt * F[4] + P.varTheta * ra.z;   // theta
+}
+
+}  // namespace mte

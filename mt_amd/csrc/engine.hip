@@ -1,0 +1,618 @@
+// engine.hip -- the C ABI of include/mt_engine.h: device state, topology
+// upload, the per-step driver and host<->device exchange.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine_internal.h"
+
+using namespace mte;
+
+static thread_local std::string g_create_error;
+
+#define HIPCHK(call)                                                                           \
+    do {                                                                                       \
+        hipError_t _s = (call);                                                                \
+        if (_s != hipSuccess) {                                                                \
+            e->err = std::string(#call) + ": " + hipGetErrorString(_s);                       \
+            return MTE_ERR_HIP;                                                                \
+        }                                                                                      \
+    } while (0)
+
+template <class T>
+static hipError_t dalloc(mte_engine *e, T **p, size_t count) {
+    void *q = nullptr;
+    hipError_t s = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+    if (s == hipSuccess) {
+        e->allocations.push_back(q);
+        *p = static_cast<T *>(q);
+    }
+    return s;
+}
+
+static int fail(mte_engine *e, int code, const std::string &msg) {
+    e->err = msg;
+    return code;
+}
+
+// src/ran2.h:18-56 with state in a struct; positive seed skips the table
+// fill (shuffle table all zero, iy = 0, idum2 = 123456789).
+namespace {
+struct Ran2 {
+    int32_t idum, idum2 = 123456789, iy = 0, iv[32] = {0};
+    double next() {
+        const int32_t IM1 = 2147483563, IM2 = 2147483399, IA1 = 40014, IA2 = 40692, IQ1 = 53668, IQ2 = 52774,
+                      IR1 = 12211, IR2 = 3791, NTAB = 32, IMM1 = IM1 - 1, NDIV = 1 + IMM1 / NTAB;
+        const double AM = 1.0 / IM1, RNMX = 1.0 - 1.2e-7;
+        int32_t j, k;
+        if (idum <= 0) {
+            idum = (-idum < 1) ? 1 : -idum;
+            idum2 = idum;
+            for (j = NTAB + 7; j >= 0; j--) {
+                k = idum / IQ1;
+                idum = IA1 * (idum - k * IQ1) - k * IR1;
+                if (idum < 0) idum += IM1;
+                if (j < NTAB) iv[j] = idum;
+            }
+            iy = iv[0];
+        }
+        k = idum / IQ1;
+        idum = IA1 * (idum - k * IQ1) - k * IR1;
+        if (idum < 0) idum += IM1;
+        k = idum2 / IQ2;
+        idum2 = IA2 * (idum2 - k * IQ2) - k * IR2;
+        if (idum2 < 0) idum2 += IM2;
+        j = iy / NDIV;
+        iy = iv[j] - idum2;
+        iv[j] = idum;
+        if (iy < 1) iy += IMM1;
+        const double t = AM * iy;
+        return t > RNMX ? RNMX : t;
+    }
+};
+}  // namespace
+
+// generateSeeds, src/HybridTaus.cu:32-48.
+static void generate_seeds(int32_t seed, size_t nwords, uint32_t *out) {
+    Ran2 r;
+    r.idum = seed;
+    for (size_t i = 0; i < nwords; i++) {
+        uint32_t v;
+        do { v = (uint32_t)(r.next() * 4294967295.0); } while (v < 128);
+        out[i] = v;
+    }
+}
+
+static void time_begin(mte_engine *e, const char *name, hipEvent_t *ev) {
+    if (!e->timing) return;
+    (void)hipEventCreate(ev);
+    (void)hipEventRecord(*ev, e->stream);
+    (void)name;
+}
+
+static void time_end(mte_engine *e, const char *name, hipEvent_t start) {
+    if (!e->timing) return;
+    hipEvent_t stop;
+    (void)hipEventCreate(&stop);
+    (void)hipEventRecord(stop, e->stream);
+    e->timers[name].pending.emplace_back(start, stop);
+}
+
+static void flush_timers(mte_engine *e) {
+    for (auto &kv : e->timers) {
+        for (auto &pr : kv.second.pending) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) kv.second.total_ms += ms;
+            kv.second.launches++;
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+        kv.second.pending.clear();
+    }
+}
+
+extern "C" {
+
+const char *mte_last_error(const mte_engine *e) { return e ? e->err.c_str() : g_create_error.c_str(); }
+
+int mte_destroy(mte_engine *e) {
+    if (!e) return MTE_OK;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    flush_timers(e);
+    for (void *p : e->allocations) (void)hipFree(p);
+    if (e->noise) (void)hipFree(e->noise);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return MTE_OK;
+}
+
+int mte_create(const mte_params *params, const mte_topology *topo, const mte_options *opt, mte_engine **out) {
+    if (!out || !params || !topo || !opt) {
+        g_create_error = "mte_create: null argument";
+        return MTE_ERR_ARG;
+    }
+    *out = nullptr;
+    mte_engine *e = new mte_engine();
+    auto bail = [&](int code) {
+        g_create_error = e->err;
+        mte_destroy(e);
+        return code;
+    };
+    e->P = *params;
+    e->O = *opt;
+    if (e->O.max_lj <= 0) e->O.max_lj = 64;
+    const int nt = topo->ntraj;
+    if (nt <= 0 || !topo->traj_offsets) { e->err = "mte_create: ntraj must be positive"; return bail(MTE_ERR_ARG); }
+    e->off.assign(topo->traj_offsets, topo->traj_offsets + nt + 1);
+    if (e->off[0] != 0) { e->err = "mte_create: traj_offsets[0] must be 0"; return bail(MTE_ERR_ARG); }
+    for (int t = 0; t < nt; t++)
+        if (e->off[t + 1] <= e->off[t]) { e->err = "mte_create: empty or decreasing trajectory"; return bail(MTE_ERR_ARG); }
+    if (e->off[nt] >= (int64_t(1) << 29)) { e->err = "mte_create: too many beads"; return bail(MTE_ERR_ARG); }
+    if (e->P.ljpairsupdatefreq <= 0) { e->err = "mte_create: LJPairsUpdateFreq must be positive"; return bail(MTE_ERR_ARG); }
+    if (e->P.tea_on && e->P.tea_epsilon_freq <= 0) { e->err = "mte_create: tea_epsilon_freq must be positive"; return bail(MTE_ERR_ARG); }
+    const int nb = (int)e->off[nt];
+    e->nb = nb;
+    e->ntraj = nt;
+    e->device = opt->device;
+    if (hipSetDevice(e->device) != hipSuccess) { e->err = "mte_create: hipSetDevice failed"; return bail(MTE_ERR_HIP); }
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+        e->err = "mte_create: stream creation failed";
+        return bail(MTE_ERR_HIP);
+    }
+
+    // Lateral offsets p1, p2 (src/mt.h:44-50) in double, rounded once.
+    const double c = std::cos(2.0 * M_PI / 13.0), s = std::sin(2.0 * M_PI / 13.0);
+    e->L = LatOffsets{(float)(0.5 * 8.12 * (c - 1.0)), (float)(0.5 * 8.12 * s), (float)(-3.0 * 2.0 / 13.0),
+                      (float)(0.5 * 8.12 * (c - 1.0)), (float)(-0.5 * 8.12 * s), (float)(3.0 * 2.0 / 13.0)};
+
+    // Host-side topology translation: local reference encodings -> packed
+    // global entries, k-major list slots.
+    std::vector<int> tid(nb), tbase(nt), tsize(nt), harm(nb), harm_raw(nb), lon_n(nb, 0), lat_n(nb, 0);
+    std::vector<int> lon((size_t)MAXLON * nb, 0), lat((size_t)MAXLAT * nb, 0);
+    for (int t = 0; t < nt; t++) {
+        tbase[t] = (int)e->off[t];
+        tsize[t] = (int)(e->off[t + 1] - e->off[t]);
+        for (int i = 0; i < tsize[t]; i++) tid[tbase[t] + i] = t;
+    }
+    for (int b = 0; b < nb; b++) {
+        const int t = tid[b], n = tsize[t], base = tbase[t];
+        const int hc = topo->harmonic_count ? topo->harmonic_count[b] : 0;
+        harm_raw[b] = topo->harmonic ? topo->harmonic[b] : -1;
+        if (hc > 1) { e->err = "mte_create: at most one harmonic partner per bead"; return bail(MTE_ERR_ARG); }
+        if (hc == 1) {
+            const int h = harm_raw[b];
+            const int j = h < 0 ? -h : h;
+            if (j >= n) { e->err = "mte_create: harmonic partner out of range"; return bail(MTE_ERR_ARG); }
+            harm[b] = ((base + j) << 1) | (h < 0 ? 1 : 0);
+        } else {
+            harm[b] = -1;
+        }
+        const int nl = topo->longitudinal_count ? topo->longitudinal_count[b] : 0;
+        const int nr = topo->lateral_count ? topo->lateral_count[b] : 0;
+        if (nl < 0 || nl > MAXLON || nr < 0 || nr > MAXLAT) { e->err = "mte_create: list count out of range"; return bail(MTE_ERR_ARG); }
+        for (int k = 0; k < nl; k++) {
+            const int v = topo->longitudinal[(size_t)b * MAXLON + k];
+            const int j = v < 0 ? -v : v;
+            if (j >= n) { e->err = "mte_create: longitudinal entry out of range"; return bail(MTE_ERR_ARG); }
+            lon[(size_t)k * nb + b] = ((base + j) << 1) | (v < 0 ? 1 : 0);
+        }
+        for (int k = 0; k < nr; k++) {
+            const int v = topo->lateral[(size_t)b * MAXLAT + k];
+            int j = v <= 0 ? -v : v;
+            if (j == 999999) j = 0;
+            if (j >= n) { e->err = "mte_create: lateral entry out of range"; return bail(MTE_ERR_ARG); }
+            lat[(size_t)k * nb + b] = ((base + j) << 1) | (v <= 0 ? 1 : 0);
+        }
+        lon_n[b] = nl;
+        lat_n[b] = nr;
+    }
+    std::vector<int2> tiles;
+    for (int t = 0; t < nt; t++)
+        for (int i0 = 0; i0 < tsize[t]; i0 += 256) tiles.push_back(make_int2(t, i0));
+    e->ntiles = (int)tiles.size();
+
+    DevTopo &T = e->T;
+    T.nb = nb;
+    T.ntraj = nt;
+    T.max_lj = e->O.max_lj;
+    int *p_tid, *p_tbase, *p_tsize, *p_harm, *p_harm_raw, *p_mon, *p_gtp, *p_ont;
+    uint8_t *p_fixed, *p_extra;
+    float *p_zs;
+#define ALLOC(ptr, n)                                                          \
+    if (dalloc(e, &(ptr), (n)) != hipSuccess) {                                \
+        e->err = "mte_create: hipMalloc failed for " #ptr;                     \
+        return bail(MTE_ERR_HIP);                                              \
+    }
+    ALLOC(p_tid, nb) ALLOC(p_tbase, nt) ALLOC(p_tsize, nt) ALLOC(p_harm, nb) ALLOC(p_harm_raw, nb)
+    ALLOC(T.lon_n, nb) ALLOC(T.lon, (size_t)MAXLON * nb) ALLOC(T.lat_n, nb) ALLOC(T.lat, (size_t)MAXLAT * nb)
+    ALLOC(T.lj_n, nb) ALLOC(T.lj, (size_t)T.max_lj * nb) ALLOC(p_fixed, nb) ALLOC(p_extra, nb) ALLOC(p_mon, nb)
+    ALLOC(p_gtp, nb) ALLOC(p_ont, nb) ALLOC(p_zs, nt) ALLOC(T.overflow, 1) ALLOC(e->tiles, tiles.size())
+    for (int k = 0; k < 2; k++) {
+        ALLOC(e->S[k].xyz, nb) ALLOC(e->S[k].ang, nb) ALLOC(e->S[k].fn, nb) ALLOC(e->S[k].fp1, nb)
+        ALLOC(e->S[k].fp2, nb)
+    }
+    ALLOC(e->seeds, (size_t)2 * nb) ALLOC(e->fxyz, nb) ALLOC(e->fang, nb) ALLOC(e->en_bead, (size_t)7 * nb)
+    ALLOC(e->en_traj, (size_t)7 * nt)
+    if (e->P.tea_on) {
+        ALLOC(e->tea.ci, nb) ALLOC(e->tea.eps, nb) ALLOC(e->tea.beta, nt) ALLOC(e->tea.stats, 2 * nt)
+        ALLOC(e->tea.status, 1) ALLOC(e->tea.R, nb)
+    }
+#undef ALLOC
+    T.tid = p_tid; T.tbase = p_tbase; T.tsize = p_tsize; T.harm = p_harm; T.harm_raw = p_harm_raw;
+    T.fixed = p_fixed; T.extra = p_extra; T.mon_type = p_mon; T.gtp = p_gtp; T.on_tub = p_ont; T.zs = p_zs;
+
+    std::vector<uint8_t> fixed(nb, 0), extra(nb, 0);
+    std::vector<int> mon(nb, 0), gtp(nb, 1), ont(nb, 0);
+    std::vector<float> zs(nt, e->P.rep_h);
+    for (int b = 0; b < nb; b++) {
+        if (topo->fixed) fixed[b] = topo->fixed[b] ? 1 : 0;
+        if (topo->extra) extra[b] = topo->extra[b] ? 1 : 0;
+        if (topo->mon_type) mon[b] = topo->mon_type[b];
+        if (topo->gtp) gtp[b] = topo->gtp[b];
+        if (topo->on_tubule) ont[b] = topo->on_tubule[b];
+    }
+    if (topo->wall_z) std::copy(topo->wall_z, topo->wall_z + nt, zs.begin());
+    std::vector<uint32_t> seeds((size_t)8 * nb);
+    generate_seeds(opt->rseed, seeds.size(), seeds.data());
+
+#define UP(dst, src, n)                                                                         \
+    if (hipMemcpyAsync((void *)(dst), (src), (size_t)(n) * sizeof(*(src)), hipMemcpyHostToDevice, \
+                       e->stream) != hipSuccess) {                                              \
+        e->err = "mte_create: upload failed for " #dst;                                         \
+        return bail(MTE_ERR_HIP);                                                               \
+    }
+    UP(p_tid, tid.data(), nb) UP(p_tbase, tbase.data(), nt) UP(p_tsize, tsize.data(), nt)
+    UP(p_harm, harm.data(), nb) UP(p_harm_raw, harm_raw.data(), nb) UP(T.lon_n, lon_n.data(), nb)
+    UP(T.lon, lon.data(), lon.size()) UP(T.lat_n, lat_n.data(), nb) UP(T.lat, lat.data(), lat.size())
+    UP(p_fixed, fixed.data(), nb) UP(p_extra, extra.data(), nb) UP(p_mon, mon.data(), nb)
+    UP(p_gtp, gtp.data(), nb) UP(p_ont, ont.data(), nb) UP(p_zs, zs.data(), nt)
+    UP(e->tiles, tiles.data(), tiles.size()) UP(e->seeds, seeds.data(), seeds.size())
+#undef UP
+    if (hipMemsetAsync(T.lj_n, 0, sizeof(int) * nb, e->stream) != hipSuccess ||
+        hipMemsetAsync(T.overflow, 0, sizeof(int), e->stream) != hipSuccess) {
+        e->err = "mte_create: memset failed";
+        return bail(MTE_ERR_HIP);
+    }
+    if (e->P.tea_on && hipMemsetAsync(e->tea.status, 0, sizeof(int), e->stream) != hipSuccess) {
+        e->err = "mte_create: memset failed";
+        return bail(MTE_ERR_HIP);
+    }
+    ResidentPlan plan;
+    const bool fits = resident_fits(e, &plan);
+    if (opt->mode == MTE_MODE_RESIDENT && !fits) {
+        e->err = "mte_create: resident mode requested but the batch does not fit (TEA on, or a trajectory too large)";
+        return bail(MTE_ERR_ARG);
+    }
+    e->mode = (opt->mode == MTE_MODE_STREAMED || !fits) ? MTE_MODE_STREAMED : MTE_MODE_RESIDENT;
+    e->rplan = plan;
+    if (hipStreamSynchronize(e->stream) != hipSuccess) { e->err = "mte_create: sync failed"; return bail(MTE_ERR_HIP); }
+    *out = e;
+    return MTE_OK;
+}
+
+int mte_num_beads(const mte_engine *e, int64_t *nbeads, int32_t *ntraj) {
+    if (!e) return MTE_ERR_ARG;
+    if (nbeads) *nbeads = e->nb;
+    if (ntraj) *ntraj = e->ntraj;
+    return MTE_OK;
+}
+
+int mte_get_mode(const mte_engine *e, int32_t *mode) {
+    if (!e || !mode) return MTE_ERR_ARG;
+    *mode = e->mode;
+    return MTE_OK;
+}
+
+int mte_set_state(mte_engine *e, const mte_coord *r) {
+    if (!e || !r) return MTE_ERR_ARG;
+    HIPCHK(hipSetDevice(e->device));
+    std::vector<float4> xyz(e->nb), ang(e->nb);
+    for (int b = 0; b < e->nb; b++) {
+        xyz[b] = make_float4(r[b].x, r[b].y, r[b].z, 0.f);
+        ang[b] = make_float4(r[b].fi, r[b].theta, r[b].psi, 0.f);
+    }
+    HIPCHK(hipMemcpyAsync(e->S[e->cur].xyz, xyz.data(), sizeof(float4) * e->nb, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->S[e->cur].ang, ang.data(), sizeof(float4) * e->nb, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(launch_frames(e, e->cur));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return MTE_OK;
+}
+
+int mte_get_state(mte_engine *e, mte_coord *r) {
+    if (!e || !r) return MTE_ERR_ARG;
+    HIPCHK(hipSetDevice(e->device));
+    std::vector<float4> xyz(e->nb), ang(e->nb);
+    HIPCHK(hipMemcpyAsync(xyz.data(), e->S[e->cur].xyz, sizeof(float4) * e->nb, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(ang.data(), e->S[e->cur].ang, sizeof(float4) * e->nb, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int b = 0; b < e->nb; b++) {
+        r[b].x = xyz[b].x; r[b].y = xyz[b].y; r[b].z = xyz[b].z;
+        r[b].fi = ang[b].x; r[b].theta = ang[b].y; r[b].psi = ang[b].z; r[b].w = 0.f;
+    }
+    return MTE_OK;
+}
+
+int mte_set_flags(mte_engine *e, const uint8_t *extra, const int32_t *gtp, const int32_t *on_tubule) {
+    if (!e) return MTE_ERR_ARG;
+    HIPCHK(hipSetDevice(e->device));
+    if (extra) {
+        std::vector<uint8_t> x(extra, extra + e->nb);
+        for (auto &v : x) v = v ? 1 : 0;
+        HIPCHK(hipMemcpyAsync(e->T.extra, x.data(), e->nb, hipMemcpyHostToDevice, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+    }
+    if (gtp) HIPCHK(hipMemcpyAsync(e->T.gtp, gtp, sizeof(int) * e->nb, hipMemcpyHostToDevice, e->stream));
+    if (on_tubule) HIPCHK(hipMemcpyAsync(e->T.on_tub, on_tubule, sizeof(int) * e->nb, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return MTE_OK;
+}
+
+int mte_set_step(mte_engine *e, int64_t step) {
+    if (!e || step < 0) return MTE_ERR_ARG;
+    e->step = step;
+    return MTE_OK;
+}
+
+int mte_get_step(const mte_engine *e, int64_t *step) {
+    if (!e || !step) return MTE_ERR_ARG;
+    *step = e->step;
+    return MTE_OK;
+}
+
+static int check_overflow(mte_engine *e) {
+    int ov = 0;
+    HIPCHK(hipMemcpyAsync(&ov, e->T.overflow, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (ov) {
+        (void)hipMemsetAsync(e->T.overflow, 0, sizeof(int), e->stream);
+        return fail(e, MTE_ERR_OVERFLOW, "neighbour list capacity exceeded (longitudinal 8, lateral 16 or LJ max_lj)");
+    }
+    return MTE_OK;
+}
+
+static int rebuild_now(mte_engine *e) {
+    hipEvent_t ev{};
+    time_begin(e, "rebuild", &ev);
+    HIPCHK(launch_rebuild(e));
+    time_end(e, "rebuild", ev);
+    e->lists_step = e->step;
+    e->lists_valid = true;
+    return MTE_OK;
+}
+
+static bool rebuild_due(const mte_engine *e) {
+    return (e->step % e->P.ljpairsupdatefreq) == 0 && e->lists_step != e->step;
+}
+
+int mte_rebuild_lists(mte_engine *e) {
+    if (!e) return MTE_ERR_ARG;
+    HIPCHK(hipSetDevice(e->device));
+    int rc = rebuild_now(e);
+    if (rc) return rc;
+    return check_overflow(e);
+}
+
+static int check_tea(mte_engine *e) {
+    int st = 0;
+    HIPCHK(hipMemcpyAsync(&st, e->tea.status, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (st) {
+        (void)hipMemsetAsync(e->tea.status, 0, sizeof(int), e->stream);
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "HI tensor check failed for trajectory %d at step %lld (epsilon/beta out of range)",
+                      st - 1, (long long)e->step);
+        return fail(e, MTE_ERR_TEA, buf);
+    }
+    return MTE_OK;
+}
+
+int mte_run(mte_engine *e, int64_t nsteps) {
+    if (!e || nsteps < 0) return MTE_ERR_ARG;
+    HIPCHK(hipSetDevice(e->device));
+    if (e->noise && e->noise_cursor + nsteps > e->noise_steps)
+        return fail(e, MTE_ERR_NOISE, "injected noise exhausted");
+    int64_t done = 0;
+    const int64_t freq = e->P.ljpairsupdatefreq;
+    const bool need_lists = e->P.lj_on || e->P.is_assembly;
+    while (done < nsteps) {
+        if (e->mode == MTE_MODE_RESIDENT && !e->noise) {
+            // One launch covers every step up to the next rebuild boundary.
+            const int64_t to_boundary = freq - (e->step % freq);
+            const int64_t chunk = std::min<int64_t>(nsteps - done, to_boundary);
+            const bool rb = need_lists && rebuild_due(e);
+            hipEvent_t ev{};
+            time_begin(e, "step", &ev);
+            HIPCHK(launch_resident(e, (int)chunk, rb));
+            time_end(e, "step", ev);
+            if (rb) { e->lists_step = e->step; e->lists_valid = true; }
+            e->step += chunk;
+            done += chunk;
+            continue;
+        }
+        if (need_lists && rebuild_due(e)) {
+            int rc = rebuild_now(e);
+            if (rc) return rc;
+        }
+        const float *nz = e->noise ? e->noise + (size_t)6 * e->nb * e->noise_cursor : nullptr;
+        if (e->P.tea_on) {
+            hipEvent_t ev{};
+            time_begin(e, "forces", &ev);
+            HIPCHK(launch_forces(e));
+            time_end(e, "forces", ev);
+            if (e->step % e->P.tea_epsilon_freq == 0) {
+                HIPCHK(launch_tea_eps(e));
+                int rc = check_tea(e);
+                if (rc) return rc;
+                e->tea_valid = true;
+            }
+            time_begin(e, "tea", &ev);
+            HIPCHK(launch_tea_step(e, nz));
+            time_end(e, "tea", ev);
+        } else {
+            hipEvent_t ev{};
+            time_begin(e, "step", &ev);
+            HIPCHK(launch_step_plain(e, nz));
+            time_end(e, "step", ev);
+        }
+        if (e->noise) e->noise_cursor++;
+        e->step++;
+        done++;
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    flush_timers(e);
+    if (need_lists) return check_overflow(e);
+    return MTE_OK;
+}
+
+int mte_compute_forces(mte_engine *e, mte_coord *f) {
+    if (!e || !f) return MTE_ERR_ARG;
+    HIPCHK(hipSetDevice(e->device));
+    if ((e->P.lj_on || e->P.is_assembly) && rebuild_due(e)) {
+        int rc = rebuild_now(e);
+        if (rc) return rc;
+    }
+    HIPCHK(launch_forces(e));
+    std::vector<float4> fx(e->nb), fa(e->nb);
+    HIPCHK(hipMemcpyAsync(fx.data(), e->fxyz, sizeof(float4) * e->nb, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(fa.data(), e->fang, sizeof(float4) * e->nb, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int b = 0; b < e->nb; b++) {
+        f[b].x = fx[b].x; f[b].y = fx[b].y; f[b].z = fx[b].z;
+        f[b].fi = fa[b].x; f[b].theta = fa[b].y; f[b].psi = fa[b].z; f[b].w = 0.f;
+    }
+    return check_overflow(e);
+}
+
+int mte_compute_energies(mte_engine *e, mte_energies *per_bead, double *per_traj) {
+    if (!e) return MTE_ERR_ARG;
+    HIPCHK(hipSetDevice(e->device));
+    if ((e->P.lj_on || e->P.is_assembly) && rebuild_due(e)) {
+        int rc = rebuild_now(e);
+        if (rc) return rc;
+    }
+    HIPCHK(launch_energies(e));
+    if (per_bead)
+        HIPCHK(hipMemcpyAsync(per_bead, e->en_bead, sizeof(double) * 7 * e->nb, hipMemcpyDeviceToHost, e->stream));
+    if (per_traj)
+        HIPCHK(hipMemcpyAsync(per_traj, e->en_traj, sizeof(double) * 7 * e->ntraj, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return check_overflow(e);
+}
+
+int mte_get_lists(mte_engine *e, int32_t *lon_count, int32_t *lon, int32_t *lat_count, int32_t *lat,
+                  int32_t *lj_count, int32_t *lj) {
+    if (!e) return MTE_ERR_ARG;
+    HIPCHK(hipSetDevice(e->device));
+    const int nb = e->nb, ml = e->T.max_lj;
+    std::vector<int> ln(nb), l((size_t)MAXLON * nb), rn(nb), r((size_t)MAXLAT * nb), jn(nb), j((size_t)ml * nb);
+    HIPCHK(hipMemcpyAsync(ln.data(), e->T.lon_n, sizeof(int) * nb, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(l.data(), e->T.lon, sizeof(int) * l.size(), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(rn.data(), e->T.lat_n, sizeof(int) * nb, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(r.data(), e->T.lat, sizeof(int) * r.size(), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(jn.data(), e->T.lj_n, sizeof(int) * nb, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(j.data(), e->T.lj, sizeof(int) * j.size(), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    std::vector<int> tbase(nb);
+    for (int t = 0; t < e->ntraj; t++)
+        for (int64_t b = e->off[t]; b < e->off[t + 1]; b++) tbase[b] = (int)e->off[t];
+    for (int b = 0; b < nb; b++) {
+        if (lon_count) lon_count[b] = ln[b];
+        if (lat_count) lat_count[b] = rn[b];
+        if (lj_count) lj_count[b] = jn[b];
+        if (lon)
+            for (int k = 0; k < MAXLON; k++) {
+                const int v = l[(size_t)k * nb + b];
+                const int jj = (v >> 1) - tbase[b];
+                lon[(size_t)b * MAXLON + k] = k < ln[b] ? ((v & 1) ? -jj : jj) : 0;
+            }
+        if (lat)
+            for (int k = 0; k < MAXLAT; k++) {
+                const int v = r[(size_t)k * nb + b];
+                const int jj = (v >> 1) - tbase[b];
+                int enc = (v & 1) ? (jj != 0 ? -jj : -999999) : (jj != 0 ? jj : 999999);
+                lat[(size_t)b * MAXLAT + k] = k < rn[b] ? enc : 0;
+            }
+        if (lj)
+            for (int k = 0; k < ml; k++) lj[(size_t)b * ml + k] = k < jn[b] ? j[(size_t)k * nb + b] - tbase[b] : 0;
+    }
+    return MTE_OK;
+}
+
+int mte_set_noise(mte_engine *e, const float *noise, int64_t nsteps) {
+    if (!e) return MTE_ERR_ARG;
+    HIPCHK(hipSetDevice(e->device));
+    if (e->noise) {
+        HIPCHK(hipStreamSynchronize(e->stream));
+        HIPCHK(hipFree(e->noise));
+        e->noise = nullptr;
+    }
+    e->noise_steps = 0;
+    e->noise_cursor = 0;
+    if (!noise || nsteps <= 0) return MTE_OK;
+    const size_t bytes = sizeof(float) * 6 * (size_t)e->nb * nsteps;
+    HIPCHK(hipMalloc(&e->noise, bytes));
+    HIPCHK(hipMemcpyAsync(e->noise, noise, bytes, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->noise_steps = nsteps;
+    return MTE_OK;
+}
+
+int mte_get_rng(mte_engine *e, uint32_t *seeds) {
+    if (!e || !seeds) return MTE_ERR_ARG;
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipMemcpyAsync(seeds, e->seeds, sizeof(uint4) * 2 * e->nb, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return MTE_OK;
+}
+
+int mte_set_rng(mte_engine *e, const uint32_t *seeds) {
+    if (!e || !seeds) return MTE_ERR_ARG;
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipMemcpyAsync(e->seeds, seeds, sizeof(uint4) * 2 * e->nb, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return MTE_OK;
+}
+
+int mte_tea_stats(mte_engine *e, double *eps, double *beta) {
+    if (!e) return MTE_ERR_ARG;
+    if (!e->P.tea_on || !e->tea_valid) return fail(e, MTE_ERR_STATE, "no TEA refresh has run");
+    HIPCHK(hipSetDevice(e->device));
+    std::vector<double> st(2 * e->ntraj);
+    HIPCHK(hipMemcpyAsync(st.data(), e->tea.stats, sizeof(double) * st.size(), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int t = 0; t < e->ntraj; t++) {
+        if (eps) eps[t] = st[2 * t];
+        if (beta) beta[t] = st[2 * t + 1];
+    }
+    return MTE_OK;
+}
+
+int mte_set_timing(mte_engine *e, int32_t on) {
+    if (!e) return MTE_ERR_ARG;
+    e->timing = on != 0;
+    return MTE_OK;
+}
+
+int mte_reset_timing(mte_engine *e) {
+    if (!e) return MTE_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    flush_timers(e);
+    e->timers.clear();
+    return MTE_OK;
+}
+
+int mte_kernel_time(mte_engine *e, const char *name, double *total_ms, int64_t *launches) {
+    if (!e || !name) return MTE_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    flush_timers(e);
+    auto it = e->timers.find(name);
+    if (total_ms) *total_ms = it == e->timers.end() ? 0.0 : it->second.total_ms;
+    if (launches) *launches = it == e->timers.end() ? 0 : it->second.launches;
+    return MTE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+// engine_internal.h -- the engine object and the kernel launchers that the
+// C ABI (engine.hip) drives.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "device_common.h"
+
+namespace mte {
+
+struct TimingSlot {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    double total_ms = 0.0;
+    int64_t launches = 0;
+};
+
+struct TeaBuffers {
+    float4 *ci = nullptr;      // per bead: row sums of squares of D_hat (x, y, z)
+    float *eps = nullptr;      // per bead: sum of all D_hat entries
+    float *beta = nullptr;     // per trajectory
+    double *stats = nullptr;   // per trajectory: eps, beta (FP64)
+    int *status = nullptr;     // first capricious failure (1 + traj) or 0
+    float4 *R = nullptr;       // per bead: scaled random force
+};
+
+// Resident-mode launch plan: trajectories grouped by workgroup.
+struct ResidentPlan {
+    int threads = 0;        // workgroup size
+    int per_thread = 0;     // beads per thread
+    int max_beads = 0;      // largest trajectory
+    size_t lds_bytes = 0;
+};
+
+}  // namespace mte
+
+struct mte_engine {
+    mte_params P{};
+    mte_options O{};
+    int nb = 0, ntraj = 0;
+    std::vector<int64_t> off;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int mode = MTE_MODE_STREAMED;
+
+    mte::DevTopo T{};
+    mte::DevState S[2]{};
+    int cur = 0;
+    mte::LatOffsets L{};
+
+    uint4 *seeds = nullptr;
+    float *noise = nullptr;
+    int64_t noise_steps = 0, noise_cursor = 0;
+    float4 *fxyz = nullptr, *fang = nullptr;
+    double *en_bead = nullptr, *en_traj = nullptr;
+    int2 *tiles = nullptr;
+    int ntiles = 0;
+    int *d_overflow = nullptr;
+    mte::TeaBuffers tea;
+    mte::ResidentPlan rplan;
+
+    int64_t step = 0;
+    int64_t lists_step = -1;
+    bool lists_valid = false;
+    bool tea_valid = false;
+
+    bool timing = false;
+    std::map<std::string, mte::TimingSlot> timers;
+    std::vector<void *> allocations;
+    std::string err;
+};
+
+namespace mte {
+
+// Launchers (kernels live in forces.hip, lists.hip, tea.hip, resident.hip).
+hipError_t launch_frames(mte_engine *e, int buf);
+hipError_t launch_step_plain(mte_engine *e, const float *noise_step);
+hipError_t launch_forces(mte_engine *e);
+hipError_t launch_energies(mte_engine *e);
+hipError_t launch_rebuild(mte_engine *e);
+hipError_t launch_tea_eps(mte_engine *e);
+hipError_t launch_tea_step(mte_engine *e, const float *noise_step);
+hipError_t launch_resident(mte_engine *e, int nsteps, bool rebuild_first);
+bool resident_fits(const mte_engine *e, ResidentPlan *plan);
+
+}  // namespace mte

@@ -1,0 +1,149 @@
+// forces.hip -- streamed-mode kernels: one thread per bead, bead state in
+// HBM (float4 SoA, double-buffered), neighbour lists k-major so that a
+// wavefront's list loads coalesce.
+#include "bead_physics.h"
+#include "engine_internal.h"
+
+namespace mte {
+
+constexpr int STREAM_BLOCK = 256;
+
+struct GlobalAcc {
+    DevState S;
+    __device__ __forceinline__ float4 xyz(int j) const { return S.xyz[j]; }
+    __device__ __forceinline__ float4 ang(int j) const { return S.ang[j]; }
+    __device__ __forceinline__ float4 fn(int j) const { return S.fn[j]; }
+    __device__ __forceinline__ float4 fp1(int j) const { return S.fp1[j]; }
+    __device__ __forceinline__ float4 fp2(int j) const { return S.fp2[j]; }
+};
+
+__global__ void __launch_bounds__(STREAM_BLOCK) k_frames(DevState S, int nb, LatOffsets L) {
+    const int b = blockIdx.x * STREAM_BLOCK + threadIdx.x;
+    if (b >= nb) return;
+    const float4 a = S.ang[b];
+    float4 n, p1, p2;
+    frame_vectors(a.x, a.y, a.z, L, n, p1, p2);
+    S.fn[b] = n;
+    S.fp1[b] = p1;
+    S.fp2[b] = p2;
+}
+
+// Force + Euler-Maruyama step.  Reads Sin, writes Sout (both positions and
+// rotated offsets), advances the HybridTaus states of mobile beads only
+// (src/compute_cuda.cu:949-953) unless noise is injected.
+__global__ void __launch_bounds__(STREAM_BLOCK)
+    k_step_plain(mte_params P, DevTopo T, DevState Sin, DevState Sout, uint4 *seeds, const float *noise,
+                 LatOffsets L) {
+    const int b = blockIdx.x * STREAM_BLOCK + threadIdx.x;
+    if (b >= T.nb) return;
+    float4 x = Sin.xyz[b], a = Sin.ang[b];
+    const bool extra = T.extra[b];
+    const bool mobile = !T.fixed[b] && !extra;
+    if (!mobile) {
+        Sout.xyz[b] = x;
+        Sout.ang[b] = a;
+        Sout.fn[b] = Sin.fn[b];
+        Sout.fp1[b] = Sin.fp1[b];
+        Sout.fp2[b] = Sin.fp2[b];
+        return;
+    }
+    const Own I = make_own(x, a, Sin.fn[b], Sin.fp1[b], Sin.fp2[b]);
+    float F[6];
+    bead_force(P, T, GlobalAcc{Sin}, b, I, F);
+    float4 rx, ra;
+    if (noise) {
+        const float *nz = noise + 6 * (size_t)b;
+        rx = make_float4(nz[0], nz[1], nz[2], 0.f);
+        ra = make_float4(nz[3], nz[4], nz[5], 0.f);
+    } else {
+        uint4 s = seeds[b];
+        rx = normals4(s);
+        seeds[b] = s;
+        s = seeds[b + T.nb];
+        ra = normals4(s);
+        seeds[b + T.nb] = s;
+    }
+    em_update(P, F, rx, ra, x, a);
+    float4 n, p1, p2;
+    frame_vectors(a.x, a.y, a.z, L, n, p1, p2);
+    Sout.xyz[b] = x;
+    Sout.ang[b] = a;
+    Sout.fn[b] = n;
+    Sout.fp1[b] = p1;
+    Sout.fp2[b] = p2;
+}
+
+// Forces only (compute_kernel semantics; extra beads get zero).
+__global__ void __launch_bounds__(STREAM_BLOCK)
+    k_forces(mte_params P, DevTopo T, DevState S, float4 *fxyz, float4 *fang) {
+    const int b = blockIdx.x * STREAM_BLOCK + threadIdx.x;
+    if (b >= T.nb) return;
+    float F[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (!T.extra[b]) {
+        const Own I = make_own(S.xyz[b], S.ang[b], S.fn[b], S.fp1[b], S.fp2[b]);
+        bead_force(P, T, GlobalAcc{S}, b, I, F);
+    }
+    fxyz[b] = make_float4(F[0], F[1], F[2], 0.f);
+    fang[b] = make_float4(F[3], F[4], F[5], 0.f);   // fi, theta, psi
+}
+
+__global__ void __launch_bounds__(STREAM_BLOCK)
+    k_energies(mte_params P, DevTopo T, DevState S, double *en) {
+    const int b = blockIdx.x * STREAM_BLOCK + threadIdx.x;
+    if (b >= T.nb) return;
+    float U[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (!T.extra[b]) {
+        const Own I = make_own(S.xyz[b], S.ang[b], S.fn[b], S.fp1[b], S.fp2[b]);
+        bead_energy(P, T, GlobalAcc{S}, b, I, U);
+    }
+#pragma unroll
+    for (int q = 0; q < 7; q++) en[7 * (size_t)b + q] = (double)U[q];
+}
+
+// Per-trajectory FP64 sums of the per-bead energies (OutputAllEnergies,
+// src/updater.cpp:3-43), one workgroup per trajectory, fixed reduction order.
+__global__ void __launch_bounds__(256) k_traj_sum(DevTopo T, const double *en, double *out) {
+    __shared__ double red[7][256];
+    const int t = blockIdx.x;
+    const int base = T.tbase[t], n = T.tsize[t];
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < n; i += 256)
+        for (int q = 0; q < 7; q++) acc[q] += en[7 * (size_t)(base + i) + q];
+    for (int q = 0; q < 7; q++) red[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int q = 0; q < 7; q++) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 7) out[7 * t + threadIdx.x] = red[threadIdx.x][0];
+}
+
+static inline int grid_for(int n) { return (n + STREAM_BLOCK - 1) / STREAM_BLOCK; }
+
+hipError_t launch_frames(mte_engine *e, int buf) {
+    hipLaunchKernelGGL(k_frames, dim3(grid_for(e->nb)), dim3(STREAM_BLOCK), 0, e->stream, e->S[buf], e->nb, e->L);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_plain(mte_engine *e, const float *noise_step) {
+    hipLaunchKernelGGL(k_step_plain, dim3(grid_for(e->nb)), dim3(STREAM_BLOCK), 0, e->stream, e->P, e->T,
+                       e->S[e->cur], e->S[1 - e->cur], e->seeds, noise_step, e->L);
+    e->cur = 1 - e->cur;
+    return hipGetLastError();
+}
+
+hipError_t launch_forces(mte_engine *e) {
+    hipLaunchKernelGGL(k_forces, dim3(grid_for(e->nb)), dim3(STREAM_BLOCK), 0, e->stream, e->P, e->T, e->S[e->cur],
+                       e->fxyz, e->fang);
+    return hipGetLastError();
+}
+
+hipError_t launch_energies(mte_engine *e) {
+    hipLaunchKernelGGL(k_energies, dim3(grid_for(e->nb)), dim3(STREAM_BLOCK), 0, e->stream, e->P, e->T,
+                       e->S[e->cur], e->en_bead);
+    hipLaunchKernelGGL(k_traj_sum, dim3(e->ntraj), dim3(256), 0, e->stream, e->T, e->en_bead, e->en_traj);
+    return hipGetLastError();
+}
+
+}  // namespace mte

@@ -1,0 +1,233 @@
+"""Python front end of libmt_engine.so (include/mt_engine.h) via ctypes.
+
+The engine is the product path: it fails loudly if the HIP library is
+missing or cannot be loaded; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .params import Params
+from .topology import Batch, MAXLAT, MAXLON
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmt_engine.so")
+
+MTE_OK = 0
+MODE_AUTO, MODE_STREAMED, MODE_RESIDENT = 0, 1, 2
+ERRORS = {1: "MTE_ERR_ARG", 2: "MTE_ERR_HIP", 3: "MTE_ERR_OVERFLOW", 4: "MTE_ERR_TEA", 5: "MTE_ERR_STATE",
+          6: "MTE_ERR_NOISE"}
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Topology(ctypes.Structure):
+    _fields_ = [("ntraj", ctypes.c_int32), ("traj_offsets", ctypes.c_void_p),
+                ("harmonic_count", ctypes.c_void_p), ("harmonic", ctypes.c_void_p),
+                ("longitudinal_count", ctypes.c_void_p), ("longitudinal", ctypes.c_void_p),
+                ("lateral_count", ctypes.c_void_p), ("lateral", ctypes.c_void_p),
+                ("fixed", ctypes.c_void_p), ("extra", ctypes.c_void_p),
+                ("mon_type", ctypes.c_void_p), ("gtp", ctypes.c_void_p), ("on_tubule", ctypes.c_void_p),
+                ("wall_z", ctypes.c_void_p)]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("max_lj", ctypes.c_int32), ("mode", ctypes.c_int32), ("rseed", ctypes.c_int32),
+                ("device", ctypes.c_int32)]
+
+
+# Every symbol include/mt_engine.h declares, with its ctypes signature.
+_P = ctypes.c_void_p
+SIGNATURES = {
+    "mte_create": ([_P, _P, _P, _P], ctypes.c_int),
+    "mte_destroy": ([_P], ctypes.c_int),
+    "mte_last_error": ([_P], ctypes.c_char_p),
+    "mte_num_beads": ([_P, _P, _P], ctypes.c_int),
+    "mte_set_state": ([_P, _P], ctypes.c_int),
+    "mte_get_state": ([_P, _P], ctypes.c_int),
+    "mte_set_flags": ([_P, _P, _P, _P], ctypes.c_int),
+    "mte_set_step": ([_P, ctypes.c_int64], ctypes.c_int),
+    "mte_get_step": ([_P, _P], ctypes.c_int),
+    "mte_run": ([_P, ctypes.c_int64], ctypes.c_int),
+    "mte_compute_forces": ([_P, _P], ctypes.c_int),
+    "mte_compute_energies": ([_P, _P, _P], ctypes.c_int),
+    "mte_rebuild_lists": ([_P], ctypes.c_int),
+    "mte_get_lists": ([_P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "mte_set_noise": ([_P, _P, ctypes.c_int64], ctypes.c_int),
+    "mte_get_rng": ([_P, _P], ctypes.c_int),
+    "mte_set_rng": ([_P, _P], ctypes.c_int),
+    "mte_tea_stats": ([_P, _P, _P], ctypes.c_int),
+    "mte_set_timing": ([_P, ctypes.c_int32], ctypes.c_int),
+    "mte_kernel_time": ([_P, ctypes.c_char_p, _P, _P], ctypes.c_int),
+    "mte_reset_timing": ([_P], ctypes.c_int),
+    "mte_get_mode": ([_P, _P], ctypes.c_int),
+}
+
+_LIB = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libmt_engine.so; raises if it is absent (no fallback)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python mt_amd/build.py` (hipcc, gfx950)")
+    lib = ctypes.CDLL(path)
+    for name, (args, res) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _LIB = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Engine:
+    """One engine instance over a CSR batch (see include/mt_engine.h)."""
+
+    def __init__(self, params: Params, batch: Batch, max_lj: int = 64, mode: int = MODE_AUTO, rseed: int = 1234567,
+                 device: int = 0):
+        self.lib = load_library()
+        self.params = params
+        self.nb = batch.nb
+        self.ntraj = batch.ntraj
+        self.max_lj = max_lj
+        keep = {}
+
+        def arr(name, dt):
+            a = np.ascontiguousarray(getattr(batch, name), dt)
+            keep[name] = a
+            return _p(a)
+
+        topo = Topology(batch.ntraj, arr("off", np.int64), arr("harm_count", np.int32), arr("harm", np.int32),
+                        arr("lon_count", np.int32), arr("lon", np.int32), arr("lat_count", np.int32),
+                        arr("lat", np.int32), arr("fixed", np.uint8), arr("extra", np.uint8),
+                        arr("mon_type", np.int32), arr("gtp", np.int32), arr("on_tub", np.int32),
+                        arr("zs", np.float32))
+        opt = Options(max_lj, mode, rseed, device)
+        h = ctypes.c_void_p()
+        rc = self.lib.mte_create(ctypes.byref(params), ctypes.byref(topo), ctypes.byref(opt), ctypes.byref(h))
+        if rc != MTE_OK:
+            raise EngineError(rc, self.lib.mte_last_error(None).decode())
+        self.h = h
+        self.set_state(batch.coords)
+
+    def _chk(self, rc):
+        if rc != MTE_OK:
+            raise EngineError(rc, self.lib.mte_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mte_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def mode(self) -> int:
+        m = ctypes.c_int32()
+        self._chk(self.lib.mte_get_mode(self.h, ctypes.byref(m)))
+        return m.value
+
+    def set_state(self, coords):
+        c = np.ascontiguousarray(coords, np.float32)
+        assert c.shape == (self.nb, 7)
+        self._chk(self.lib.mte_set_state(self.h, _p(c)))
+
+    def get_state(self):
+        c = np.zeros((self.nb, 7), np.float32)
+        self._chk(self.lib.mte_get_state(self.h, _p(c)))
+        return c
+
+    def set_flags(self, extra=None, gtp=None, on_tubule=None):
+        e = None if extra is None else np.ascontiguousarray(extra, np.uint8)
+        g = None if gtp is None else np.ascontiguousarray(gtp, np.int32)
+        o = None if on_tubule is None else np.ascontiguousarray(on_tubule, np.int32)
+        self._chk(self.lib.mte_set_flags(self.h, _p(e), _p(g), _p(o)))
+
+    @property
+    def step(self) -> int:
+        s = ctypes.c_int64()
+        self._chk(self.lib.mte_get_step(self.h, ctypes.byref(s)))
+        return s.value
+
+    @step.setter
+    def step(self, v: int):
+        self._chk(self.lib.mte_set_step(self.h, int(v)))
+
+    def run(self, nsteps: int):
+        self._chk(self.lib.mte_run(self.h, int(nsteps)))
+
+    def forces(self):
+        f = np.zeros((self.nb, 7), np.float32)
+        self._chk(self.lib.mte_compute_forces(self.h, _p(f)))
+        return f
+
+    def energies(self):
+        per_bead = np.zeros((self.nb, 7), np.float64)
+        per_traj = np.zeros((self.ntraj, 7), np.float64)
+        self._chk(self.lib.mte_compute_energies(self.h, _p(per_bead), _p(per_traj)))
+        return per_bead, per_traj
+
+    def rebuild(self):
+        self._chk(self.lib.mte_rebuild_lists(self.h))
+
+    def lists(self):
+        nb, ml = self.nb, self.max_lj
+        out = dict(lon_count=np.zeros(nb, np.int32), lon=np.zeros((nb, MAXLON), np.int32),
+                   lat_count=np.zeros(nb, np.int32), lat=np.zeros((nb, MAXLAT), np.int32),
+                   lj_count=np.zeros(nb, np.int32), lj=np.zeros((nb, ml), np.int32))
+        self._chk(self.lib.mte_get_lists(self.h, _p(out["lon_count"]), _p(out["lon"]), _p(out["lat_count"]),
+                                         _p(out["lat"]), _p(out["lj_count"]), _p(out["lj"])))
+        return out
+
+    def set_noise(self, noise):
+        if noise is None:
+            self._chk(self.lib.mte_set_noise(self.h, None, 0))
+            return
+        n = np.ascontiguousarray(noise, np.float32)
+        assert n.size % (6 * self.nb) == 0
+        self._chk(self.lib.mte_set_noise(self.h, _p(n), n.size // (6 * self.nb)))
+
+    def get_rng(self):
+        s = np.zeros(8 * self.nb, np.uint32)
+        self._chk(self.lib.mte_get_rng(self.h, _p(s)))
+        return s
+
+    def set_rng(self, seeds):
+        s = np.ascontiguousarray(seeds, np.uint32)
+        assert s.size == 8 * self.nb
+        self._chk(self.lib.mte_set_rng(self.h, _p(s)))
+
+    def tea_stats(self):
+        eps = np.zeros(self.ntraj)
+        beta = np.zeros(self.ntraj)
+        self._chk(self.lib.mte_tea_stats(self.h, _p(eps), _p(beta)))
+        return eps, beta
+
+    def set_timing(self, on: bool):
+        self._chk(self.lib.mte_set_timing(self.h, int(on)))
+
+    def reset_timing(self):
+        self._chk(self.lib.mte_reset_timing(self.h))
+
+    def kernel_time(self, name: str):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self._chk(self.lib.mte_kernel_time(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

@@ -1,0 +1,48 @@
+"""Shared fixtures-as-functions for the parity tests."""
+from __future__ import annotations
+
+import numpy as np
+
+from mt_amd import lattice, params, topology
+
+XYZ = [0, 1, 2]
+ANG = [3, 4, 5]
+
+
+def rel_err(a, b, cols):
+    """max |a - b| over the columns, relative to max |b| over the same columns."""
+    a = np.asarray(a, np.float64)[:, cols]
+    b = np.asarray(b, np.float64)[:, cols]
+    scale = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / max(scale, 1e-30))
+
+
+def c1_batch(ntraj=1, jitter=0.05, seed=1234567, n_dimers=12, cfg=None):
+    """Config-1 style: 13 PF x n_dimers dimer microtubule(s), jittered."""
+    at = lattice.microtubule(n_dimers)
+    return topology.replicate(at, ntraj, jitter, seed)
+
+
+def uneven_batch(seed=7, jitter=0.05, lengths=(12, 5, 20), n_free=(0, 6, 3)):
+    """Trajectories of different lengths; some carry free dimers placed
+    around the tube (not periodic)."""
+    rng = np.random.default_rng(seed)
+    atoms, coords = [], []
+    for nd, nf in zip(lengths, n_free):
+        a = lattice.microtubule(nd)
+        if nf:
+            box = 4 * nd + 40.0
+            fd = lattice.free_dimers(nf, box, rng, first_id=a.n + 1, avoid=a.xyz)
+            fd.xyz[:, 0:2] += 30.0  # keep them off the lattice
+            a = lattice.concat(a, fd)
+        atoms.append(a)
+        coords.append(lattice.coords7(a, jitter, rng))
+    return topology.make_batch(atoms, coords)
+
+
+def base_params(**over):
+    return params.derive(dict(params.BASE, **over))
+
+
+def noise(rng, nsteps, nb):
+    return rng.standard_normal((nsteps, nb, 6)).astype(np.float32)

@@ -1,0 +1,118 @@
+"""GPU parity: the HIP engine against the CPU oracle on identical inputs.
+
+Tolerances (FP32 engine vs FP64 oracle): forces and energies 1e-5 relative
+to the largest component of the same kind; lists bit-exact; trajectories with
+injected identical noise compared against the FP32 oracle.
+"""
+import numpy as np
+import pytest
+
+from mt_amd.engine import Engine, MODE_STREAMED
+from oracle.oracle import Oracle
+from tests.helpers import ANG, XYZ, base_params, c1_batch, noise, rel_err, uneven_batch
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(p, b, mode=MODE_STREAMED, precision="f64", max_lj=64):
+    eng = Engine(p, b.copy(), max_lj=max_lj, mode=mode)
+    orc = Oracle(p, b.copy(), max_lj=max_lj, precision=precision)
+    return eng, orc
+
+
+def _lists_equal(eng, orc):
+    L = eng.lists()
+    b = orc.b
+    np.testing.assert_array_equal(L["lon_count"], b.lon_count)
+    np.testing.assert_array_equal(L["lat_count"], b.lat_count)
+    np.testing.assert_array_equal(L["lj_count"], orc.lj_count)
+    for k in range(8):
+        m = b.lon_count > k
+        np.testing.assert_array_equal(L["lon"][m, k], b.lon[m, k])
+    for k in range(16):
+        m = b.lat_count > k
+        np.testing.assert_array_equal(L["lat"][m, k], b.lat[m, k])
+    for k in range(orc.max_lj):
+        m = orc.lj_count > k
+        np.testing.assert_array_equal(L["lj"][m, k], orc.lj[m, k])
+
+
+@pytest.mark.parametrize("maker", [c1_batch, uneven_batch])
+def test_lists_forces_energies(maker):
+    print(f"case lists/forces/energies {maker.__name__}", flush=True)
+    p = base_params()
+    b = maker()
+    eng, orc = _pair(p, b)
+    eng.rebuild()
+    assert orc.rebuild() == 0
+    _lists_equal(eng, orc)
+    f = eng.forces()
+    fo = orc.forces()
+    assert rel_err(f, fo, XYZ) < 1e-5
+    assert rel_err(f, fo, ANG) < 1e-5
+    eb, et = eng.energies()
+    eo = orc.energies()
+    for q in range(7):
+        s = np.abs(eo[:, q]).max()
+        if s > 0:
+            assert np.abs(eb[:, q] - eo[:, q]).max() / s < 1e-5, q
+    eng.close()
+
+
+def test_trajectory_injected_noise():
+    print("case trajectory 1000 steps injected noise", flush=True)
+    p = base_params()
+    b = c1_batch()
+    eng, orc = _pair(p, b, precision="f32")
+    rng = np.random.default_rng(11)
+    nsteps = 1000
+    nz = noise(rng, nsteps, b.nb)
+    eng.set_noise(nz)
+    eng.run(nsteps)
+    assert orc.run(0, nsteps, noise=nz) == 0
+    g = eng.get_state()
+    o = orc.b.coords
+    dx = np.abs(g[:, :3] - o[:, :3]).max()
+    da = np.abs(g[:, 3:6] - o[:, 3:6]).max()
+    print("max |dx| nm", dx, "max |dang| rad", da, flush=True)
+    assert dx < 1e-3 and da < 1e-3
+    eng.close()
+
+
+def test_rng_streams_bit_exact():
+    print("case HybridTaus streams", flush=True)
+    from oracle.oracle import generate_seeds
+    p = base_params()
+    b = c1_batch()
+    eng, orc = _pair(p, b, precision="f32")
+    s0 = eng.get_rng()
+    np.testing.assert_array_equal(s0, generate_seeds(1234567, 2 * b.nb))
+    seeds = s0.copy()
+    eng.run(20)
+    assert orc.run(0, 20, seeds=seeds) == 0
+    np.testing.assert_array_equal(eng.get_rng(), seeds)
+    g = eng.get_state()
+    assert np.abs(g[:, :3] - orc.b.coords[:, :3]).max() < 1e-3
+    eng.close()
+
+
+def test_tea_step_parity():
+    print("case TEA eps/beta + steps", flush=True)
+    from mt_amd import params as P
+    p = P.derive(dict(P.TEA, tea_epsilon_freq=5))
+    b = c1_batch()
+    eng, orc = _pair(p, b, precision="f64")
+    rng = np.random.default_rng(5)
+    nsteps = 10
+    nz = noise(rng, nsteps, b.nb)
+    eng.set_noise(nz)
+    eng.run(nsteps)
+    assert orc.run(0, nsteps, noise=nz) == 0
+    eps, beta = eng.tea_stats()
+    assert abs(eps[0] - orc.eps_t[0]) / orc.eps_t[0] < 1e-5
+    assert abs(beta[0] - orc.beta_t[0]) / orc.beta_t[0] < 1e-5
+    g = eng.get_state()
+    dx = np.abs(g[:, :3] - orc.b.coords[:, :3]).max()
+    print("TEA max |dx|", dx, flush=True)
+    assert dx < 1e-4
+    eng.close()
