@@ -1,0 +1,150 @@
+"""Throughput of the hot path on one MI355X: batched plain Langevin dynamics.
+
+Workload (batched plain Langevin, FP32, no HI): 256 independent trajectories
+of a 13-protofilament x 40-dimer microtubule (1040 beads each, 266,240 beads
+in total), per-trajectory HybridTaus streams, bottom ring fixed, LJ pair list
+cutoff 8 nm rebuilt every 100 steps, synthetic lattice jittered N(0, 0.05 nm).
+One step = forces + Euler-Maruyama for every bead of the batch (and the list
+rebuild every 100th step).  value = monomer-steps/s (beads x steps / s).
+
+Prints one JSON line.  Also reports the dominant kernel's HBM roofline
+fraction (algorithmic bytes per monomer-step x monomer-steps per launch /
+mean launch time from HIP events on the engine stream) and the CPU oracle
+timed on this host's cores on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+ALG_BYTES_PER_MONOMER_STEP = 112   # 6 x f32 state in + out (48 B) + two HybridTaus uint4 in + out (64 B)
+HBM_PEAK_GBS = 8000.0
+
+
+def build_workload(ntraj=256, n_dimers=40, seed=20240601):
+    from mt_amd import lattice, params, topology
+    p = params.derive(params.BASE)
+    b = topology.replicate(lattice.microtubule(n_dimers), ntraj, 0.05, seed)
+    b.coords = lattice.wrap_angles(b.coords)
+    return p, b
+
+
+def read_traffic(name):
+    """HBM bytes per dispatch of the dominant kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE / WRITE_SIZE, KiB per dispatch,
+    scaled by the copy-kernel calibration factor recorded beside them)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        m = re.match(r"\s*(fetch_kib_per_dispatch|write_kib_per_dispatch|calibration)\s*[:=]\s*([0-9.eE+-]+)", line)
+        if m:
+            vals[m.group(1)] = float(m.group(2))
+    if not {"fetch_kib_per_dispatch", "write_kib_per_dispatch"} <= vals.keys():
+        return None
+    cal = vals.get("calibration", 1.0)
+    return (vals["fetch_kib_per_dispatch"] + vals["write_kib_per_dispatch"]) * 1024.0 * cal
+
+
+def cpu_baseline(ntraj_sample, steps_sample):
+    """The FP32 CPU oracle (OpenMP over beads, all host cores) on a bounded
+    sample: ntraj_sample trajectories of the same 1040-bead microtubule."""
+    from oracle.oracle import Oracle, max_threads
+    p, b = build_workload(ntraj=ntraj_sample)
+    o = Oracle(p, b, max_lj=64, precision="f32")
+    seeds = np.zeros(8 * b.nb, np.uint32)
+    from oracle.oracle import generate_seeds
+    seeds[:] = generate_seeds(1234567, 2 * b.nb)
+    o.run(0, 1, seeds=seeds)  # warm-up (one rebuild)
+    t0 = time.perf_counter()
+    rc = o.run(1, steps_sample, seeds=seeds)
+    dt = time.perf_counter() - t0
+    if rc != 0:
+        raise RuntimeError(f"cpu oracle status {rc}")
+    return {"value": b.nb * steps_sample / dt, "unit": "monomer-steps/s", "cores": max_threads(), "kind": "port",
+            "sample": f"{ntraj_sample} x 1040-bead trajectories x {steps_sample} steps (steps 1..{steps_sample}, "
+                      f"rebuilds at multiples of 100), FP32 oracle, OpenMP"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--mode", choices=["auto", "streamed", "resident"], default="auto")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-traj", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=1000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(trajectory batches shard across GPUs without communication).", file=sys.stderr)
+        sys.exit(2)
+
+    from mt_amd.engine import Engine, MODE_AUTO, MODE_RESIDENT, MODE_STREAMED
+    mode = {"auto": MODE_AUTO, "streamed": MODE_STREAMED, "resident": MODE_RESIDENT}[args.mode]
+    p, b = build_workload()
+    eng = Engine(p, b, max_lj=48, mode=mode)
+    print(f"[bench] beads={b.nb} trajectories={b.ntraj} mode={eng.mode} warmup={args.warmup}", file=sys.stderr,
+          flush=True)
+    eng.run(args.warmup)
+    eng.reset_timing()
+    eng.set_timing(True)
+    t0 = time.perf_counter()
+    eng.run(args.steps)   # synchronous at return
+    elapsed = time.perf_counter() - t0
+    step_ms, step_n = eng.kernel_time("step")
+    reb_ms, reb_n = eng.kernel_time("rebuild")
+    eng.set_timing(False)
+    st = eng.get_state()
+    if not np.all(np.isfinite(st)):
+        raise RuntimeError("non-finite state after the timed run")
+    value = b.nb * args.steps / elapsed
+    steps_per_launch = args.steps / max(step_n, 1)
+    avg_launch_s = step_ms / 1e3 / max(step_n, 1)
+    alg_bytes = ALG_BYTES_PER_MONOMER_STEP * b.nb * steps_per_launch
+    achieved = alg_bytes / avg_launch_s / 1e9
+    kname = "k_resident" if eng.mode == MODE_RESIDENT else "k_step_plain"
+    traffic = read_traffic(f"traffic_{kname}_c2.txt")
+    out = {
+        "metric": "monomer-steps/s/GPU (batched plain Langevin)",
+        "value": value,
+        "unit": "monomer-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batched plain Langevin: 256 trajectories x 1040-bead 13PFx40-dimer microtubule, "
+                               "LJ list 8 nm rebuilt every 100 steps, HybridTaus noise",
+                   "trajectories": b.ntraj, "beads_per_trajectory": int(b.sizes()[0]), "beads": b.nb,
+                   "mode": "resident" if eng.mode == MODE_RESIDENT else "streamed"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": kname, "launches": step_n, "avg_launch_ms": avg_launch_s * 1e3,
+                     "alg_bytes_per_launch": alg_bytes},
+        "rebuild": {"launches": reb_n, "avg_ms": reb_ms / max(reb_n, 1)},
+    }
+    eng.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_traj, args.cpu_steps)
+        out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -66,7 +66,7 @@ __device__ __forceinline__ void bead_force(const mte_params &P, const DevTopo &T
 #pragma unroll
         for (int q = 0; q < 6; q++) F[q] -= P.C * G.g[q];
         if (G.dr < ANGLE_CUT) {
-            const float th0 = (T.gtp[b] == 1) ? P.theta0_gtp : P.theta0_gdp;
+            const float th0 = A.gtp1(b) ? P.theta0_gtp : P.theta0_gdp;
             bending(P, rm, th0, I, A.ang(j), F);
         }
     }
@@ -79,13 +79,13 @@ __device__ __forceinline__ void bead_force(const mte_params &P, const DevTopo &T
         const float4 xj = A.xyz(j);
         const Grad G = gap_grad(I.r, scale3(rm, I.n), f3(xj), scale3(-rm, f3(A.fn(j))), I.uphi, I.uth);
         float dUdr = (G.dr == 0.f) ? 0.f : dmorse(P.D_long, P.A_long, G.dr) / G.dr;
-        if (P.barrier && T.on_tub[b] == 0 && T.on_tub[j] == 0 && G.dr != 0.f)
+        if (P.barrier && A.offtub(b) && A.offtub(j) && G.dr != 0.f)
             dUdr += dbarr(P.a_barr_long, P.r_barr_long, P.w_barr_long, G.dr) / G.dr;
 #pragma unroll
         for (int q = 0; q < 6; q++) F[q] -= dUdr * G.g[q];
         if (G.dr < ANGLE_CUT) {
             const int last = (I.r.z > xj.z) ? b : j;
-            const float th0 = (T.gtp[last] == 1) ? P.theta0_gtp : P.theta0_gdp;
+            const float th0 = A.gtp1(last) ? P.theta0_gtp : P.theta0_gdp;
             bending(P, rm, th0, I, A.ang(j), F);
         }
     }
@@ -100,9 +100,9 @@ __device__ __forceinline__ void bead_force(const mte_params &P, const DevTopo &T
         const Grad G = gap_grad(I.r, a, f3(A.xyz(j)), bj, I.uphi, I.uth);
         float dUdr;
         if (G.dr == 0.f) dUdr = 0.f;
-        else if (T.mon_type[b] != T.mon_type[j]) dUdr = dmorse(P.D_lat / P.seam_coeff, P.A_lat, G.dr) / G.dr;
+        else if (A.mon(b) != A.mon(j)) dUdr = dmorse(P.D_lat / P.seam_coeff, P.A_lat, G.dr) / G.dr;
         else dUdr = dmorse(P.D_lat, P.A_lat, G.dr) / G.dr;
-        if (P.barrier && T.on_tub[b] == 0 && T.on_tub[j] == 0 && G.dr != 0.f)
+        if (P.barrier && A.offtub(b) && A.offtub(j) && G.dr != 0.f)
             dUdr += dbarr(P.a_barr_lat, P.r_barr_lat, P.w_barr_lat, G.dr) / G.dr;
 #pragma unroll
         for (int q = 0; q < 6; q++) F[q] -= dUdr * G.g[q];
@@ -160,7 +160,7 @@ __device__ __forceinline__ void bead_energy(const mte_params &P, const DevTopo &
             const float psiij = I.psi - aj.z, fiij = I.fi - aj.x, thetaji = aj.y - I.theta;
             U[3] += P.B_psi * (1.f - cosf(psiij - P.psi_0));
             U[4] += P.B_fi * (1.f - cosf(fiij - P.fi_0));
-            U[5] += P.B_theta * (1.f - cosf(thetaji - ((T.gtp[b] == 1) ? P.theta0_gtp : P.theta0_gdp)));
+            U[5] += P.B_theta * (1.f - cosf(thetaji - (A.gtp1(b) ? P.theta0_gtp : P.theta0_gdp)));
         }
     }
     const int nlon = T.lon_n[b];
@@ -171,13 +171,13 @@ __device__ __forceinline__ void bead_energy(const mte_params &P, const DevTopo &
         const float4 xj = A.xyz(j);
         const Grad G = gap_grad(I.r, scale3(rm, I.n), f3(xj), scale3(-rm, f3(A.fn(j))), I.uphi, I.uth);
         U[1] += morse_en(P.D_long, P.A_long, G.dr);
-        if (P.barrier && T.on_tub[b] == 0 && T.on_tub[j] == 0)
+        if (P.barrier && A.offtub(b) && A.offtub(j))
             U[1] += barr(P.a_barr_long, P.r_barr_long, P.w_barr_long, G.dr);
         if (G.dr < ANGLE_CUT) {
             const float4 aj = A.ang(j);
             const float psiij = I.psi - aj.z, fiij = I.fi - aj.x, thetaij = I.theta - aj.y;
             const int last = (I.r.z > xj.z) ? b : j;
-            const float th0 = (T.gtp[last] == 1) ? P.theta0_gtp : P.theta0_gdp;
+            const float th0 = A.gtp1(last) ? P.theta0_gtp : P.theta0_gdp;
             U[3] += P.B_psi * (1.f - cosf(psiij - P.psi_0));
             U[4] += P.B_fi * (1.f - cosf(fiij - P.fi_0));
             U[5] += P.B_theta * (1.f - cosf(thetaij - th0));
@@ -189,9 +189,9 @@ __device__ __forceinline__ void bead_energy(const mte_params &P, const DevTopo &
         const int j = ent_j(e);
         const int sw = ent_f(e);
         const Grad G = gap_grad(I.r, sw ? I.a1 : I.a2, f3(A.xyz(j)), f3(sw ? A.fp2(j) : A.fp1(j)), I.uphi, I.uth);
-        if (T.mon_type[b] != T.mon_type[j]) U[2] += morse_en(P.D_lat / P.seam_coeff, P.A_lat, G.dr);
+        if (A.mon(b) != A.mon(j)) U[2] += morse_en(P.D_lat / P.seam_coeff, P.A_lat, G.dr);
         else U[2] += morse_en(P.D_lat, P.A_lat, G.dr);
-        if (P.barrier && T.on_tub[b] == 0 && T.on_tub[j] == 0)
+        if (P.barrier && A.offtub(b) && A.offtub(j))
             U[2] += barr(P.a_barr_lat, P.r_barr_lat, P.w_barr_lat, G.dr);
     }
     if (P.lj_on) {

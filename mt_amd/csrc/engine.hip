@@ -419,16 +419,18 @@ int mte_run(mte_engine *e, int64_t nsteps) {
     const int64_t freq = e->P.ljpairsupdatefreq;
     const bool need_lists = e->P.lj_on || e->P.is_assembly;
     while (done < nsteps) {
-        if (e->mode == MTE_MODE_RESIDENT && !e->noise) {
+        if (e->mode == MTE_MODE_RESIDENT) {
             // One launch covers every step up to the next rebuild boundary.
             const int64_t to_boundary = freq - (e->step % freq);
             const int64_t chunk = std::min<int64_t>(nsteps - done, to_boundary);
             const bool rb = need_lists && rebuild_due(e);
+            const float *nz = e->noise ? e->noise + (size_t)6 * e->nb * e->noise_cursor : nullptr;
             hipEvent_t ev{};
             time_begin(e, "step", &ev);
-            HIPCHK(launch_resident(e, (int)chunk, rb));
+            HIPCHK(launch_resident(e, (int)chunk, rb, nz));
             time_end(e, "step", ev);
             if (rb) { e->lists_step = e->step; e->lists_valid = true; }
+            if (e->noise) e->noise_cursor += chunk;
             e->step += chunk;
             done += chunk;
             continue;

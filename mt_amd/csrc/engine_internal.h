@@ -82,7 +82,7 @@ hipError_t launch_energies(mte_engine *e);
 hipError_t launch_rebuild(mte_engine *e);
 hipError_t launch_tea_eps(mte_engine *e);
 hipError_t launch_tea_step(mte_engine *e, const float *noise_step);
-hipError_t launch_resident(mte_engine *e, int nsteps, bool rebuild_first);
+hipError_t launch_resident(mte_engine *e, int nsteps, bool rebuild_first, const float *noise);
 bool resident_fits(const mte_engine *e, ResidentPlan *plan);
 
 }  // namespace mte

@@ -10,6 +10,10 @@ constexpr int STREAM_BLOCK = 256;
 
 struct GlobalAcc {
     DevState S;
+    const DevTopo &T;
+    __device__ __forceinline__ int mon(int j) const { return T.mon_type[j]; }
+    __device__ __forceinline__ bool gtp1(int j) const { return T.gtp[j] == 1; }
+    __device__ __forceinline__ bool offtub(int j) const { return T.on_tub[j] == 0; }
     __device__ __forceinline__ float4 xyz(int j) const { return S.xyz[j]; }
     __device__ __forceinline__ float4 ang(int j) const { return S.ang[j]; }
     __device__ __forceinline__ float4 fn(int j) const { return S.fn[j]; }
@@ -49,7 +53,7 @@ __global__ void __launch_bounds__(STREAM_BLOCK)
     }
     const Own I = make_own(x, a, Sin.fn[b], Sin.fp1[b], Sin.fp2[b]);
     float F[6];
-    bead_force(P, T, GlobalAcc{Sin}, b, I, F);
+    bead_force(P, T, GlobalAcc{Sin, T}, b, I, F);
     float4 rx, ra;
     if (noise) {
         const float *nz = noise + 6 * (size_t)b;
@@ -81,7 +85,7 @@ __global__ void __launch_bounds__(STREAM_BLOCK)
     float F[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (!T.extra[b]) {
         const Own I = make_own(S.xyz[b], S.ang[b], S.fn[b], S.fp1[b], S.fp2[b]);
-        bead_force(P, T, GlobalAcc{S}, b, I, F);
+        bead_force(P, T, GlobalAcc{S, T}, b, I, F);
     }
     fxyz[b] = make_float4(F[0], F[1], F[2], 0.f);
     fang[b] = make_float4(F[3], F[4], F[5], 0.f);   // fi, theta, psi
@@ -94,7 +98,7 @@ __global__ void __launch_bounds__(STREAM_BLOCK)
     float U[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (!T.extra[b]) {
         const Own I = make_own(S.xyz[b], S.ang[b], S.fn[b], S.fp1[b], S.fp2[b]);
-        bead_energy(P, T, GlobalAcc{S}, b, I, U);
+        bead_energy(P, T, GlobalAcc{S, T}, b, I, U);
     }
 #pragma unroll
     for (int q = 0; q < 7; q++) en[7 * (size_t)b + q] = (double)U[q];

@@ -116,3 +116,44 @@ def test_tea_step_parity():
     print("TEA max |dx|", dx, flush=True)
     assert dx < 1e-4
     eng.close()
+
+
+def test_resident_injected_noise_matches_oracle():
+    print("case resident 300 steps injected noise, 2 trajectories", flush=True)
+    from mt_amd.engine import MODE_RESIDENT
+    p = base_params()
+    b = c1_batch(ntraj=2)
+    eng, orc = _pair(p, b, mode=MODE_RESIDENT, precision="f32")
+    assert eng.mode == MODE_RESIDENT
+    rng = np.random.default_rng(21)
+    nsteps = 300
+    nz = noise(rng, nsteps, b.nb)
+    eng.set_noise(nz)
+    eng.run(nsteps)
+    assert orc.run(0, nsteps, noise=nz) == 0
+    g = eng.get_state()
+    o = orc.b.coords
+    dx = np.abs(g[:, :3] - o[:, :3]).max()
+    da = np.abs(g[:, 3:6] - o[:, 3:6]).max()
+    print("resident max |dx|", dx, "max |dang|", da, flush=True)
+    assert dx < 1e-3 and da < 1e-3
+    _lists_equal(eng, orc)
+    eng.close()
+
+
+def test_resident_rng_and_uneven_batch():
+    print("case resident HybridTaus, uneven batch, 250 steps", flush=True)
+    from mt_amd.engine import MODE_RESIDENT
+    p = base_params()
+    b = uneven_batch()
+    eng, orc = _pair(p, b, mode=MODE_RESIDENT, precision="f32")
+    assert eng.mode == MODE_RESIDENT
+    seeds = eng.get_rng().copy()
+    eng.run(250)
+    assert orc.run(0, 250, seeds=seeds) == 0
+    np.testing.assert_array_equal(eng.get_rng(), seeds)
+    g = eng.get_state()
+    dx = np.abs(g[:, :3] - orc.b.coords[:, :3]).max()
+    print("resident uneven max |dx|", dx, flush=True)
+    assert dx < 1e-3
+    eng.close()
