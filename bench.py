@@ -40,20 +40,22 @@ def build_workload(ntraj=256, n_dimers=40, seed=20240601):
 
 def read_traffic(name):
     """HBM bytes per dispatch of the dominant kernel from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE / WRITE_SIZE, KiB per dispatch,
-    scaled by the copy-kernel calibration factor recorded beside them)."""
+    rocprofv3 --pmc summary: FETCH_SIZE and WRITE_SIZE (KiB per dispatch),
+    each scaled by what the same counter reads for a plain 1 GiB copy kernel
+    (the calibration factors recorded beside them)."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     vals = {}
     for line in open(path):
-        m = re.match(r"\s*(fetch_kib_per_dispatch|write_kib_per_dispatch|calibration)\s*[:=]\s*([0-9.eE+-]+)", line)
+        m = re.match(r"\s*(fetch_kib_per_dispatch|write_kib_per_dispatch|fetch_calibration|write_calibration)"
+                     r"\s*[:=]\s*([0-9.eE+-]+)", line)
         if m:
             vals[m.group(1)] = float(m.group(2))
     if not {"fetch_kib_per_dispatch", "write_kib_per_dispatch"} <= vals.keys():
         return None
-    cal = vals.get("calibration", 1.0)
-    return (vals["fetch_kib_per_dispatch"] + vals["write_kib_per_dispatch"]) * 1024.0 * cal
+    return 1024.0 * (vals["fetch_kib_per_dispatch"] * vals.get("fetch_calibration", 1.0)
+                     + vals["write_kib_per_dispatch"] * vals.get("write_calibration", 1.0))
 
 
 def cpu_baseline(ntraj_sample, steps_sample):
@@ -85,7 +87,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-traj", type=int, default=8)
     ap.add_argument("--cpu-steps", type=int, default=1000)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 1 GiB device-to-device copy kernel (for scaling FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": x.numel() * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(trajectory batches shard across GPUs without communication).", file=sys.stderr)

@@ -15,7 +15,8 @@ LIB = os.path.join(HERE, "libmt_engine.so")
 SOURCES = ["engine.hip", "forces.hip", "lists.hip", "tea.hip", "resident.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wno-unused-function",
-         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"]
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics",
+         "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
 
 
 def build(verbose: bool = False) -> str:

@@ -18,6 +18,21 @@ struct Own {
     float fi, theta, psi;
 };
 
+// trig = (sin theta, cos theta, sin psi, cos psi) of the bead.
+__device__ __forceinline__ Own make_own_trig(float4 xyz, float4 ang, float4 fn, float4 fp1, float4 fp2, float4 trig) {
+    Own o;
+    o.r = f3(xyz);
+    o.n = f3(fn);
+    o.a1 = f3(fp1);
+    o.a2 = f3(fp2);
+    o.fi = ang.x;
+    o.theta = ang.y;
+    o.psi = ang.z;
+    o.uphi = make_float3(trig.w * trig.y, trig.z * trig.y, -trig.x);
+    o.uth = make_float2(-trig.z, trig.w);
+    return o;
+}
+
 __device__ __forceinline__ Own make_own(float4 xyz, float4 ang, float4 fn, float4 fp1, float4 fp2) {
     Own o;
     o.r = f3(xyz);
@@ -28,8 +43,8 @@ __device__ __forceinline__ Own make_own(float4 xyz, float4 ang, float4 fn, float
     o.theta = ang.y;
     o.psi = ang.z;
     float st, ct, sp, cp;
-    sincosf(ang.y, &st, &ct);
-    sincosf(ang.z, &sp, &cp);
+    sincos_fast(ang.y, &st, &ct);
+    sincos_fast(ang.z, &sp, &cp);
     o.uphi = make_float3(cp * ct, sp * ct, -st);
     o.uth = make_float2(-sp, cp);
     return o;
@@ -40,24 +55,36 @@ __device__ __forceinline__ void bending(const mte_params &P, float rm, float th0
                                         float F[6]) {
     const float psiji = aj.z - I.psi, thetaji = aj.y - I.theta, fiji = aj.x - I.fi;
     if (rm > 0.f) {
-        F[5] += P.B_psi * sinf(psiji - P.psi_0);
-        F[3] += P.B_fi * sinf(fiji - P.fi_0);
-        F[4] += P.B_theta * sinf(thetaji - th0);
+        F[5] += P.B_psi * sin_fast(psiji - P.psi_0);
+        F[3] += P.B_fi * sin_fast(fiji - P.fi_0);
+        F[4] += P.B_theta * sin_fast(thetaji - th0);
     } else {
-        F[5] -= P.B_psi * sinf(-psiji - P.psi_0);
-        F[3] -= P.B_fi * sinf(-fiji - P.fi_0);
-        F[4] -= P.B_theta * sinf(-thetaji - th0);
+        F[5] -= P.B_psi * sin_fast(-psiji - P.psi_0);
+        F[3] -= P.B_fi * sin_fast(-fiji - P.fi_0);
+        F[4] -= P.B_theta * sin_fast(-thetaji - th0);
     }
 }
 
-template <class Acc>
-__device__ __forceinline__ void bead_force(const mte_params &P, const DevTopo &T, const Acc &A, int b, const Own &I,
-                                           float F[6]) {
+// Neighbour lists held in the k-major global arrays (packed global entries).
+struct GlobalLists {
+    const DevTopo &T;
+    int b;
+    __device__ __forceinline__ int harm() const { return T.harm[b]; }
+    __device__ __forceinline__ int nlon() const { return T.lon_n[b]; }
+    __device__ __forceinline__ int lon(int k) const { return T.lon[k * T.nb + b]; }
+    __device__ __forceinline__ int nlat() const { return T.lat_n[b]; }
+    __device__ __forceinline__ int lat(int k) const { return T.lat[k * T.nb + b]; }
+    __device__ __forceinline__ int nlj() const { return T.lj_n[b]; }
+    __device__ __forceinline__ int lj(int k) const { return T.lj[k * T.nb + b]; }
+};
+
+template <class Acc, class Lists>
+__device__ __forceinline__ void bead_force(const mte_params &P, const DevTopo &T, const Acc &A, const Lists &Ls,
+                                           int b, const Own &I, float F[6]) {
 #pragma unroll
     for (int q = 0; q < 6; q++) F[q] = 0.f;
-    const int nb = T.nb;
     // harmonic (intra-dimer spring + bending), :70-182
-    const int h = T.harm[b];
+    const int h = Ls.harm();
     if (h >= 0) {
         const int j = ent_j(h);
         const float rm = ent_f(h) ? RMON : -RMON;
@@ -71,16 +98,16 @@ __device__ __forceinline__ void bead_force(const mte_params &P, const DevTopo &T
         }
     }
     // longitudinal Morse + barrier + bending, :189-299
-    const int nlon = T.lon_n[b];
+    const int nlon = Ls.nlon();
     for (int k = 0; k < nlon; k++) {
-        const int e = T.lon[k * nb + b];
+        const int e = Ls.lon(k);
         const int j = ent_j(e);
         const float rm = ent_f(e) ? RMON : -RMON;
         const float4 xj = A.xyz(j);
         const Grad G = gap_grad(I.r, scale3(rm, I.n), f3(xj), scale3(-rm, f3(A.fn(j))), I.uphi, I.uth);
-        float dUdr = (G.dr == 0.f) ? 0.f : dmorse(P.D_long, P.A_long, G.dr) / G.dr;
+        float dUdr = (G.dr == 0.f) ? 0.f : dmorse(P.D_long, P.A_long, G.dr) * G.inv;
         if (P.barrier && A.offtub(b) && A.offtub(j) && G.dr != 0.f)
-            dUdr += dbarr(P.a_barr_long, P.r_barr_long, P.w_barr_long, G.dr) / G.dr;
+            dUdr += dbarr(P.a_barr_long, P.r_barr_long, P.w_barr_long, G.dr) * G.inv;
 #pragma unroll
         for (int q = 0; q < 6; q++) F[q] -= dUdr * G.g[q];
         if (G.dr < ANGLE_CUT) {
@@ -90,9 +117,9 @@ __device__ __forceinline__ void bead_force(const mte_params &P, const DevTopo &T
         }
     }
     // lateral Morse + barrier with seam weakening, :304-466
-    const int nlat = T.lat_n[b];
+    const int nlat = Ls.nlat();
     for (int k = 0; k < nlat; k++) {
-        const int e = T.lat[k * nb + b];
+        const int e = Ls.lat(k);
         const int j = ent_j(e);
         const int sw = ent_f(e);
         const float3 a = sw ? I.a1 : I.a2;
@@ -100,24 +127,24 @@ __device__ __forceinline__ void bead_force(const mte_params &P, const DevTopo &T
         const Grad G = gap_grad(I.r, a, f3(A.xyz(j)), bj, I.uphi, I.uth);
         float dUdr;
         if (G.dr == 0.f) dUdr = 0.f;
-        else if (A.mon(b) != A.mon(j)) dUdr = dmorse(P.D_lat / P.seam_coeff, P.A_lat, G.dr) / G.dr;
-        else dUdr = dmorse(P.D_lat, P.A_lat, G.dr) / G.dr;
+        else if (A.mon(b) != A.mon(j)) dUdr = dmorse(P.D_lat / P.seam_coeff, P.A_lat, G.dr) * G.inv;
+        else dUdr = dmorse(P.D_lat, P.A_lat, G.dr) * G.inv;
         if (P.barrier && A.offtub(b) && A.offtub(j) && G.dr != 0.f)
-            dUdr += dbarr(P.a_barr_lat, P.r_barr_lat, P.w_barr_lat, G.dr) / G.dr;
+            dUdr += dbarr(P.a_barr_lat, P.r_barr_lat, P.w_barr_lat, G.dr) * G.inv;
 #pragma unroll
         for (int q = 0; q < 6; q++) F[q] -= dUdr * G.g[q];
     }
     // LJ repulsion r^-6 below the hard 6 nm cutoff, :470-495
     if (P.lj_on) {
-        const int nlj = T.lj_n[b];
+        const int nlj = Ls.nlj();
         const float s0 = P.ljscale * P.ljsigma6;
         for (int k = 0; k < nlj; k++) {
-            const int j = T.lj[k * nb + b];
+            const int j = Ls.lj(k);
             const float3 rj = f3(A.xyz(j));
             const float dx = I.r.x - rj.x, dy = I.r.y - rj.y, dz = I.r.z - rj.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            if (sqrtf(r2) < LJ_CUT) {
-                const float inv = 1.f / r2;
+            if (r2 < LJ_CUT * LJ_CUT) {
+                const float inv = rcp_fast(r2);
                 const float df = 6.f * (inv * inv) * (inv * inv);
                 const float s = s0 * df;
                 F[0] += s * dx;

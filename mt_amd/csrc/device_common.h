@@ -64,6 +64,38 @@ struct DevState {
 
 __device__ __forceinline__ float3 f3(float4 v) { return make_float3(v.x, v.y, v.z); }
 
+// sin and cos by Cody-Waite reduction modulo pi/2 (two-part pi/2, exact
+// with FMA for |x| < 8192) and the single-precision minimax polynomials of
+// the Cephes library (S. L. Moshier, sinf.c / cosf.c) on [-pi/4, pi/4];
+// about 1 ulp.  |x| >= 8192 is first reduced modulo 2*pi in FP64.
+__device__ __forceinline__ void sincos_fast(float x, float *s, float *c) {
+    if (!(fabsf(x) < 8192.f)) {
+        const double d = (double)x;
+        x = (float)(d - 6.283185307179586477 * rint(d * 0.15915494309189533577));
+    }
+    const float j = rintf(x * 0.63661977236758134f);
+    const int q = (int)j;
+    float r = fmaf(j, -1.57079637050628662109375f, x);
+    r = fmaf(j, 4.37113900018624283e-8f, r);
+    const float z = r * r;
+    const float sp = fmaf(z * r, fmaf(z, fmaf(z, -1.9515295891e-4f, 8.3321608736e-3f), -1.6666654611e-1f), r);
+    const float cp = fmaf(z * z, fmaf(z, fmaf(z, 2.443315711809948e-5f, -1.388731625493765e-3f), 4.166664568298827e-2f),
+                          fmaf(-0.5f, z, 1.0f));
+    float ss = (q & 1) ? cp : sp;
+    float cc = (q & 1) ? sp : cp;
+    if (q & 2) ss = -ss;
+    if ((q + 1) & 2) cc = -cc;
+    *s = ss;
+    *c = cc;
+}
+__device__ __forceinline__ float sin_fast(float x) {
+    float s, c;
+    sincos_fast(x, &s, &c);
+    return s;
+}
+__device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rsq_fast(float x) { return __builtin_amdgcn_rsqf(x); }
+
 struct Rot {
     float cf, sf, ct, st, cp, sp;
     __device__ __forceinline__ float3 apply(float vx, float vy, float vz) const {
@@ -75,9 +107,9 @@ struct Rot {
 
 __device__ __forceinline__ Rot make_rot(float fi, float theta, float psi) {
     Rot r;
-    sincosf(fi, &r.sf, &r.cf);
-    sincosf(theta, &r.st, &r.ct);
-    sincosf(psi, &r.sp, &r.cp);
+    sincos_fast(fi, &r.sf, &r.cf);
+    sincos_fast(theta, &r.st, &r.ct);
+    sincos_fast(psi, &r.sp, &r.cp);
     return r;
 }
 
@@ -95,7 +127,7 @@ __device__ __forceinline__ void frame_vectors(float fi, float theta, float psi, 
 
 // Gap and gradient of |g|^2/2 with respect to bead i.  grad order:
 // x, y, z, fi, theta, psi.  uphi = (cp*ct, sp*ct, -st), uth = (-sp, cp, 0).
-struct Grad { float g[6]; float dr; };
+struct Grad { float g[6]; float dr, inv; };   // inv = 1/dr (0 when dr == 0)
 
 __device__ __forceinline__ Grad gap_grad(float3 ri, float3 a, float3 rj, float3 b, float3 uphi, float2 uth) {
     Grad o;
@@ -109,24 +141,27 @@ __device__ __forceinline__ Grad gap_grad(float3 ri, float3 a, float3 rj, float3 
     o.g[3] = uphi.x * t0 + uphi.y * t1 + uphi.z * t2;
     o.g[4] = uth.x * t0 + uth.y * t1;
     o.g[5] = t2;
-    o.dr = sqrtf(gx * gx + gy * gy + gz * gz);
+    const float d2 = gx * gx + gy * gy + gz * gz;
+    o.inv = d2 > 0.f ? rsq_fast(d2) : 0.f;
+    o.dr = d2 * o.inv;
     return o;
 }
 
 // src/compute_cuda.cu:16-30
 __device__ __forceinline__ float dmorse(float D, float a, float x) {
-    const float e = expf(-a * x);
+    const float e = __expf(-a * x);
     return 2.f * a * D * (1.f - e) * e;
 }
 __device__ __forceinline__ float morse_en(float D, float a, float x) {
-    const float e = 1.f - expf(-a * x);
+    const float e = 1.f - __expf(-a * x);
     return D * e * e - D;
 }
 __device__ __forceinline__ float dbarr(float a, float r, float w, float x) {
-    return -a * expf(-(x - r) * (x - r) / (2.f * w * w)) * (x - r) / (w * w);
+    const float iw2 = rcp_fast(w * w);
+    return -a * __expf(-0.5f * (x - r) * (x - r) * iw2) * (x - r) * iw2;
 }
 __device__ __forceinline__ float barr(float a, float r, float w, float x) {
-    return a * expf(-(x - r) * (x - r) / (2.f * w * w));
+    return a * __expf(-0.5f * (x - r) * (x - r) * rcp_fast(w * w));
 }
 
 // ---------------------------------------------------------------- HybridTaus

@@ -145,6 +145,7 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     e->P = *params;
     e->O = *opt;
     if (e->O.max_lj <= 0) e->O.max_lj = 64;
+    if (e->O.max_lj > 1023) { e->err = "mte_create: max_lj must be at most 1023"; return bail(MTE_ERR_ARG); }
     const int nt = topo->ntraj;
     if (nt <= 0 || !topo->traj_offsets) { e->err = "mte_create: ntraj must be positive"; return bail(MTE_ERR_ARG); }
     e->off.assign(topo->traj_offsets, topo->traj_offsets + nt + 1);

@@ -32,6 +32,7 @@ struct ResidentPlan {
     int per_thread = 0;     // beads per thread
     int max_beads = 0;      // largest trajectory
     size_t lds_bytes = 0;
+    int entry_cap = 0;      // uint16 list entries that fit beside the state
 };
 
 }  // namespace mte

@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(STREAM_BLOCK)
     }
     const Own I = make_own(x, a, Sin.fn[b], Sin.fp1[b], Sin.fp2[b]);
     float F[6];
-    bead_force(P, T, GlobalAcc{Sin, T}, b, I, F);
+    bead_force(P, T, GlobalAcc{Sin, T}, GlobalLists{T, b}, b, I, F);
     float4 rx, ra;
     if (noise) {
         const float *nz = noise + 6 * (size_t)b;
@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(STREAM_BLOCK)
     float F[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (!T.extra[b]) {
         const Own I = make_own(S.xyz[b], S.ang[b], S.fn[b], S.fp1[b], S.fp2[b]);
-        bead_force(P, T, GlobalAcc{S, T}, b, I, F);
+        bead_force(P, T, GlobalAcc{S, T}, GlobalLists{T, b}, b, I, F);
     }
     fxyz[b] = make_float4(F[0], F[1], F[2], 0.f);
     fang[b] = make_float4(F[3], F[4], F[5], 0.f);   // fi, theta, psi

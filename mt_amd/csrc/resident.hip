@@ -1,36 +1,50 @@
 // resident.hip -- plain Langevin dynamics with each trajectory resident in
 // one workgroup's LDS across many steps.
 //
-// One workgroup owns one trajectory: positions, angles and rotated offsets of
-// all its beads (80 B per bead) live in LDS, each thread owns PT beads and
-// keeps their two HybridTaus states in registers.  A launch runs every step up
-// to the next list-rebuild boundary (LJPairsUpdateFreq steps, 100 in the
-// batched workload), doing the rebuild itself first when it is due, so HBM
-// sees the state once per launch instead of twice per step.  Per step:
-//   forces + Euler-Maruyama for own beads (neighbours read from LDS, lists
-//   from L2-resident k-major arrays) -> barrier -> write own new state and
-//   rotated offsets to LDS -> barrier.
-// The arithmetic is bead_force / em_update, the same code the streamed step
-// runs, so the two modes agree to rounding.
+// One workgroup owns one trajectory.  LDS holds, per bead, the position and
+// the three rotated offsets with the Euler angles packed into their spare
+// lanes (64 B), plus the bead's neighbour lists compacted into a uint16 CSR
+// (local index << 1 | flag).  Each thread owns PT beads and keeps their two
+// HybridTaus states in registers.  A launch runs every step up to the next
+// list-rebuild boundary (LJPairsUpdateFreq steps; 100 in the batched
+// workload), doing the rebuild itself first when it is due, so HBM sees the
+// state once per launch instead of twice per step.  Per step:
+//   forces + Euler-Maruyama for own beads, every operand from LDS
+//   -> barrier -> own new state to LDS -> barrier.
+// The arithmetic is bead_force / em_update, the code the streamed step runs.
+// If a trajectory's lists do not fit the LDS left over, that workgroup reads
+// them from the global k-major arrays instead (same results, slower).
 #include "bead_physics.h"
 #include "engine_internal.h"
 #include "rebuild_common.h"
 
 namespace mte {
 
-constexpr int RES_MAX_THREADS = 1024;  // one bead per thread
+constexpr int RES_MAX_THREADS = 1024;   // one bead per thread
 constexpr int RES_MULTI_THREADS = 768;  // several beads per thread: leaves room for more VGPRs
 constexpr int RES_MAX_PT = 3;
-constexpr int LDS_ARRAYS = 5;   // xyz, ang, fn, fp1, fp2
+constexpr int STATE_ARRAYS = 4;         // xyz|flags, n|fi, p1|theta, p2|psi
+constexpr int HDR_INTS = 2;             // per bead: list offset, packed counts
+constexpr int SCAN_INTS = RES_MAX_THREADS;
+// Minimum room for list entries per bead when sizing the plan.
+constexpr int MIN_ENTRIES_PER_BEAD = 24;
 
 // Per-bead flags packed into xyz.w in LDS.
 constexpr int FL_MON = 1, FL_GTP1 = 2, FL_OFFTUB = 4;
 
+// Packed counts: bit 0 harmonic present, bits 1-4 longitudinal, 5-9 lateral,
+// 10-19 LJ.
+__device__ __forceinline__ int pack_counts(int h, int nlon, int nlat, int nlj) {
+    return (h ? 1 : 0) | (nlon << 1) | (nlat << 5) | (nlj << 10);
+}
+
 struct LdsAcc {
-    const float4 *x, *a, *n, *p1, *p2;
+    const float4 *x, *n, *p1, *p2;
     int base;
     __device__ __forceinline__ float4 xyz(int j) const { return x[j - base]; }
-    __device__ __forceinline__ float4 ang(int j) const { return a[j - base]; }
+    __device__ __forceinline__ float4 ang(int j) const {
+        return make_float4(n[j - base].w, p1[j - base].w, p2[j - base].w, 0.f);
+    }
     __device__ __forceinline__ float4 fn(int j) const { return n[j - base]; }
     __device__ __forceinline__ float4 fp1(int j) const { return p1[j - base]; }
     __device__ __forceinline__ float4 fp2(int j) const { return p2[j - base]; }
@@ -40,12 +54,95 @@ struct LdsAcc {
     __device__ __forceinline__ bool offtub(int j) const { return (flags(j) & FL_OFFTUB) != 0; }
 };
 
+// Lists compacted in LDS; entries carry local indices, returned as packed
+// global entries so bead_force sees the same encoding as GlobalLists.
+struct LdsLists {
+    const uint16_t *e;
+    int off, cnt, base;
+    __device__ __forceinline__ int conv(int v) const { return ((base + (v >> 1)) << 1) | (v & 1); }
+    __device__ __forceinline__ int h() const { return cnt & 1; }
+    __device__ __forceinline__ int harm() const { return h() ? conv(e[off]) : -1; }
+    __device__ __forceinline__ int nlon() const { return (cnt >> 1) & 15; }
+    __device__ __forceinline__ int lon(int k) const { return conv(e[off + h() + k]); }
+    __device__ __forceinline__ int nlat() const { return (cnt >> 5) & 31; }
+    __device__ __forceinline__ int lat(int k) const { return conv(e[off + h() + nlon() + k]); }
+    __device__ __forceinline__ int nlj() const { return (cnt >> 10) & 1023; }
+    __device__ __forceinline__ int lj(int k) const { return base + e[off + h() + nlon() + nlat() + k]; }
+};
+
+// Writes a bead's new state to LDS; returns (sin theta, cos theta, sin psi,
+// cos psi) for the owner's next force evaluation.
+__device__ __forceinline__ float4 store_state(float4 *lx, float4 *ln, float4 *l1, float4 *l2, int i, float4 x,
+                                              float4 a, const LatOffsets &L) {
+    const Rot R = make_rot(a.x, a.y, a.z);
+    float3 v = R.apply(0.f, 0.f, 1.f);
+    float4 n = make_float4(v.x, v.y, v.z, 0.f);
+    v = R.apply(L.p1x, L.p1y, L.p1z);
+    float4 p1 = make_float4(v.x, v.y, v.z, 0.f);
+    v = R.apply(L.p2x, L.p2y, L.p2z);
+    float4 p2 = make_float4(v.x, v.y, v.z, 0.f);
+    n.w = a.x;   // fi
+    p1.w = a.y;  // theta
+    p2.w = a.z;  // psi
+    lx[i] = x;
+    ln[i] = n;
+    l1[i] = p1;
+    l2[i] = p2;
+    return make_float4(R.st, R.ct, R.sp, R.cp);
+}
+
+template <int PT, class Lists, class MakeLists>
+__device__ __forceinline__ void run_steps(const mte_params &P, const DevTopo &T, const LdsAcc &A, float4 *lx,
+                                          float4 *ln, float4 *l1, float4 *l2, const bool mob[PT], uint4 s0[PT],
+                                          uint4 s1[PT], float4 trig[PT], const float *noise, int nsteps, const LatOffsets &L,
+                                          const MakeLists &make_lists) {
+    const int base = A.base, nb = T.nb;
+    for (int s = 0; s < nsteps; s++) {
+        float4 nx[PT], na[PT];
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            if (!mob[k]) continue;
+            const int i = threadIdx.x + k * blockDim.x;
+            const int b = base + i;
+            float4 x = lx[i];
+            float4 a = make_float4(ln[i].w, l1[i].w, l2[i].w, 0.f);
+            const Own I = make_own_trig(x, a, ln[i], l1[i], l2[i], trig[k]);
+            float F[6];
+            const Lists Ls = make_lists(i, b);
+            bead_force(P, T, A, Ls, b, I, F);
+            float4 rx, ra;
+            if (noise) {
+                const float *nz = noise + (size_t)s * 6 * nb + 6 * (size_t)b;
+                rx = make_float4(nz[0], nz[1], nz[2], 0.f);
+                ra = make_float4(nz[3], nz[4], nz[5], 0.f);
+            } else {
+                rx = normals4(s0[k]);
+                ra = normals4(s1[k]);
+            }
+            em_update(P, F, rx, ra, x, a);
+            nx[k] = x;   // w keeps the packed flags
+            na[k] = a;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            if (!mob[k]) continue;
+            trig[k] = store_state(lx, ln, l1, l2, threadIdx.x + k * blockDim.x, nx[k], na[k], L);
+        }
+        __syncthreads();
+    }
+}
+
 template <int PT, int MAXT>
 __global__ void __launch_bounds__(MAXT)
     k_resident(mte_params P, DevTopo T, DevState S, uint4 *seeds, const float *noise, int nsteps, int rebuild,
-               int do_lj, int do_bonded, LatOffsets L, int stride) {
+               int do_lj, int do_bonded, LatOffsets L, int stride, int entry_cap) {
     extern __shared__ float4 lds[];
-    float4 *lx = lds, *la = lds + stride, *ln = lds + 2 * stride, *l1 = lds + 3 * stride, *l2 = lds + 4 * stride;
+    float4 *lx = lds, *ln = lds + stride, *l1 = lds + 2 * stride, *l2 = lds + 3 * stride;
+    int *hoff = reinterpret_cast<int *>(lds + STATE_ARRAYS * stride);
+    int *hcnt = hoff + stride;
+    int *scan = hcnt + stride;
+    uint16_t *ent = reinterpret_cast<uint16_t *>(scan + SCAN_INTS);
     const int t = blockIdx.x;
     const int base = T.tbase[t], n = T.tsize[t];
     const int nb = T.nb;
@@ -54,26 +151,37 @@ __global__ void __launch_bounds__(MAXT)
         float4 x = S.xyz[b];
         const int fl = (T.mon_type[b] ? FL_MON : 0) | (T.gtp[b] == 1 ? FL_GTP1 : 0) | (T.on_tub[b] == 0 ? FL_OFFTUB : 0);
         x.w = __int_as_float(fl);
+        const float4 a = S.ang[b];
+        float4 fn = S.fn[b], f1 = S.fp1[b], f2 = S.fp2[b];
+        fn.w = a.x;
+        f1.w = a.y;
+        f2.w = a.z;
         lx[i] = x;
-        la[i] = S.ang[b];
-        ln[i] = S.fn[b];
-        l1[i] = S.fp1[b];
-        l2[i] = S.fp2[b];
+        ln[i] = fn;
+        l1[i] = f1;
+        l2[i] = f2;
     }
     bool mob[PT];
     uint4 s0[PT], s1[PT];
+    float4 trig[PT];
 #pragma unroll
     for (int k = 0; k < PT; k++) {
         const int i = threadIdx.x + k * blockDim.x;
         const int b = base + i;
         mob[k] = i < n && !T.fixed[b] && !T.extra[b];
+        trig[k] = make_float4(0.f, 1.f, 0.f, 1.f);
+        if (mob[k]) {
+            const float4 a = S.ang[b];
+            sincos_fast(a.y, &trig[k].x, &trig[k].y);
+            sincos_fast(a.z, &trig[k].z, &trig[k].w);
+        }
         if (mob[k] && !noise) {
             s0[k] = seeds[b];
             s1[k] = seeds[b + nb];
         }
     }
     __syncthreads();
-    const LdsAcc A{lx, la, ln, l1, l2, base};
+    const LdsAcc A{lx, ln, l1, l2, base};
     if (rebuild) {
 #pragma unroll
         for (int k = 0; k < PT; k++) {
@@ -88,54 +196,87 @@ __global__ void __launch_bounds__(MAXT)
             rebuild_finish(T, o, do_lj, do_bonded);
         }
     }
-    for (int s = 0; s < nsteps; s++) {
-        float4 nx[PT], na[PT];
+    // Compact this trajectory's lists into LDS: totals -> block scan -> copy.
+    int mytot[PT];
+#pragma unroll
+    for (int k = 0; k < PT; k++) {
+        const int i = threadIdx.x + k * blockDim.x;
+        mytot[k] = 0;
+        if (i >= n) continue;
+        const int b = base + i;
+        const int h = T.harm[b] >= 0 ? 1 : 0;
+        const int nlon = T.lon_n[b], nlat = T.lat_n[b], nlj = P.lj_on ? T.lj_n[b] : 0;
+        hcnt[i] = pack_counts(h, nlon, nlat, nlj);
+        mytot[k] = h + nlon + nlat + nlj;
+    }
+    // Chunked exclusive scan: thread t owns beads [t*c, (t+1)*c).
+    const int c = (n + blockDim.x - 1) / blockDim.x;
+    // totals go through hoff, indexed by bead
+#pragma unroll
+    for (int k = 0; k < PT; k++) {
+        const int i = threadIdx.x + k * blockDim.x;
+        if (i < n) hoff[i] = mytot[k];
+    }
+    __syncthreads();
+    int run = 0;
+    const int lo = threadIdx.x * c, hi = min(n, lo + c);
+    for (int i = lo; i < hi; i++) run += hoff[i];
+    scan[threadIdx.x] = run;
+    __syncthreads();
+    for (int d = 1; d < (int)blockDim.x; d <<= 1) {
+        const int v = threadIdx.x >= (unsigned)d ? scan[threadIdx.x - d] : 0;
+        __syncthreads();
+        scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const int total = scan[blockDim.x - 1];
+    int acc = scan[threadIdx.x] - run;   // exclusive prefix of this chunk
+    for (int i = lo; i < hi; i++) {
+        const int v = hoff[i];
+        hoff[i] = acc;
+        acc += v;
+    }
+    __syncthreads();
+    const bool use_lds = total <= entry_cap;
+    if (use_lds) {
 #pragma unroll
         for (int k = 0; k < PT; k++) {
-            if (!mob[k]) continue;
             const int i = threadIdx.x + k * blockDim.x;
+            if (i >= n) continue;
             const int b = base + i;
-            float4 x = lx[i], a = la[i];
-            const Own I = make_own(x, a, ln[i], l1[i], l2[i]);
-            float F[6];
-            bead_force(P, T, A, b, I, F);
-            float4 rx, ra;
-            if (noise) {
-                const float *nz = noise + (size_t)s * 6 * nb + 6 * (size_t)b;
-                rx = make_float4(nz[0], nz[1], nz[2], 0.f);
-                ra = make_float4(nz[3], nz[4], nz[5], 0.f);
-            } else {
-                rx = normals4(s0[k]);
-                ra = normals4(s1[k]);
+            int o = hoff[i];
+            const int hc = hcnt[i];
+            if (hc & 1) {
+                const int e = T.harm[b];
+                ent[o++] = (uint16_t)((((e >> 1) - base) << 1) | (e & 1));
             }
-            em_update(P, F, rx, ra, x, a);
-            nx[k] = x;
-            na[k] = a;
+            for (int q = 0; q < ((hc >> 1) & 15); q++) {
+                const int e = T.lon[q * nb + b];
+                ent[o++] = (uint16_t)((((e >> 1) - base) << 1) | (e & 1));
+            }
+            for (int q = 0; q < ((hc >> 5) & 31); q++) {
+                const int e = T.lat[q * nb + b];
+                ent[o++] = (uint16_t)((((e >> 1) - base) << 1) | (e & 1));
+            }
+            for (int q = 0; q < ((hc >> 10) & 1023); q++) ent[o++] = (uint16_t)(T.lj[q * nb + b] - base);
         }
         __syncthreads();
-#pragma unroll
-        for (int k = 0; k < PT; k++) {
-            if (!mob[k]) continue;
-            const int i = threadIdx.x + k * blockDim.x;
-            lx[i] = nx[k];   // w keeps the packed flags
-            la[i] = na[k];
-            float4 nn, p1, p2;
-            frame_vectors(na[k].x, na[k].y, na[k].z, L, nn, p1, p2);
-            ln[i] = nn;
-            l1[i] = p1;
-            l2[i] = p2;
-        }
-        __syncthreads();
+        auto mk = [&](int i, int) { return LdsLists{ent, hoff[i], hcnt[i], base}; };
+        run_steps<PT, LdsLists>(P, T, A, lx, ln, l1, l2, mob, s0, s1, trig, noise, nsteps, L, mk);
+    } else {
+        auto mk = [&](int, int b) { return GlobalLists{T, b}; };
+        run_steps<PT, GlobalLists>(P, T, A, lx, ln, l1, l2, mob, s0, s1, trig, noise, nsteps, L, mk);
     }
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int b = base + i;
-        float4 x = lx[i];
+        float4 x = lx[i], fn = ln[i], f1 = l1[i], f2 = l2[i];
         x.w = 0.f;
         S.xyz[b] = x;
-        S.ang[b] = la[i];
-        S.fn[b] = ln[i];
-        S.fp1[b] = l1[i];
-        S.fp2[b] = l2[i];
+        S.ang[b] = make_float4(fn.w, f1.w, f2.w, 0.f);
+        fn.w = f1.w = f2.w = 0.f;
+        S.fn[b] = fn;
+        S.fp1[b] = f1;
+        S.fp2[b] = f2;
     }
 #pragma unroll
     for (int k = 0; k < PT; k++) {
@@ -147,16 +288,22 @@ __global__ void __launch_bounds__(MAXT)
     }
 }
 
+static size_t fixed_lds_bytes(int nmax) {
+    return (size_t)STATE_ARRAYS * sizeof(float4) * nmax + (size_t)HDR_INTS * sizeof(int) * nmax +
+           (size_t)SCAN_INTS * sizeof(int);
+}
+
 bool resident_fits(const mte_engine *e, ResidentPlan *plan) {
     *plan = ResidentPlan{};
     if (e->P.tea_on) return false;
     int nmax = 0;
     for (int t = 0; t < e->ntraj; t++) nmax = std::max<int>(nmax, (int)(e->off[t + 1] - e->off[t]));
+    if (nmax >= 32768) return false;   // uint16 list entries
     int lds_limit = 0;
     if (hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device) != hipSuccess)
         return false;
-    const size_t bytes = (size_t)LDS_ARRAYS * sizeof(float4) * nmax;
-    if (bytes > (size_t)lds_limit) return false;
+    const size_t fixed = fixed_lds_bytes(nmax);
+    if (fixed + (size_t)MIN_ENTRIES_PER_BEAD * 2 * nmax > (size_t)lds_limit) return false;
     int pt = 1;
     if (nmax > RES_MAX_THREADS) pt = (nmax + RES_MULTI_THREADS - 1) / RES_MULTI_THREADS;
     if (pt > RES_MAX_PT) return false;
@@ -165,7 +312,8 @@ bool resident_fits(const mte_engine *e, ResidentPlan *plan) {
     plan->threads = threads;
     plan->per_thread = pt;
     plan->max_beads = nmax;
-    plan->lds_bytes = bytes;
+    plan->lds_bytes = (size_t)lds_limit & ~(size_t)15;
+    plan->entry_cap = (int)std::min<size_t>((plan->lds_bytes - fixed) / 2, 1u << 30);
     return true;
 }
 
@@ -176,7 +324,8 @@ static hipError_t launch_pt(mte_engine *e, int nsteps, int rb, int do_lj, int do
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds_bytes);
     if (s != hipSuccess) return s;
     hipLaunchKernelGGL((k_resident<PT, MAXT>), dim3(e->ntraj), dim3(pl.threads), pl.lds_bytes, e->stream, e->P,
-                       e->T, e->S[e->cur], e->seeds, noise, nsteps, rb, do_lj, do_bonded, e->L, pl.max_beads);
+                       e->T, e->S[e->cur], e->seeds, noise, nsteps, rb, do_lj, do_bonded, e->L, pl.max_beads,
+                       pl.entry_cap);
     return hipGetLastError();
 }
 
@@ -191,7 +340,6 @@ hipError_t launch_resident(mte_engine *e, int nsteps, bool rebuild_first, const 
     case 3: return launch_pt<3, RES_MULTI_THREADS>(e, nsteps, rb, do_lj, do_bonded, noise);
     default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 }  // namespace mte
