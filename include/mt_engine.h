@@ -153,7 +153,8 @@ int mte_tea_stats(mte_engine *e, double *eps, double *beta);
 
 /* Kernel timing with HIP events on the engine's stream. */
 int mte_set_timing(mte_engine *e, int32_t on);
-/* name: "step" (dominant kernel of mte_run), "rebuild", "tea", "forces".
+/* name: "step" (dominant kernel of plain mte_run), "rebuild", "forces",
+ * "tea_pairs" (the TEA pair-sum kernel).
  * Returns summed milliseconds and launch count since the last reset. */
 int mte_kernel_time(mte_engine *e, const char *name, double *total_ms, int64_t *launches);
 int mte_reset_timing(mte_engine *e);

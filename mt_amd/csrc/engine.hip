@@ -123,6 +123,7 @@ int mte_destroy(mte_engine *e) {
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     flush_timers(e);
+    for (auto &kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
     for (void *p : e->allocations) (void)hipFree(p);
     if (e->noise) (void)hipFree(e->noise);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -238,9 +239,20 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     }
     ALLOC(e->seeds, (size_t)2 * nb) ALLOC(e->fxyz, nb) ALLOC(e->fang, nb) ALLOC(e->en_bead, (size_t)7 * nb)
     ALLOC(e->en_traj, (size_t)7 * nt)
+    std::vector<int4> tea_units;
+    std::vector<int> tea_nchunks(nt, 0);
     if (e->P.tea_on) {
+        for (int t = 0; t < nt; t++) {
+            int c = 0;
+            for (int j0 = 0; j0 < tsize[t]; j0 += TEA_CHUNK_BEADS, c++)
+                for (int i0 = 0; i0 < tsize[t]; i0 += 256) tea_units.push_back(make_int4(t, i0, j0, c));
+            tea_nchunks[t] = c;
+            e->tea.max_chunks = std::max(e->tea.max_chunks, c);
+        }
+        e->tea.nunits = (int)tea_units.size();
         ALLOC(e->tea.ci, nb) ALLOC(e->tea.eps, nb) ALLOC(e->tea.beta, nt) ALLOC(e->tea.stats, 2 * nt)
-        ALLOC(e->tea.status, 1) ALLOC(e->tea.R, nb)
+        ALLOC(e->tea.status, 1) ALLOC(e->tea.R, nb) ALLOC(e->tea.units, tea_units.size())
+        ALLOC(e->tea.nchunks, nt) ALLOC(e->tea.partial, (size_t)e->tea.max_chunks * nb)
     }
 #undef ALLOC
     T.tid = p_tid; T.tbase = p_tbase; T.tsize = p_tsize; T.harm = p_harm; T.harm_raw = p_harm_raw;
@@ -272,6 +284,9 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     UP(p_fixed, fixed.data(), nb) UP(p_extra, extra.data(), nb) UP(p_mon, mon.data(), nb)
     UP(p_gtp, gtp.data(), nb) UP(p_ont, ont.data(), nb) UP(p_zs, zs.data(), nt)
     UP(e->tiles, tiles.data(), tiles.size()) UP(e->seeds, seeds.data(), seeds.size())
+    if (e->P.tea_on) {
+        UP(e->tea.units, tea_units.data(), tea_units.size()) UP(e->tea.nchunks, tea_nchunks.data(), nt)
+    }
 #undef UP
     if (hipMemsetAsync(T.lj_n, 0, sizeof(int) * nb, e->stream) != hipSuccess ||
         hipMemsetAsync(T.overflow, 0, sizeof(int), e->stream) != hipSuccess) {
@@ -411,6 +426,49 @@ static int check_tea(mte_engine *e) {
     return MTE_OK;
 }
 
+// Enqueue `len` streamed steps starting at global step s0 (list rebuilds and
+// TEA refreshes at their periods), flipping e->cur per step.  The same
+// sequence is launched directly or recorded into a graph.
+static hipError_t enqueue_streamed(mte_engine *e, int64_t s0, int64_t len, int64_t noise_cursor) {
+    const bool need_lists = e->P.lj_on || e->P.is_assembly;
+    hipError_t st = hipSuccess;
+    for (int64_t k = 0; k < len && st == hipSuccess; k++) {
+        const int64_t step = s0 + k;
+        if (need_lists && step % e->P.ljpairsupdatefreq == 0 && e->lists_step != step) {
+            hipEvent_t ev{};
+            time_begin(e, "rebuild", &ev);
+            st = launch_rebuild(e);
+            time_end(e, "rebuild", ev);
+            if (st != hipSuccess) break;
+        }
+        const float *nz = e->noise ? e->noise + (size_t)6 * e->nb * (noise_cursor + k) : nullptr;
+        hipEvent_t ev{};
+        if (e->P.tea_on) {
+            st = launch_forces_tea(e, nz);
+            if (st == hipSuccess && step % e->P.tea_epsilon_freq == 0) st = launch_tea_eps(e);
+            if (st != hipSuccess) break;
+            time_begin(e, "tea_pairs", &ev);
+            st = launch_tea_pairs(e);
+            time_end(e, "tea_pairs", ev);
+            if (st == hipSuccess) st = launch_tea_finish(e, nz);
+        } else {
+            time_begin(e, "step", &ev);
+            st = launch_step_plain(e, nz);
+            time_end(e, "step", ev);
+        }
+    }
+    return st;
+}
+
+// Steps from `step` to the next list rebuild or TEA refresh (exclusive), so
+// that a segment only carries events at its first step.
+static int64_t steps_to_next_event(const mte_engine *e, int64_t step) {
+    const int64_t f = e->P.ljpairsupdatefreq;
+    int64_t n = f - (step % f);
+    if (e->P.tea_on) n = std::min<int64_t>(n, e->P.tea_epsilon_freq - (step % e->P.tea_epsilon_freq));
+    return n;
+}
+
 int mte_run(mte_engine *e, int64_t nsteps) {
     if (!e || nsteps < 0) return MTE_ERR_ARG;
     HIPCHK(hipSetDevice(e->device));
@@ -436,34 +494,45 @@ int mte_run(mte_engine *e, int64_t nsteps) {
             done += chunk;
             continue;
         }
-        if (need_lists && rebuild_due(e)) {
-            int rc = rebuild_now(e);
-            if (rc) return rc;
-        }
-        const float *nz = e->noise ? e->noise + (size_t)6 * e->nb * e->noise_cursor : nullptr;
-        if (e->P.tea_on) {
-            hipEvent_t ev{};
-            time_begin(e, "forces", &ev);
-            HIPCHK(launch_forces(e));
-            time_end(e, "forces", ev);
-            if (e->step % e->P.tea_epsilon_freq == 0) {
-                HIPCHK(launch_tea_eps(e));
-                int rc = check_tea(e);
-                if (rc) return rc;
-                e->tea_valid = true;
+        const int64_t seg = std::min<int64_t>(nsteps - done, steps_to_next_event(e, e->step));
+        const bool tea_refresh = e->P.tea_on && e->step % e->P.tea_epsilon_freq == 0;
+        const bool rebuilds = need_lists && rebuild_due(e);
+        if (e->use_graphs && !e->noise && !e->timing) {
+            // The rebuild-at-first-step decision is part of the key: a segment
+            // starting on a rebuild step whose lists already exist skips it.
+            const int64_t lj_phase = rebuilds ? e->step % freq : (e->step % freq == 0 ? -1 : e->step % freq);
+            const auto key = std::make_tuple(lj_phase, e->P.tea_on ? e->step % e->P.tea_epsilon_freq : 0, seg,
+                                             e->cur);
+            auto it = e->graphs.find(key);
+            const int cur0 = e->cur;
+            if (it == e->graphs.end()) {
+                hipGraph_t g;
+                HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+                const hipError_t st = enqueue_streamed(e, e->step, seg, 0);
+                const hipError_t st2 = hipStreamEndCapture(e->stream, &g);
+                e->cur = cur0;
+                HIPCHK(st);
+                HIPCHK(st2);
+                hipGraphExec_t ex;
+                const hipError_t st3 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                HIPCHK(st3);
+                it = e->graphs.emplace(key, ex).first;
             }
-            time_begin(e, "tea", &ev);
-            HIPCHK(launch_tea_step(e, nz));
-            time_end(e, "tea", ev);
+            HIPCHK(hipGraphLaunch(it->second, e->stream));
+            e->cur = cur0 ^ (int)(seg & 1);
         } else {
-            hipEvent_t ev{};
-            time_begin(e, "step", &ev);
-            HIPCHK(launch_step_plain(e, nz));
-            time_end(e, "step", ev);
+            HIPCHK(enqueue_streamed(e, e->step, seg, e->noise_cursor));
         }
-        if (e->noise) e->noise_cursor++;
-        e->step++;
-        done++;
+        if (rebuilds) { e->lists_step = e->step; e->lists_valid = true; }
+        if (tea_refresh) {
+            int rc = check_tea(e);
+            if (rc) return rc;
+            e->tea_valid = true;
+        }
+        if (e->noise) e->noise_cursor += seg;
+        e->step += seg;
+        done += seg;
     }
     HIPCHK(hipStreamSynchronize(e->stream));
     flush_timers(e);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -24,7 +25,15 @@ struct TeaBuffers {
     double *stats = nullptr;   // per trajectory: eps, beta (FP64)
     int *status = nullptr;     // first capricious failure (1 + traj) or 0
     float4 *R = nullptr;       // per bead: scaled random force
+    int4 *units = nullptr;     // pair work units: traj, i0, j0, chunk index
+    int nunits = 0;
+    int *nchunks = nullptr;    // per trajectory
+    int max_chunks = 0;
+    float4 *partial = nullptr; // max_chunks x nb pair partial sums
 };
+
+// j-chunk length of the TEA pair kernel.
+constexpr int TEA_CHUNK_BEADS = 128;
 
 // Resident-mode launch plan: trajectories grouped by workgroup.
 struct ResidentPlan {
@@ -68,6 +77,10 @@ struct mte_engine {
     bool tea_valid = false;
 
     bool timing = false;
+    // Captured step sequences of the streamed path, keyed by
+    // (step mod rebuild period, step mod epsilon period, length, buffer parity).
+    std::map<std::tuple<int64_t, int64_t, int64_t, int>, hipGraphExec_t> graphs;
+    bool use_graphs = true;
     std::map<std::string, mte::TimingSlot> timers;
     std::vector<void *> allocations;
     std::string err;
@@ -79,10 +92,12 @@ namespace mte {
 hipError_t launch_frames(mte_engine *e, int buf);
 hipError_t launch_step_plain(mte_engine *e, const float *noise_step);
 hipError_t launch_forces(mte_engine *e);
+hipError_t launch_forces_tea(mte_engine *e, const float *noise_step);
 hipError_t launch_energies(mte_engine *e);
 hipError_t launch_rebuild(mte_engine *e);
 hipError_t launch_tea_eps(mte_engine *e);
-hipError_t launch_tea_step(mte_engine *e, const float *noise_step);
+hipError_t launch_tea_pairs(mte_engine *e);
+hipError_t launch_tea_finish(mte_engine *e, const float *noise_step);
 hipError_t launch_resident(mte_engine *e, int nsteps, bool rebuild_first, const float *noise);
 bool resident_fits(const mte_engine *e, ResidentPlan *plan);
 

@@ -77,9 +77,19 @@ __global__ void __launch_bounds__(STREAM_BLOCK)
     Sout.fp2[b] = p2;
 }
 
-// Forces only (compute_kernel semantics; extra beads get zero).
+// Random-force draw for TEA steps, fused into the force pass: every bead
+// draws sqrt(2 kB T gammaR / dt) xi from its xyz stream
+// (integrateTea_prepare, src/bdhitea_kernel.cu:16-36).
+struct TeaDraw {
+    uint4 *seeds;
+    const float *noise;
+    float4 *R;   // nullptr: no draw
+};
+
+// Forces (compute_kernel semantics; extra beads get zero), optionally with
+// the TEA random forces.
 __global__ void __launch_bounds__(STREAM_BLOCK)
-    k_forces(mte_params P, DevTopo T, DevState S, float4 *fxyz, float4 *fang) {
+    k_forces(mte_params P, DevTopo T, DevState S, float4 *fxyz, float4 *fang, TeaDraw D) {
     const int b = blockIdx.x * STREAM_BLOCK + threadIdx.x;
     if (b >= T.nb) return;
     float F[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -89,6 +99,19 @@ __global__ void __launch_bounds__(STREAM_BLOCK)
     }
     fxyz[b] = make_float4(F[0], F[1], F[2], 0.f);
     fang[b] = make_float4(F[3], F[4], F[5], 0.f);   // fi, theta, psi
+    if (D.R) {
+        const float var = sqrtf(2.0f * KB * P.temp * P.gammaR / P.dt);
+        float4 r;
+        if (D.noise) {
+            const float *nz = D.noise + 6 * (size_t)b;
+            r = make_float4(nz[0], nz[1], nz[2], 0.f);
+        } else {
+            uint4 s = D.seeds[b];
+            r = normals4(s);
+            D.seeds[b] = s;
+        }
+        D.R[b] = make_float4(r.x * var, r.y * var, r.z * var, 0.f);
+    }
 }
 
 __global__ void __launch_bounds__(STREAM_BLOCK)
@@ -139,7 +162,13 @@ hipError_t launch_step_plain(mte_engine *e, const float *noise_step) {
 
 hipError_t launch_forces(mte_engine *e) {
     hipLaunchKernelGGL(k_forces, dim3(grid_for(e->nb)), dim3(STREAM_BLOCK), 0, e->stream, e->P, e->T, e->S[e->cur],
-                       e->fxyz, e->fang);
+                       e->fxyz, e->fang, TeaDraw{nullptr, nullptr, nullptr});
+    return hipGetLastError();
+}
+
+hipError_t launch_forces_tea(mte_engine *e, const float *noise_step) {
+    hipLaunchKernelGGL(k_forces, dim3(grid_for(e->nb)), dim3(STREAM_BLOCK), 0, e->stream, e->P, e->T, e->S[e->cur],
+                       e->fxyz, e->fang, TeaDraw{e->seeds, noise_step, e->tea.R});
     return hipGetLastError();
 }
 

@@ -11,15 +11,17 @@
 // with D_hat_ij = D_ij/D_ii the Rotne-Prager-Yamakawa block, N the count of
 // non-extra beads of the trajectory.
 //
-// Work layout: one workgroup per (trajectory, 256-bead tile of i), the j range
-// of the trajectory streamed through LDS (position + extra flag, force, random
-// force) in 256-bead tiles; each (i, j) block costs one rsqrt and ~25 FP32
-// operations, so the product is VALU-bound.
+// Work layout of the per-step product: work units of (trajectory, 256-bead i
+// group, 128-bead j chunk), the chunk's positions, forces and random forces in
+// LDS, FP32 partial sums per chunk reduced in fixed order by the finish kernel,
+// which adds the self term and integrates.  Each (i, j) block costs one rsqrt
+// and ~35 FP32 operations: the product is VALU-bound.
 #include "engine_internal.h"
 
 namespace mte {
 
 constexpr int TEA_TILE = 256;
+constexpr int TEA_CHUNK = TEA_CHUNK_BEADS;
 
 struct RpyCoef { float crr, cii; };
 
@@ -135,93 +137,94 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// Forces for TEA plus the random forces of every bead
-// (integrateTea_prepare, src/bdhitea_kernel.cu:16-36: all beads draw).
-struct TeaNoiseArgs {
-    uint4 *seeds;
-    const float *noise;
-    float4 *R;
-};
-
+// Pair sums of one (trajectory, 256-bead i group, j chunk) work unit:
+//   partial_i = sum_{j in chunk, j != i, non-extra} D_hat_ij (F_j + beta C_i . R_j)
+// The chunk's j data sit in LDS; extra beads enter with zero force and
+// noise at a far position, so they contribute nothing and need no branch.
+// Splitting j into chunks gives ~10 work units per SIMD at 64 x 1040 beads,
+// against ~1 per SIMD for one thread per bead over the whole trajectory.
 __global__ void __launch_bounds__(TEA_TILE)
-    k_tea_noise(mte_params P, int nb, TeaNoiseArgs A) {
-    const int b = blockIdx.x * TEA_TILE + threadIdx.x;
-    if (b >= nb) return;
-    const float var = sqrtf(2.0f * KB * P.temp * P.gammaR / P.dt);
-    float4 r;
-    if (A.noise) {
-        const float *nz = A.noise + 6 * (size_t)b;
-        r = make_float4(nz[0], nz[1], nz[2], 0.f);
-    } else {
-        uint4 s = A.seeds[b];
-        r = normals4(s);
-        A.seeds[b] = s;
+    k_tea_pairs(mte_params P, DevTopo T, DevState S, const float4 *fxyz, const float4 *R, const float4 *ci,
+                const float *beta, const int4 *units, int chunk_stride, float4 *partial) {
+    __shared__ float4 sx[TEA_CHUNK], sf[TEA_CHUNK], sr[TEA_CHUNK];
+    const int4 u = units[blockIdx.x];   // traj, i0, j0, chunk index
+    const int base = T.tbase[u.x], n = T.tsize[u.x];
+    const int jn = min(TEA_CHUNK, n - u.z);
+    for (int q = threadIdx.x; q < jn; q += TEA_TILE) {
+        const int jb = base + u.z + q;
+        float4 x = S.xyz[jb];
+        float4 f = fxyz[jb], r = R[jb];
+        if (T.extra[jb]) {
+            x = make_float4(3.0e18f, 3.0e18f, 3.0e18f, 0.f);
+            f = r = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        sx[q] = x;
+        sf[q] = f;
+        sr[q] = r;
     }
-    A.R[b] = make_float4(r.x * var, r.y * var, r.z * var, 0.f);
+    __syncthreads();
+    const int li = u.y + threadIdx.x;
+    if (li >= n) return;
+    const int b = base + li;
+    float3 acc = make_float3(0.f, 0.f, 0.f);
+    if (!T.extra[b]) {
+        const float3 ri = f3(S.xyz[b]);
+        const float bt = beta[u.x];
+        const float4 cc = ci[b];
+        const float w = bt * bt;
+        const float cx = bt * rsq_fast(1.f + w * cc.x), cy = bt * rsq_fast(1.f + w * cc.y),
+                    cz = bt * rsq_fast(1.f + w * cc.z);
+        const float a = P.tea_a, ia = rcp_fast(P.tea_a);
+        const int self = li - u.z;   // position of i in this chunk, if any
+        for (int q = 0; q < jn; q++) {
+            const float4 xj = sx[q], Fj = sf[q], Rj = sr[q];
+            const float dx = xj.x - ri.x, dy = xj.y - ri.y, dz = xj.z - ri.z;
+            const bool is_self = q == self;
+            const float r2 = is_self ? 1.f : dx * dx + dy * dy + dz * dz;   // keeps the j == i slot finite
+            const float rinv = rsq_fast(r2);
+            const float r = r2 * rinv;
+            // D_hat = cii I + crr e e^T (src/bdhitea_kernel.cu:38-58); both branches, then select
+            const float u1 = a * rinv, u2 = u1 * u1;
+            const float far_ii = 0.75f * u1 * fmaf(0.66666666666666667f, u2, 1.f);
+            const float far_rr = 0.75f * u1 * fmaf(-2.f, u2, 1.f);
+            const float ra = r * ia;
+            const bool near = !(ra > 2.f);
+            float cii = near ? fmaf(-0.28125f, ra, 1.f) : far_ii;
+            float crr = near ? 0.09375f * ra : far_rr;
+            if (is_self) cii = crr = 0.f;
+            const float vx = fmaf(cx, Rj.x, Fj.x), vy = fmaf(cy, Rj.y, Fj.y), vz = fmaf(cz, Rj.z, Fj.z);
+            const float s = (crr * rinv * rinv) * (dx * vx + dy * vy + dz * vz);
+            acc.x += fmaf(cii, vx, s * dx);
+            acc.y += fmaf(cii, vy, s * dy);
+            acc.z += fmaf(cii, vz, s * dz);
+        }
+    }
+    partial[(size_t)u.w * chunk_stride + b] = make_float4(acc.x, acc.y, acc.z, 0.f);
 }
 
-// One TEA integration step (integrateTea_kernel_unlisted,
-// src/bdhitea_kernel.cu:148-213).
+// Self term, chunk partials in fixed order, and the update of one bead
+// (integrateTea_kernel_unlisted, src/bdhitea_kernel.cu:148-213).
 __global__ void __launch_bounds__(TEA_TILE)
-    k_tea_step(mte_params P, DevTopo T, DevState Sin, DevState Sout, const float4 *fxyz, const float4 *fang,
-               const float4 *R, const float4 *ci, const float *beta, uint4 *seeds, const float *noise,
-               const int2 *tiles, LatOffsets L) {
-    __shared__ float4 sx[TEA_TILE], sf[TEA_TILE], sr[TEA_TILE];
-    const int2 tl = tiles[blockIdx.x];
-    const int t = tl.x;
-    const int base = T.tbase[t], n = T.tsize[t];
-    const int li = tl.y + threadIdx.x;
-    const bool active = li < n;
-    const int b = base + li;
+    k_tea_finish(mte_params P, DevTopo T, DevState Sin, DevState Sout, const float4 *fxyz, const float4 *fang,
+                 const float4 *R, const float4 *ci, const float *beta, const int *nchunks, int chunk_stride,
+                 const float4 *partial, uint4 *seeds, const float *noise, LatOffsets L) {
+    const int b = blockIdx.x * TEA_TILE + threadIdx.x;
+    if (b >= T.nb) return;
+    const int t = T.tid[b];
     const float bt = beta[t];
-    float3 ri = make_float3(0, 0, 0), f = ri, c = ri;
-    bool skip = true;
-    if (active) {
-        ri = f3(Sin.xyz[b]);
-        const float4 cc = ci[b];
-        const float4 Fi = fxyz[b], Ri = R[b];
-        const float w = bt * bt;
-        const float cx = 1.f / sqrtf(1.f + w * cc.x), cy = 1.f / sqrtf(1.f + w * cc.y),
-                    cz = 1.f / sqrtf(1.f + w * cc.z);
-        f = make_float3(Fi.x + Ri.x * cx, Fi.y + Ri.y * cy, Fi.z + Ri.z * cz);
-        c = make_float3(cx * bt, cy * bt, cz * bt);
-        skip = T.extra[b] != 0;
+    const float4 cc = ci[b], Fi = fxyz[b], Ri = R[b];
+    const float w = bt * bt;
+    // C_i = 1/sqrt(1 + beta^2 ci) per row (eq. 19)
+    const float cx = rsq_fast(1.f + w * cc.x), cy = rsq_fast(1.f + w * cc.y), cz = rsq_fast(1.f + w * cc.z);
+    float fx = fmaf(Ri.x, cx, Fi.x), fy = fmaf(Ri.y, cy, Fi.y), fz = fmaf(Ri.z, cz, Fi.z);
+    const int nc = nchunks[t];
+    for (int c = 0; c < nc; c++) {
+        const float4 pp = partial[(size_t)c * chunk_stride + b];
+        fx += pp.x;
+        fy += pp.y;
+        fz += pp.z;
     }
-    const float a = P.tea_a;
-    for (int j0 = 0; j0 < n; j0 += TEA_TILE) {
-        const int jl = j0 + threadIdx.x;
-        if (jl < n) {
-            float4 v = Sin.xyz[base + jl];
-            v.w = T.extra[base + jl] ? 1.f : 0.f;
-            sx[threadIdx.x] = v;
-            sf[threadIdx.x] = fxyz[base + jl];
-            sr[threadIdx.x] = R[base + jl];
-        }
-        __syncthreads();
-        const int m = min(TEA_TILE, n - j0);
-        if (active && !skip) {
-            for (int jj = 0; jj < m; jj++) {
-                const float4 xj = sx[jj];
-                if (j0 + jj == li || xj.w != 0.f) continue;
-                const float4 Fj = sf[jj], Rj = sr[jj];
-                const float dx = xj.x - ri.x, dy = xj.y - ri.y, dz = xj.z - ri.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                const float rinv = rsqrtf(r2);
-                const float r = r2 * rinv;
-                const RpyCoef k = rpy_coef(r, rinv, a);
-                const float vx = Fj.x + c.x * Rj.x, vy = Fj.y + c.y * Rj.y, vz = Fj.z + c.z * Rj.z;
-                const float s = k.crr * (rinv * rinv) * (dx * vx + dy * vy + dz * vz);
-                f.x += k.cii * vx + s * dx;
-                f.y += k.cii * vy + s * dy;
-                f.z += k.cii * vz + s * dz;
-            }
-        }
-        __syncthreads();
-    }
-    if (!active) return;
     float4 x = Sin.xyz[b], an = Sin.ang[b];
-    const float mult = P.dt / P.gammaR;
-    const float nx = ri.x + mult * f.x, ny = ri.y + mult * f.y, nz = ri.z + mult * f.z;
     float4 ra;
     if (noise) {
         const float *q = noise + 6 * (size_t)b;
@@ -233,9 +236,10 @@ __global__ void __launch_bounds__(TEA_TILE)
     }
     float4 n4 = Sin.fn[b], p1 = Sin.fp1[b], p2 = Sin.fp2[b];
     if (!T.fixed[b] && !T.extra[b]) {
-        x.x = nx;
-        x.y = ny;
-        x.z = nz;
+        const float mult = P.dt / P.gammaR;
+        x.x = fmaf(mult, fx, x.x);
+        x.y = fmaf(mult, fy, x.y);
+        x.z = fmaf(mult, fz, x.z);
         const float4 fa = fang[b];
         const float dta = P.dt / (P.gammaTheta * P.alpha), dtt = P.dt / P.gammaTheta;
         const float vta = P.varTheta * sqrtf(P.freeze_temp / P.alpha);
@@ -259,13 +263,16 @@ hipError_t launch_tea_eps(mte_engine *e) {
     return hipGetLastError();
 }
 
-hipError_t launch_tea_step(mte_engine *e, const float *noise_step) {
-    TeaNoiseArgs A{e->seeds, noise_step, e->tea.R};
-    hipLaunchKernelGGL(k_tea_noise, dim3((e->nb + TEA_TILE - 1) / TEA_TILE), dim3(TEA_TILE), 0, e->stream, e->P,
-                       e->nb, A);
-    hipLaunchKernelGGL(k_tea_step, dim3(e->ntiles), dim3(TEA_TILE), 0, e->stream, e->P, e->T, e->S[e->cur],
-                       e->S[1 - e->cur], e->fxyz, e->fang, e->tea.R, e->tea.ci, e->tea.beta, e->seeds, noise_step,
-                       e->tiles, e->L);
+hipError_t launch_tea_pairs(mte_engine *e) {
+    hipLaunchKernelGGL(k_tea_pairs, dim3(e->tea.nunits), dim3(TEA_TILE), 0, e->stream, e->P, e->T, e->S[e->cur],
+                       e->fxyz, e->tea.R, e->tea.ci, e->tea.beta, e->tea.units, e->nb, e->tea.partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_tea_finish(mte_engine *e, const float *noise_step) {
+    hipLaunchKernelGGL(k_tea_finish, dim3((e->nb + TEA_TILE - 1) / TEA_TILE), dim3(TEA_TILE), 0, e->stream, e->P,
+                       e->T, e->S[e->cur], e->S[1 - e->cur], e->fxyz, e->fang, e->tea.R, e->tea.ci, e->tea.beta,
+                       e->tea.nchunks, e->nb, e->tea.partial, e->seeds, noise_step, e->L);
     e->cur = 1 - e->cur;
     return hipGetLastError();
 }

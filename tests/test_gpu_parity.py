@@ -157,3 +157,51 @@ def test_resident_rng_and_uneven_batch():
     print("resident uneven max |dx|", dx, flush=True)
     assert dx < 1e-3
     eng.close()
+
+
+def test_tea_drift_against_fp64_exact_em():
+    print("case TEA drift vs FP64 exact EM (312 beads)", flush=True)
+    from mt_amd import params as P
+    from oracle import em_exact
+    # dt = 1000 ps: the zero-noise step is ~10 nm, resolvable in float32 positions
+    p = P.derive(dict(P.TEA, tea_epsilon_freq=1, dt=1000.0))
+    b = c1_batch()
+    eng = Engine(p, b.copy(), mode=MODE_STREAMED)
+    orc = Oracle(p, b.copy())
+    orc.rebuild()
+    F = orc.forces()
+    eng.set_noise(np.zeros((1, b.nb, 6), np.float32))
+    c0 = b.coords.astype(np.float64)
+    eng.run(1)
+    g = eng.get_state().astype(np.float64)
+    mob = (b.fixed == 0) & (b.extra == 0)
+    exact = em_exact.drift(c0[:, :3], F[:, :3], p.tea_a, p.dt, p.gammaR)
+    d_gpu = g[:, :3] - c0[:, :3]
+    err = np.abs(d_gpu[mob] - exact[mob]).max() / np.abs(exact[mob]).max()
+    print("TEA drift relative error vs FP64", err, flush=True)
+    assert err < 1e-4
+    eng.close()
+
+
+def test_tea_batch_with_extra_beads_matches_oracle():
+    print("case TEA uneven batch with extra beads", flush=True)
+    from mt_amd import params as P
+    p = P.derive(dict(P.TEA, tea_epsilon_freq=3))
+    b = uneven_batch()
+    b.extra[b.mon_type == 5] = 0
+    # mark the free dimers of trajectory 1 as reservoir (extra) beads
+    s, e = int(b.off[1]), int(b.off[2])
+    n_mt = 13 * 2 * 5
+    b.extra[s + n_mt:e] = 1
+    eng, orc = _pair(p, b, precision="f64")
+    rng = np.random.default_rng(8)
+    nz = noise(rng, 6, b.nb)
+    eng.set_noise(nz)
+    eng.run(6)
+    assert orc.run(0, 6, noise=nz) == 0
+    eps, beta = eng.tea_stats()
+    np.testing.assert_allclose(eps, orc.eps_t, rtol=1e-5)
+    np.testing.assert_allclose(beta, orc.beta_t, rtol=1e-5)
+    g = eng.get_state()
+    assert np.abs(g[:, :3] - orc.b.coords[:, :3]).max() < 1e-4
+    eng.close()

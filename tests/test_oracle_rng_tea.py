@@ -108,3 +108,34 @@ def test_tea_far_apart_reduces_to_free_diffusion():
     assert o.run(0, 1, noise=nz) == 0
     dx = o.b.coords[0, :3].astype(np.float64)       # bead 0 starts at the origin
     np.testing.assert_allclose(dx, p.varR * nz[0, 0, :3], rtol=2e-4, atol=1e-7)
+
+
+def test_exact_em_tensor_properties():
+    from oracle import em_exact
+    from tests.helpers import c1_batch
+    pos = c1_batch().coords[:, :3].astype(np.float64)
+    D = em_exact.rpy_hat(pos, 2.0)
+    assert np.abs(D - D.T).max() < 1e-12
+    assert np.linalg.eigvalsh(D).min() > 0          # RPY is positive definite
+    cii, crr = _rpy_textbook(2.0, float(np.linalg.norm(pos[1] - pos[0])))
+    e = (pos[1] - pos[0]) / np.linalg.norm(pos[1] - pos[0])
+    np.testing.assert_allclose(D[0:3, 3:6], cii * np.eye(3) + crr * np.outer(e, e), rtol=1e-12)
+
+
+def test_oracle_tea_drift_is_exact_em_drift():
+    # TEA's deterministic part equals the exact EM drift (dt/gammaR) D_hat F.
+    from oracle import em_exact
+    from tests.helpers import c1_batch
+    # dt = 1000 ps makes the zero-noise step ~10 nm, so float32 positions resolve
+    # the displacement to ~1e-6 (the drift is linear in dt)
+    p = params.derive(dict(params.TEA, tea_epsilon_freq=1, dt=1000.0))
+    b = c1_batch()
+    o = Oracle(p, b.copy())
+    o.rebuild()
+    F = o.forces()
+    c0 = b.coords.astype(np.float64)
+    assert o.run(0, 1, noise=np.zeros((1, b.nb, 6), np.float32)) == 0
+    mob = (b.fixed == 0) & (b.extra == 0)
+    exact = em_exact.drift(c0[:, :3], F[:, :3], p.tea_a, p.dt, p.gammaR)
+    d = o.b.coords[:, :3].astype(np.float64) - c0[:, :3]
+    assert np.abs(d[mob] - exact[mob]).max() / np.abs(exact[mob]).max() < 5e-6
