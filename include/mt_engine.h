@@ -93,6 +93,8 @@ typedef struct {
     const uint8_t *fixed, *extra;       /* B */
     const int32_t *mon_type, *gtp, *on_tubule; /* B */
     const float *wall_z;                /* ntraj: Parameters.zs */
+    const float *box;                   /* ntraj: periodic cubic box side, 0 = open (the
+                                           reference has open boundaries only); may be NULL */
 } mte_topology;
 
 #define MTE_MODE_AUTO 0
@@ -104,7 +106,12 @@ typedef struct {
     int32_t mode;     /* MTE_MODE_* */
     int32_t rseed;    /* HybridTaus seeding via ran2 (src/HybridTaus.cu:32-48) */
     int32_t device;   /* HIP device ordinal */
+    int32_t list_method; /* MTE_LISTS_*: how streamed-mode rebuilds find candidates */
 } mte_options;
+
+#define MTE_LISTS_AUTO 0      /* cell lists for trajectories above 2048 beads */
+#define MTE_LISTS_ALLPAIRS 1  /* every bead of the trajectory, LDS-tiled */
+#define MTE_LISTS_CELLS 2     /* hashed cell lists, sort and scan */
 
 typedef struct mte_engine mte_engine;
 

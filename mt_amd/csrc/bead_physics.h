@@ -16,6 +16,7 @@ struct Own {
     float3 r, n, a1, a2, uphi;
     float2 uth;
     float fi, theta, psi;
+    float box;   // periodic side of the bead's trajectory (0: open)
 };
 
 // trig = (sin theta, cos theta, sin psi, cos psi) of the bead.
@@ -30,6 +31,7 @@ __device__ __forceinline__ Own make_own_trig(float4 xyz, float4 ang, float4 fn, 
     o.psi = ang.z;
     o.uphi = make_float3(trig.w * trig.y, trig.z * trig.y, -trig.x);
     o.uth = make_float2(-trig.z, trig.w);
+    o.box = 0.f;
     return o;
 }
 
@@ -47,6 +49,7 @@ __device__ __forceinline__ Own make_own(float4 xyz, float4 ang, float4 fn, float
     sincos_fast(ang.z, &sp, &cp);
     o.uphi = make_float3(cp * ct, sp * ct, -st);
     o.uth = make_float2(-sp, cp);
+    o.box = 0.f;
     return o;
 }
 
@@ -89,7 +92,7 @@ __device__ __forceinline__ void bead_force(const mte_params &P, const DevTopo &T
         const int j = ent_j(h);
         const float rm = ent_f(h) ? RMON : -RMON;
         const float3 nj = f3(A.fn(j));
-        const Grad G = gap_grad(I.r, scale3(rm, I.n), f3(A.xyz(j)), scale3(-rm, nj), I.uphi, I.uth);
+        const Grad G = gap_grad(min_image(sub3(I.r, f3(A.xyz(j))), I.box), scale3(rm, I.n), scale3(-rm, nj), I.uphi, I.uth);
 #pragma unroll
         for (int q = 0; q < 6; q++) F[q] -= P.C * G.g[q];
         if (G.dr < ANGLE_CUT) {
@@ -104,7 +107,7 @@ __device__ __forceinline__ void bead_force(const mte_params &P, const DevTopo &T
         const int j = ent_j(e);
         const float rm = ent_f(e) ? RMON : -RMON;
         const float4 xj = A.xyz(j);
-        const Grad G = gap_grad(I.r, scale3(rm, I.n), f3(xj), scale3(-rm, f3(A.fn(j))), I.uphi, I.uth);
+        const Grad G = gap_grad(min_image(sub3(I.r, f3(xj)), I.box), scale3(rm, I.n), scale3(-rm, f3(A.fn(j))), I.uphi, I.uth);
         float dUdr = (G.dr == 0.f) ? 0.f : dmorse(P.D_long, P.A_long, G.dr) * G.inv;
         if (P.barrier && A.offtub(b) && A.offtub(j) && G.dr != 0.f)
             dUdr += dbarr(P.a_barr_long, P.r_barr_long, P.w_barr_long, G.dr) * G.inv;
@@ -124,7 +127,7 @@ __device__ __forceinline__ void bead_force(const mte_params &P, const DevTopo &T
         const int sw = ent_f(e);
         const float3 a = sw ? I.a1 : I.a2;
         const float3 bj = f3(sw ? A.fp2(j) : A.fp1(j));
-        const Grad G = gap_grad(I.r, a, f3(A.xyz(j)), bj, I.uphi, I.uth);
+        const Grad G = gap_grad(min_image(sub3(I.r, f3(A.xyz(j))), I.box), a, bj, I.uphi, I.uth);
         float dUdr;
         if (G.dr == 0.f) dUdr = 0.f;
         else if (A.mon(b) != A.mon(j)) dUdr = dmorse(P.D_lat / P.seam_coeff, P.A_lat, G.dr) * G.inv;
@@ -140,8 +143,8 @@ __device__ __forceinline__ void bead_force(const mte_params &P, const DevTopo &T
         const float s0 = P.ljscale * P.ljsigma6;
         for (int k = 0; k < nlj; k++) {
             const int j = Ls.lj(k);
-            const float3 rj = f3(A.xyz(j));
-            const float dx = I.r.x - rj.x, dy = I.r.y - rj.y, dz = I.r.z - rj.z;
+            const float3 d = min_image(sub3(I.r, f3(A.xyz(j))), I.box);
+            const float dx = d.x, dy = d.y, dz = d.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             if (r2 < LJ_CUT * LJ_CUT) {
                 const float inv = rcp_fast(r2);
@@ -180,7 +183,7 @@ __device__ __forceinline__ void bead_energy(const mte_params &P, const DevTopo &
     if (h >= 0) {
         const int j = ent_j(h);
         const float rm = ent_f(h) ? RMON : -RMON;
-        const Grad G = gap_grad(I.r, scale3(rm, I.n), f3(A.xyz(j)), scale3(-rm, f3(A.fn(j))), I.uphi, I.uth);
+        const Grad G = gap_grad(min_image(sub3(I.r, f3(A.xyz(j))), I.box), scale3(rm, I.n), scale3(-rm, f3(A.fn(j))), I.uphi, I.uth);
         U[0] += (P.C / 2.f) * G.dr * G.dr;
         if (G.dr < ANGLE_CUT) {
             const float4 aj = A.ang(j);
@@ -196,7 +199,7 @@ __device__ __forceinline__ void bead_energy(const mte_params &P, const DevTopo &
         const int j = ent_j(e);
         const float rm = ent_f(e) ? RMON : -RMON;
         const float4 xj = A.xyz(j);
-        const Grad G = gap_grad(I.r, scale3(rm, I.n), f3(xj), scale3(-rm, f3(A.fn(j))), I.uphi, I.uth);
+        const Grad G = gap_grad(min_image(sub3(I.r, f3(xj)), I.box), scale3(rm, I.n), scale3(-rm, f3(A.fn(j))), I.uphi, I.uth);
         U[1] += morse_en(P.D_long, P.A_long, G.dr);
         if (P.barrier && A.offtub(b) && A.offtub(j))
             U[1] += barr(P.a_barr_long, P.r_barr_long, P.w_barr_long, G.dr);
@@ -215,7 +218,8 @@ __device__ __forceinline__ void bead_energy(const mte_params &P, const DevTopo &
         const int e = T.lat[k * nb + b];
         const int j = ent_j(e);
         const int sw = ent_f(e);
-        const Grad G = gap_grad(I.r, sw ? I.a1 : I.a2, f3(A.xyz(j)), f3(sw ? A.fp2(j) : A.fp1(j)), I.uphi, I.uth);
+        const Grad G = gap_grad(min_image(sub3(I.r, f3(A.xyz(j))), I.box), sw ? I.a1 : I.a2, f3(sw ? A.fp2(j) : A.fp1(j)),
+                                I.uphi, I.uth);
         if (A.mon(b) != A.mon(j)) U[2] += morse_en(P.D_lat / P.seam_coeff, P.A_lat, G.dr);
         else U[2] += morse_en(P.D_lat, P.A_lat, G.dr);
         if (P.barrier && A.offtub(b) && A.offtub(j))
@@ -225,8 +229,8 @@ __device__ __forceinline__ void bead_energy(const mte_params &P, const DevTopo &
         const int nlj = T.lj_n[b];
         for (int k = 0; k < nlj; k++) {
             const int j = T.lj[k * nb + b];
-            const float3 rj = f3(A.xyz(j));
-            const float dx = I.r.x - rj.x, dy = I.r.y - rj.y, dz = I.r.z - rj.z;
+            const float3 d = min_image(sub3(I.r, f3(A.xyz(j))), I.box);
+            const float dx = d.x, dy = d.y, dz = d.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             if (sqrtf(r2) < LJ_CUT) {
                 const float inv = 1.f / r2;

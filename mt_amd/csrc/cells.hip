@@ -1,0 +1,206 @@
+// cells.hip -- cell-list neighbour search (sort and scan) for list rebuilds
+// of large trajectories.
+//
+// The reference rebuilds its lists by scanning every bead of the trajectory
+// for every bead (pairs_kernel, src/compute_cuda.cu:527-674; LJ_kernel,
+// :913-940), O(N^2) per trajectory.  Here the rebuild is:
+//   1. assign: cell coordinates floor(x / cs) per bead (wrapped in periodic
+//      boxes), hashed into a per-trajectory table of 2^k buckets; count;
+//   2. scan: one workgroup per trajectory turns bucket counts into offsets;
+//   3. scatter: beads into bucket order (counting sort);
+//   4. search: each bead visits the 27 neighbouring cells, keeps the beads of
+//      exactly those cells within the candidate radius, sorts the candidates
+//      by index and runs the reference's per-pair tests on them in that
+//      order, so the lists come out identical to the all-pairs scan (same
+//      entries, same order).
+// A hash collision only adds beads that the exact cell check then rejects.
+#include "engine_internal.h"
+#include "rebuild_common.h"
+
+namespace mte {
+
+constexpr int CELL_BLOCK = 64;
+constexpr int CAND_CAP = 128;   // candidates per bead within the candidate radius
+
+__device__ __forceinline__ uint32_t cell_hash(int cx, int cy, int cz) {
+    return ((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u);
+}
+
+struct CellGeom {
+    float cs;      // cell side
+    int dims;      // cells per side in a periodic box (0: open)
+};
+
+__device__ __forceinline__ CellGeom cell_geom(float box, float radius) {
+    CellGeom g;
+    if (box > 0.f) {
+        g.dims = max(3, (int)floorf(box / radius));
+        g.cs = box / (float)g.dims;
+    } else {
+        g.dims = 0;
+        g.cs = radius;
+    }
+    return g;
+}
+
+__device__ __forceinline__ int3 cell_of(float3 r, float box, const CellGeom &g) {
+    if (g.dims > 0) {
+        const float wx = r.x - box * floorf(r.x / box), wy = r.y - box * floorf(r.y / box),
+                    wz = r.z - box * floorf(r.z / box);
+        return make_int3(min(g.dims - 1, (int)(wx / g.cs)), min(g.dims - 1, (int)(wy / g.cs)),
+                         min(g.dims - 1, (int)(wz / g.cs)));
+    }
+    return make_int3((int)floorf(r.x / g.cs), (int)floorf(r.y / g.cs), (int)floorf(r.z / g.cs));
+}
+
+__global__ void __launch_bounds__(256) k_fill_int(int *p, int n, int v) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void __launch_bounds__(256)
+    k_cell_assign(DevTopo T, DevState S, CellTables C, float radius) {
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= T.nb) return;
+    const int t = T.tid[b];
+    const float box = T.tbox[t];
+    const CellGeom g = cell_geom(box, radius);
+    const int3 c = cell_of(f3(S.xyz[b]), box, g);
+    const int bucket = C.toff[t] + (int)(cell_hash(c.x, c.y, c.z) & (uint32_t)(C.tsize[t] - 1));
+    C.cell[b] = make_int4(c.x, c.y, c.z, bucket);
+    atomicAdd(&C.count[bucket], 1);
+}
+
+// Exclusive scan of one trajectory's bucket counts; bucket starts are global
+// bead slots.  Also clears the scatter cursors.
+__global__ void __launch_bounds__(1024) k_cell_scan(DevTopo T, CellTables C) {
+    __shared__ int part[1024];
+    const int t = blockIdx.x;
+    const int off = C.toff[t], n = C.tsize[t];
+    const int per = (n + 1023) / 1024;
+    const int lo = threadIdx.x * per, hi = min(n, lo + per);
+    int s = 0;
+    for (int i = lo; i < hi; i++) s += C.count[off + i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int v = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int acc = T.tbase[t] + part[threadIdx.x] - s;
+    for (int i = lo; i < hi; i++) {
+        const int v = C.count[off + i];
+        C.start[off + i] = acc;
+        C.fill[off + i] = 0;
+        acc += v;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_cell_scatter(DevTopo T, CellTables C) {
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= T.nb) return;
+    const int bucket = C.cell[b].w;
+    C.sorted[C.start[bucket] + atomicAdd(&C.fill[bucket], 1)] = b;
+}
+
+// Candidates of bead b: beads of the 27 neighbouring cells within `radius`
+// (minimum image), as local indices in ascending order.  Returns the count,
+// or -1 when more than CAND_CAP were found.
+__device__ int gather_candidates(const DevTopo &T, const DevState &S, const CellTables &C, int b, float radius,
+                                 int *cand) {
+    const int t = T.tid[b], base = T.tbase[t];
+    const float box = T.tbox[t];
+    const CellGeom g = cell_geom(box, radius);
+    const int4 me = C.cell[b];
+    const float3 ri = f3(S.xyz[b]);
+    const float r2max = radius * radius;
+    const uint32_t mask = (uint32_t)(C.tsize[t] - 1);
+    int n = 0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                int cx = me.x + dx, cy = me.y + dy, cz = me.z + dz;
+                if (g.dims > 0) {
+                    cx = (cx + g.dims) % g.dims;
+                    cy = (cy + g.dims) % g.dims;
+                    cz = (cz + g.dims) % g.dims;
+                }
+                const int bucket = C.toff[t] + (int)(cell_hash(cx, cy, cz) & mask);
+                const int s0 = C.start[bucket], s1 = s0 + C.count[bucket];
+                for (int q = s0; q < s1; q++) {
+                    const int j = C.sorted[q];
+                    if (j == b) continue;
+                    const int4 cj = C.cell[j];
+                    if (cj.x != cx || cj.y != cy || cj.z != cz) continue;
+                    const float3 d = min_image(sub3(ri, f3(S.xyz[j])), box);
+                    if (d.x * d.x + d.y * d.y + d.z * d.z >= r2max) continue;
+                    if (n >= CAND_CAP) return -1;
+                    // insertion keeps the candidates sorted by index
+                    int k = n++;
+                    const int lj = j - base;
+                    while (k > 0 && cand[k - 1] > lj) {
+                        cand[k] = cand[k - 1];
+                        k--;
+                    }
+                    cand[k] = lj;
+                }
+            }
+    return n;
+}
+
+__global__ void __launch_bounds__(CELL_BLOCK)
+    k_rebuild_cells(mte_params P, DevTopo T, DevState S, CellTables C, float radius, int do_lj, int do_bonded) {
+    __shared__ int cand_all[CELL_BLOCK * CAND_CAP];
+    const int b = blockIdx.x * CELL_BLOCK + threadIdx.x;
+    if (b >= T.nb) return;
+    const int t = T.tid[b], base = T.tbase[t];
+    RebuildOwn o = rebuild_own(S.xyz[b], S.fn[b], S.fp1[b], S.fp2[b], T.mon_type[b], T.harm_raw[b], b - base, b,
+                               base, T.tbox[t]);
+    if (!T.extra[b]) {
+        int *cand = cand_all + threadIdx.x * CAND_CAP;
+        const int n = gather_candidates(T, S, C, b, radius, cand);
+        if (n < 0) {
+            o.ovf = 1;
+        } else {
+            for (int k = 0; k < n; k++) {
+                const int j = base + cand[k];
+                rebuild_pair(T, o, cand[k], S.xyz[j], S.fn[j], S.fp1[j], S.fp2[j], T.mon_type[j], P.ljpairscutoff,
+                             do_lj, do_bonded);
+            }
+        }
+    }
+    rebuild_finish(T, o, do_lj, do_bonded);
+}
+
+float candidate_radius(const mte_params &P) {
+    float r = 0.f;
+    if (P.lj_on) r = fmaxf(r, P.ljpairscutoff);
+    if (P.is_assembly) r = fmaxf(r, BOND_PREFILTER);
+    return r * 1.0001f + 1e-4f;   // superset of every list test
+}
+
+hipError_t build_cells(mte_engine *e, float radius) {
+    const CellTables &C = e->cells;
+    const int gb = (e->nb + 255) / 256;
+    hipLaunchKernelGGL(k_fill_int, dim3((C.total + 255) / 256), dim3(256), 0, e->stream, C.count, C.total, 0);
+    hipLaunchKernelGGL(k_cell_assign, dim3(gb), dim3(256), 0, e->stream, e->T, e->S[e->cur], C, radius);
+    hipLaunchKernelGGL(k_cell_scan, dim3(e->ntraj), dim3(1024), 0, e->stream, e->T, C);
+    hipLaunchKernelGGL(k_cell_scatter, dim3(gb), dim3(256), 0, e->stream, e->T, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_rebuild_cells(mte_engine *e) {
+    const int do_lj = e->P.lj_on ? 1 : 0;
+    const int do_bonded = e->P.is_assembly ? 1 : 0;
+    if (!do_lj && !do_bonded) return hipSuccess;
+    const float radius = candidate_radius(e->P);
+    hipError_t s = build_cells(e, radius);
+    if (s != hipSuccess) return s;
+    hipLaunchKernelGGL(k_rebuild_cells, dim3((e->nb + CELL_BLOCK - 1) / CELL_BLOCK), dim3(CELL_BLOCK), 0, e->stream,
+                       e->P, e->T, e->S[e->cur], e->cells, radius, do_lj, do_bonded);
+    return hipGetLastError();
+}
+
+}  // namespace mte

@@ -50,6 +50,7 @@ struct DevTopo {
     const int *mon_type;
     int *gtp, *on_tub;
     const float *zs;
+    const float *tbox;    // trajectory -> periodic box side (0: open boundaries)
     int *overflow;        // set to 1 by a rebuild that ran out of list capacity
 };
 
@@ -63,6 +64,20 @@ struct DevState {
 };
 
 __device__ __forceinline__ float3 f3(float4 v) { return make_float3(v.x, v.y, v.z); }
+
+// Minimum-image separation in a cubic periodic box of side L (L == 0: open).
+// The reference has no periodic boundaries; the uneven free-dimer batches
+// run in one.
+__device__ __forceinline__ float3 min_image(float3 d, float L) {
+    if (L > 0.f) {
+        const float iL = 1.f / L;
+        d.x -= L * rintf(d.x * iL);
+        d.y -= L * rintf(d.y * iL);
+        d.z -= L * rintf(d.z * iL);
+    }
+    return d;
+}
+__device__ __forceinline__ float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
 
 // sin and cos by Cody-Waite reduction modulo pi/2 (two-part pi/2, exact
 // with FMA for |x| < 8192) and the single-precision minimax polynomials of
@@ -125,15 +140,16 @@ __device__ __forceinline__ void frame_vectors(float fi, float theta, float psi, 
     a2 = make_float4(v.x, v.y, v.z, 0.f);
 }
 
-// Gap and gradient of |g|^2/2 with respect to bead i.  grad order:
-// x, y, z, fi, theta, psi.  uphi = (cp*ct, sp*ct, -st), uth = (-sp, cp, 0).
+// Gap g = dij + (a - b), dij = r_i - r_j (minimum image), and the gradient
+// of |g|^2/2 with respect to bead i.  grad order: x, y, z, fi, theta, psi.
+// uphi = (cp*ct, sp*ct, -st), uth = (-sp, cp, 0).
 struct Grad { float g[6]; float dr, inv; };   // inv = 1/dr (0 when dr == 0)
 
-__device__ __forceinline__ Grad gap_grad(float3 ri, float3 a, float3 rj, float3 b, float3 uphi, float2 uth) {
+__device__ __forceinline__ Grad gap_grad(float3 dij, float3 a, float3 b, float3 uphi, float2 uth) {
     Grad o;
-    const float gx = (ri.x - rj.x) + (a.x - b.x);
-    const float gy = (ri.y - rj.y) + (a.y - b.y);
-    const float gz = (ri.z - rj.z) + (a.z - b.z);
+    const float gx = dij.x + (a.x - b.x);
+    const float gy = dij.y + (a.y - b.y);
+    const float gz = dij.z + (a.z - b.z);
     const float t0 = a.y * gz - a.z * gy;
     const float t1 = a.z * gx - a.x * gz;
     const float t2 = a.x * gy - a.y * gx;

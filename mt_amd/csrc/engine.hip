@@ -223,7 +223,7 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     T.max_lj = e->O.max_lj;
     int *p_tid, *p_tbase, *p_tsize, *p_harm, *p_harm_raw, *p_mon, *p_gtp, *p_ont;
     uint8_t *p_fixed, *p_extra;
-    float *p_zs;
+    float *p_zs, *p_box;
 #define ALLOC(ptr, n)                                                          \
     if (dalloc(e, &(ptr), (n)) != hipSuccess) {                                \
         e->err = "mte_create: hipMalloc failed for " #ptr;                     \
@@ -232,7 +232,7 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     ALLOC(p_tid, nb) ALLOC(p_tbase, nt) ALLOC(p_tsize, nt) ALLOC(p_harm, nb) ALLOC(p_harm_raw, nb)
     ALLOC(T.lon_n, nb) ALLOC(T.lon, (size_t)MAXLON * nb) ALLOC(T.lat_n, nb) ALLOC(T.lat, (size_t)MAXLAT * nb)
     ALLOC(T.lj_n, nb) ALLOC(T.lj, (size_t)T.max_lj * nb) ALLOC(p_fixed, nb) ALLOC(p_extra, nb) ALLOC(p_mon, nb)
-    ALLOC(p_gtp, nb) ALLOC(p_ont, nb) ALLOC(p_zs, nt) ALLOC(T.overflow, 1) ALLOC(e->tiles, tiles.size())
+    ALLOC(p_gtp, nb) ALLOC(p_ont, nb) ALLOC(p_zs, nt) ALLOC(p_box, nt) ALLOC(T.overflow, 1) ALLOC(e->tiles, tiles.size())
     for (int k = 0; k < 2; k++) {
         ALLOC(e->S[k].xyz, nb) ALLOC(e->S[k].ang, nb) ALLOC(e->S[k].fn, nb) ALLOC(e->S[k].fp1, nb)
         ALLOC(e->S[k].fp2, nb)
@@ -256,7 +256,7 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     }
 #undef ALLOC
     T.tid = p_tid; T.tbase = p_tbase; T.tsize = p_tsize; T.harm = p_harm; T.harm_raw = p_harm_raw;
-    T.fixed = p_fixed; T.extra = p_extra; T.mon_type = p_mon; T.gtp = p_gtp; T.on_tub = p_ont; T.zs = p_zs;
+    T.fixed = p_fixed; T.extra = p_extra; T.mon_type = p_mon; T.gtp = p_gtp; T.on_tub = p_ont; T.zs = p_zs; T.tbox = p_box;
 
     std::vector<uint8_t> fixed(nb, 0), extra(nb, 0);
     std::vector<int> mon(nb, 0), gtp(nb, 1), ont(nb, 0);
@@ -269,6 +269,10 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
         if (topo->on_tubule) ont[b] = topo->on_tubule[b];
     }
     if (topo->wall_z) std::copy(topo->wall_z, topo->wall_z + nt, zs.begin());
+    std::vector<float> box(nt, 0.f);
+    if (topo->box) std::copy(topo->box, topo->box + nt, box.begin());
+    for (int t = 0; t < nt; t++)
+        if (!(box[t] >= 0.f)) { e->err = "mte_create: periodic box side must be >= 0"; return bail(MTE_ERR_ARG); }
     std::vector<uint32_t> seeds((size_t)8 * nb);
     generate_seeds(opt->rseed, seeds.size(), seeds.data());
 
@@ -283,6 +287,7 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     UP(T.lon, lon.data(), lon.size()) UP(T.lat_n, lat_n.data(), nb) UP(T.lat, lat.data(), lat.size())
     UP(p_fixed, fixed.data(), nb) UP(p_extra, extra.data(), nb) UP(p_mon, mon.data(), nb)
     UP(p_gtp, gtp.data(), nb) UP(p_ont, ont.data(), nb) UP(p_zs, zs.data(), nt)
+    UP(p_box, box.data(), nt)
     UP(e->tiles, tiles.data(), tiles.size()) UP(e->seeds, seeds.data(), seeds.size())
     if (e->P.tea_on) {
         UP(e->tea.units, tea_units.data(), tea_units.size()) UP(e->tea.nchunks, tea_nchunks.data(), nt)
@@ -296,6 +301,49 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     if (e->P.tea_on && hipMemsetAsync(e->tea.status, 0, sizeof(int), e->stream) != hipSuccess) {
         e->err = "mte_create: memset failed";
         return bail(MTE_ERR_HIP);
+    }
+    // Cell lists: selection and tables.
+    {
+        int nmax = 0;
+        for (int t = 0; t < nt; t++) nmax = std::max(nmax, tsize[t]);
+        const float radius = candidate_radius(e->P);
+        bool cells_ok = radius > 0.f;
+        for (int t = 0; t < nt; t++)
+            if (box[t] > 0.f && box[t] < 3.f * radius) cells_ok = false;   // 27-cell search needs >= 3 cells per side
+        if (opt->list_method == MTE_LISTS_CELLS && !cells_ok) {
+            e->err = "mte_create: cell lists need every periodic box side >= 3 x the list radius";
+            return bail(MTE_ERR_ARG);
+        }
+        e->use_cells = cells_ok && (opt->list_method == MTE_LISTS_CELLS ||
+                                    (opt->list_method == MTE_LISTS_AUTO && nmax > 2048));
+        if (e->use_cells) {
+            std::vector<int> toff(nt), tsz(nt);
+            int total = 0;
+            for (int t = 0; t < nt; t++) {
+                int s = 64;
+                while (s < 2 * tsize[t]) s <<= 1;
+                toff[t] = total;
+                tsz[t] = s;
+                total += s;
+            }
+            CellTables &C = e->cells;
+            int *p_toff, *p_tsz;
+            if (dalloc(e, &p_toff, nt) != hipSuccess || dalloc(e, &p_tsz, nt) != hipSuccess ||
+                dalloc(e, &C.count, total) != hipSuccess || dalloc(e, &C.start, total) != hipSuccess ||
+                dalloc(e, &C.fill, total) != hipSuccess || dalloc(e, &C.cell, nb) != hipSuccess ||
+                dalloc(e, &C.sorted, nb) != hipSuccess) {
+                e->err = "mte_create: hipMalloc failed for cell tables";
+                return bail(MTE_ERR_HIP);
+            }
+            C.toff = p_toff;
+            C.tsize = p_tsz;
+            C.total = total;
+            if (hipMemcpyAsync(p_toff, toff.data(), sizeof(int) * nt, hipMemcpyHostToDevice, e->stream) != hipSuccess ||
+                hipMemcpyAsync(p_tsz, tsz.data(), sizeof(int) * nt, hipMemcpyHostToDevice, e->stream) != hipSuccess) {
+                e->err = "mte_create: upload failed for cell tables";
+                return bail(MTE_ERR_HIP);
+            }
+        }
     }
     ResidentPlan plan;
     const bool fits = resident_fits(e, &plan);

@@ -35,6 +35,16 @@ struct TeaBuffers {
 // j-chunk length of the TEA pair kernel.
 constexpr int TEA_CHUNK_BEADS = 128;
 
+// Hashed cell lists (cells.hip): per trajectory a power-of-two bucket table.
+struct CellTables {
+    const int *toff = nullptr;   // trajectory -> first bucket
+    const int *tsize = nullptr;  // trajectory -> bucket count (power of two)
+    int *count = nullptr, *start = nullptr, *fill = nullptr;   // per bucket
+    int4 *cell = nullptr;        // per bead: cell coordinates, bucket
+    int *sorted = nullptr;       // beads in bucket order
+    int total = 0;               // buckets over all trajectories
+};
+
 // Resident-mode launch plan: trajectories grouped by workgroup.
 struct ResidentPlan {
     int threads = 0;        // workgroup size
@@ -69,6 +79,8 @@ struct mte_engine {
     int ntiles = 0;
     int *d_overflow = nullptr;
     mte::TeaBuffers tea;
+    mte::CellTables cells;
+    bool use_cells = false;
     mte::ResidentPlan rplan;
 
     int64_t step = 0;
@@ -95,6 +107,9 @@ hipError_t launch_forces(mte_engine *e);
 hipError_t launch_forces_tea(mte_engine *e, const float *noise_step);
 hipError_t launch_energies(mte_engine *e);
 hipError_t launch_rebuild(mte_engine *e);
+hipError_t launch_rebuild_cells(mte_engine *e);
+hipError_t build_cells(mte_engine *e, float radius);
+float candidate_radius(const mte_params &P);
 hipError_t launch_tea_eps(mte_engine *e);
 hipError_t launch_tea_pairs(mte_engine *e);
 hipError_t launch_tea_finish(mte_engine *e, const float *noise_step);

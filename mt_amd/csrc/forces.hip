@@ -51,7 +51,8 @@ __global__ void __launch_bounds__(STREAM_BLOCK)
         Sout.fp2[b] = Sin.fp2[b];
         return;
     }
-    const Own I = make_own(x, a, Sin.fn[b], Sin.fp1[b], Sin.fp2[b]);
+    Own I = make_own(x, a, Sin.fn[b], Sin.fp1[b], Sin.fp2[b]);
+    I.box = T.tbox[T.tid[b]];
     float F[6];
     bead_force(P, T, GlobalAcc{Sin, T}, GlobalLists{T, b}, b, I, F);
     float4 rx, ra;
@@ -94,7 +95,8 @@ __global__ void __launch_bounds__(STREAM_BLOCK)
     if (b >= T.nb) return;
     float F[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (!T.extra[b]) {
-        const Own I = make_own(S.xyz[b], S.ang[b], S.fn[b], S.fp1[b], S.fp2[b]);
+        Own I = make_own(S.xyz[b], S.ang[b], S.fn[b], S.fp1[b], S.fp2[b]);
+        I.box = T.tbox[T.tid[b]];
         bead_force(P, T, GlobalAcc{S, T}, GlobalLists{T, b}, b, I, F);
     }
     fxyz[b] = make_float4(F[0], F[1], F[2], 0.f);
@@ -120,7 +122,8 @@ __global__ void __launch_bounds__(STREAM_BLOCK)
     if (b >= T.nb) return;
     float U[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (!T.extra[b]) {
-        const Own I = make_own(S.xyz[b], S.ang[b], S.fn[b], S.fp1[b], S.fp2[b]);
+        Own I = make_own(S.xyz[b], S.ang[b], S.fn[b], S.fp1[b], S.fp2[b]);
+        I.box = T.tbox[T.tid[b]];
         bead_energy(P, T, GlobalAcc{S, T}, b, I, U);
     }
 #pragma unroll

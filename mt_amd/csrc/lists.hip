@@ -34,7 +34,8 @@ __global__ void __launch_bounds__(LIST_TILE)
     RebuildOwn o{};
     bool skip = true;
     if (active) {
-        o = rebuild_own(S.xyz[b], S.fn[b], S.fp1[b], S.fp2[b], T.mon_type[b], T.harm_raw[b], li, b, base);
+        o = rebuild_own(S.xyz[b], S.fn[b], S.fp1[b], S.fp2[b], T.mon_type[b], T.harm_raw[b], li, b, base,
+                        T.tbox[tl.x]);
         skip = T.extra[b] != 0;
     }
     for (int j0 = 0; j0 < n; j0 += LIST_TILE) {
@@ -58,6 +59,7 @@ __global__ void __launch_bounds__(LIST_TILE)
 }
 
 hipError_t launch_rebuild(mte_engine *e) {
+    if (e->use_cells) return launch_rebuild_cells(e);
     const int do_lj = e->P.lj_on ? 1 : 0;
     const int do_bonded = e->P.is_assembly ? 1 : 0;
     if (!do_lj && !do_bonded) return hipSuccess;

@@ -12,13 +12,15 @@ namespace mte {
 struct RebuildOwn {
     float3 ri, ani, a1, a2;   // position, R_MON n_i, R p1, R p2
     float rp;                  // R_MON used by pairs_kernel
+    float box;                 // periodic side (0: open)
     int mti, hraw, hpart, li, b, base;
     int nlon, nlat, nlj, ovf;
 };
 
 __device__ __forceinline__ RebuildOwn rebuild_own(float4 x, float4 n, float4 p1, float4 p2, int mti, int hraw, int li,
-                                                  int b, int base) {
+                                                  int b, int base, float box) {
     RebuildOwn o;
+    o.box = box;
     o.ri = f3(x);
     // pairs_kernel takes R_MON and the excluded partner from the first
     // harmonic slot, including the -1 fill of partnerless beads (:548-551, :565).
@@ -41,9 +43,8 @@ __device__ __forceinline__ void rebuild_pair(const DevTopo &T, RebuildOwn &o, in
                                              float4 p1j, float4 p2j, int mtj, float ljcut, bool do_lj,
                                              bool do_bonded) {
     const int nb = T.nb;
-    const float3 rj = f3(xj);
-    const float dx = o.ri.x - rj.x, dy = o.ri.y - rj.y, dz = o.ri.z - rj.z;
-    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float3 dij = min_image(sub3(o.ri, f3(xj)), o.box);
+    const float d2 = dij.x * dij.x + dij.y * dij.y + dij.z * dij.z;
     if (do_lj && sqrtf(d2) < ljcut && o.li != j) {
         if (o.nlj < T.max_lj) T.lj[o.nlj * nb + o.b] = o.base + j;
         else o.ovf = 1;
@@ -53,7 +54,7 @@ __device__ __forceinline__ void rebuild_pair(const DevTopo &T, RebuildOwn &o, in
     const float3 z3 = make_float3(0, 0, 0);
     const float2 z2 = make_float2(0, 0);
     if (o.mti != mtj) {
-        const Grad G = gap_grad(o.ri, o.ani, rj, make_float3(-o.rp * nj.x, -o.rp * nj.y, -o.rp * nj.z), z3, z2);
+        const Grad G = gap_grad(dij, o.ani, make_float3(-o.rp * nj.x, -o.rp * nj.y, -o.rp * nj.z), z3, z2);
         if (G.dr < PAIR_CUT) {
             // stored (h < 0 ? j : -j) and decoded "negative -> +r_mon": R_MON > 0
             // iff h >= 0 and j != 0 (-0 == 0 decodes as -r_mon).
@@ -65,14 +66,14 @@ __device__ __forceinline__ void rebuild_pair(const DevTopo &T, RebuildOwn &o, in
     }
     if (o.li != j) {
         // orientation 0: i uses p1, j uses p2 (stored negative)
-        Grad G = gap_grad(o.ri, o.a1, rj, f3(p2j), z3, z2);
+        Grad G = gap_grad(dij, o.a1, f3(p2j), z3, z2);
         if (G.dr < PAIR_CUT) {
             if (o.nlat < MAXLAT) T.lat[o.nlat * nb + o.b] = ((o.base + j) << 1) | 1;
             else o.ovf = 1;
             o.nlat++;
         }
         // orientation 1: i uses p2, j uses p1 (stored positive)
-        G = gap_grad(o.ri, o.a2, rj, f3(p1j), z3, z2);
+        G = gap_grad(dij, o.a2, f3(p1j), z3, z2);
         if (G.dr < PAIR_CUT) {
             if (o.nlat < MAXLAT) T.lat[o.nlat * nb + o.b] = ((o.base + j) << 1);
             else o.ovf = 1;

@@ -97,6 +97,7 @@ __device__ __forceinline__ void run_steps(const mte_params &P, const DevTopo &T,
                                           uint4 s1[PT], float4 trig[PT], const float *noise, int nsteps, const LatOffsets &L,
                                           const MakeLists &make_lists) {
     const int base = A.base, nb = T.nb;
+    const float box = T.tbox[T.tid[base]];
     for (int s = 0; s < nsteps; s++) {
         float4 nx[PT], na[PT];
 #pragma unroll
@@ -106,7 +107,8 @@ __device__ __forceinline__ void run_steps(const mte_params &P, const DevTopo &T,
             const int b = base + i;
             float4 x = lx[i];
             float4 a = make_float4(ln[i].w, l1[i].w, l2[i].w, 0.f);
-            const Own I = make_own_trig(x, a, ln[i], l1[i], l2[i], trig[k]);
+            Own I = make_own_trig(x, a, ln[i], l1[i], l2[i], trig[k]);
+            I.box = box;
             float F[6];
             const Lists Ls = make_lists(i, b);
             bead_force(P, T, A, Ls, b, I, F);
@@ -188,7 +190,7 @@ __global__ void __launch_bounds__(MAXT)
             const int i = threadIdx.x + k * blockDim.x;
             if (i >= n) continue;
             const int b = base + i;
-            RebuildOwn o = rebuild_own(lx[i], ln[i], l1[i], l2[i], A.mon(b), T.harm_raw[b], i, b, base);
+            RebuildOwn o = rebuild_own(lx[i], ln[i], l1[i], l2[i], A.mon(b), T.harm_raw[b], i, b, base, T.tbox[t]);
             if (!T.extra[b])
                 for (int j = 0; j < n; j++)
                     rebuild_pair(T, o, j, lx[j], ln[j], l1[j], l2[j], __float_as_int(lx[j].w) & FL_MON,

@@ -56,6 +56,7 @@ __global__ void __launch_bounds__(TEA_TILE)
         ri = f3(S.xyz[b]);
         skip = T.extra[b] != 0;
     }
+    const float box = T.tbox[tl.x];
     float sxx = 0.f, syy = 0.f, szz = 0.f, sall = 0.f;
     for (int j0 = 0; j0 < n; j0 += TEA_TILE) {
         const int jl = j0 + threadIdx.x;
@@ -70,7 +71,8 @@ __global__ void __launch_bounds__(TEA_TILE)
             for (int jj = 0; jj < m; jj++) {
                 const float4 xj = sx[jj];
                 if (j0 + jj == li || xj.w != 0.f) continue;
-                const float dx = xj.x - ri.x, dy = xj.y - ri.y, dz = xj.z - ri.z;
+                const float3 d = min_image(sub3(f3(xj), ri), box);
+                const float dx = d.x, dy = d.y, dz = d.z;
                 const float r = sqrtf(dx * dx + dy * dy + dz * dz);
                 const float ex = dx / r, ey = dy / r, ez = dz / r;
                 const RpyCoef c = rpy_coef(r, 1.f / r, P.tea_a);
@@ -175,10 +177,12 @@ __global__ void __launch_bounds__(TEA_TILE)
         const float cx = bt * rsq_fast(1.f + w * cc.x), cy = bt * rsq_fast(1.f + w * cc.y),
                     cz = bt * rsq_fast(1.f + w * cc.z);
         const float a = P.tea_a, ia = rcp_fast(P.tea_a);
+        const float box = T.tbox[u.x];
         const int self = li - u.z;   // position of i in this chunk, if any
         for (int q = 0; q < jn; q++) {
             const float4 xj = sx[q], Fj = sf[q], Rj = sr[q];
-            const float dx = xj.x - ri.x, dy = xj.y - ri.y, dz = xj.z - ri.z;
+            const float3 d = min_image(sub3(f3(xj), ri), box);
+            const float dx = d.x, dy = d.y, dz = d.z;
             const bool is_self = q == self;
             const float r2 = is_self ? 1.f : dx * dx + dy * dy + dz * dz;   // keeps the j == i slot finite
             const float rinv = rsq_fast(r2);

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "libmt_engine.so")
 
 MTE_OK = 0
 MODE_AUTO, MODE_STREAMED, MODE_RESIDENT = 0, 1, 2
+LISTS_AUTO, LISTS_ALLPAIRS, LISTS_CELLS = 0, 1, 2
 ERRORS = {1: "MTE_ERR_ARG", 2: "MTE_ERR_HIP", 3: "MTE_ERR_OVERFLOW", 4: "MTE_ERR_TEA", 5: "MTE_ERR_STATE",
           6: "MTE_ERR_NOISE"}
 
@@ -35,12 +36,12 @@ class Topology(ctypes.Structure):
                 ("lateral_count", ctypes.c_void_p), ("lateral", ctypes.c_void_p),
                 ("fixed", ctypes.c_void_p), ("extra", ctypes.c_void_p),
                 ("mon_type", ctypes.c_void_p), ("gtp", ctypes.c_void_p), ("on_tubule", ctypes.c_void_p),
-                ("wall_z", ctypes.c_void_p)]
+                ("wall_z", ctypes.c_void_p), ("box", ctypes.c_void_p)]
 
 
 class Options(ctypes.Structure):
     _fields_ = [("max_lj", ctypes.c_int32), ("mode", ctypes.c_int32), ("rseed", ctypes.c_int32),
-                ("device", ctypes.c_int32)]
+                ("device", ctypes.c_int32), ("list_method", ctypes.c_int32)]
 
 
 # Every symbol include/mt_engine.h declares, with its ctypes signature.
@@ -97,7 +98,7 @@ class Engine:
     """One engine instance over a CSR batch (see include/mt_engine.h)."""
 
     def __init__(self, params: Params, batch: Batch, max_lj: int = 64, mode: int = MODE_AUTO, rseed: int = 1234567,
-                 device: int = 0):
+                 device: int = 0, list_method: int = LISTS_AUTO):
         self.lib = load_library()
         self.params = params
         self.nb = batch.nb
@@ -114,8 +115,8 @@ class Engine:
                         arr("lon_count", np.int32), arr("lon", np.int32), arr("lat_count", np.int32),
                         arr("lat", np.int32), arr("fixed", np.uint8), arr("extra", np.uint8),
                         arr("mon_type", np.int32), arr("gtp", np.int32), arr("on_tub", np.int32),
-                        arr("zs", np.float32))
-        opt = Options(max_lj, mode, rseed, device)
+                        arr("zs", np.float32), arr("box", np.float32))
+        opt = Options(max_lj, mode, rseed, device, list_method)
         h = ctypes.c_void_p()
         rc = self.lib.mte_create(ctypes.byref(params), ctypes.byref(topo), ctypes.byref(opt), ctypes.byref(h))
         if rc != MTE_OK:

@@ -82,9 +82,12 @@ def box_side_nm(n_free_dimers: int, conc_um: float) -> float:
 
 
 def free_dimers(n: int, box: float, rng: np.random.Generator, min_sep: float = 8.0,
-                first_id: int = 1, chain: str = "X", avoid: np.ndarray | None = None) -> Atoms:
+                first_id: int = 1, chain: str = "X", avoid: np.ndarray | None = None,
+                first_resid: int = 1000) -> Atoms:
     """n free dimers uniformly in [0, box)^3 with centre separation >= min_sep
-    (random_concentration_w-o_ring.py:39 uses 8 nm), CA below CB along +z."""
+    (random_concentration_w-o_ring.py:39 uses 8 nm), CA below CB along +z.
+    Residue numbers start at first_resid so that `fix` (resid <= fix) never
+    pins a free dimer."""
     cen = np.zeros((0, 3))
     tries = 0
     while len(cen) < n:
@@ -111,7 +114,7 @@ def free_dimers(n: int, box: float, rng: np.random.Generator, min_sep: float = 8
     ids = np.arange(first_id, first_id + nb, dtype=np.int32)
     names = ["CA" if k % 2 == 0 else "CB" for k in range(nb)]
     chains = np.full(nb, ord(chain), np.uint8)
-    resid = np.repeat(np.arange(1, n + 1, dtype=np.int32), 2)
+    resid = np.repeat(np.arange(first_resid, first_resid + n, dtype=np.int32), 2)
     xyz = np.zeros((nb, 3))
     xyz[0::2] = cen - np.array([0, 0, R_MON])
     xyz[1::2] = cen + np.array([0, 0, R_MON])

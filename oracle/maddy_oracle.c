@@ -82,9 +82,19 @@ typedef struct {
     const uint8_t *fixed, *extra;
     const int32_t *mon_type, *gtp, *on_tub;
     const float *zs;                  /* wall height per trajectory */
+    const float *box;                 /* periodic cubic box side per trajectory, 0 = open */
 } or_topo;
 
 typedef struct { float x, y, z, fi, theta, psi, w; } or_coord; /* src/mt.h:63-71 */
+
+/* Minimum image in a cubic box of side L (L == 0: open).  The reference has
+ * open boundaries only; periodic boxes are this build's extension for the
+ * free-dimer batches. */
+static void min_image(real d[3], real L)
+{
+    if (L > 0)
+        for (int k = 0; k < 3; k++) d[k] -= L * rint(d[k] / L);
+}
 
 /* ---------------------------------------------------------------- frames */
 typedef struct {
@@ -131,9 +141,12 @@ static void make_frame(const or_coord *c, frame *f)
 
 /* Gap g = (r_i + a_i) - (r_j + b_j) and gradient of |g|^2/2 wrt bead i's
  * six coordinates (x, y, z, fi, theta, psi). */
-static real gap_grad(const frame *fi_, const real a[3], const frame *fj, const real b[3], real g[3], real grad[6])
+static real gap_grad(const frame *fi_, const real a[3], const frame *fj, const real b[3], real L, real g[3],
+                     real grad[6])
 {
-    for (int k = 0; k < 3; k++) g[k] = (fi_->r[k] - fj->r[k]) + (a[k] - b[k]);
+    real d[3] = {fi_->r[0] - fj->r[0], fi_->r[1] - fj->r[1], fi_->r[2] - fj->r[2]};
+    min_image(d, L);
+    for (int k = 0; k < 3; k++) g[k] = d[k] + (a[k] - b[k]);
     const real t0 = a[1] * g[2] - a[2] * g[1];
     const real t1 = a[2] * g[0] - a[0] * g[2];
     const real t2 = a[0] * g[1] - a[1] * g[0];
@@ -147,20 +160,20 @@ static real gap_grad(const frame *fi_, const real a[3], const frame *fj, const r
 /* Harmonic / longitudinal attachment: R_MON > 0 couples i's +n end to j's +n
  * end pointing back (src/compute_cuda.cu:89-96): d = r_j - r_i - R(n_i+n_j),
  * i.e. a_i = R n_i, b_j = -R n_j, g = -d. */
-static real gap_axial(const frame *fi_, const frame *fj, real rmon, real g[3], real grad[6])
+static real gap_axial(const frame *fi_, const frame *fj, real rmon, real L, real g[3], real grad[6])
 {
     real a[3], b[3];
     for (int k = 0; k < 3; k++) { a[k] = rmon * fi_->n[k]; b[k] = -rmon * fj->n[k]; }
-    return gap_grad(fi_, a, fj, b, g, grad);
+    return gap_grad(fi_, a, fj, b, L, g, grad);
 }
 
 /* Lateral: g = r_i - r_j + R_i p2 - R_j p1 (src/compute_cuda.cu:337-352);
  * swapped==1 exchanges p1 and p2 (entries <= 0, :309-324). */
-static real gap_lateral(const frame *fi_, const frame *fj, int swapped, real g[3], real grad[6])
+static real gap_lateral(const frame *fi_, const frame *fj, int swapped, real L, real g[3], real grad[6])
 {
     const real *a = swapped ? fi_->ap1 : fi_->ap2;
     const real *b = swapped ? fj->ap2 : fj->ap1;
-    return gap_grad(fi_, a, fj, b, g, grad);
+    return gap_grad(fi_, a, fj, b, L, g, grad);
 }
 
 /* src/compute_cuda.cu:16-30 */
@@ -208,6 +221,7 @@ void or_forces(const or_params *p, const or_topo *t, const or_coord *rc, int64_t
         if (t->extra[b]) continue;
         const int tr = traj_of(t, b);
         const int64_t base = t->off[tr];
+        const real L = t->box ? t->box[tr] : 0;
         frame fi_, fj;
         make_frame(&rc[b], &fi_);
         real F[6] = {0, 0, 0, 0, 0, 0}; /* x y z fi theta psi */
@@ -217,7 +231,7 @@ void or_forces(const or_params *p, const or_topo *t, const or_coord *rc, int64_t
             int j; real rm;
             decode_axial(t->harm[b], &j, &rm);
             make_frame(&rc[base + j], &fj);
-            const real dr = gap_axial(&fi_, &fj, rm, g, gr);
+            const real dr = gap_axial(&fi_, &fj, rm, L, g, gr);
             for (int q = 0; q < 6; q++) F[q] -= (real)p->C * gr[q];
             if (dr < O_ANGLE_CUTOFF) {
                 const real psiji = fj.psi - fi_.psi, thetaji = fj.theta - fi_.theta, fiji = fj.fi - fi_.fi;
@@ -238,7 +252,7 @@ void or_forces(const or_params *p, const or_topo *t, const or_coord *rc, int64_t
             int j; real rm;
             decode_axial(t->lon[b * O_MAXLON + k], &j, &rm);
             make_frame(&rc[base + j], &fj);
-            const real dr = gap_axial(&fi_, &fj, rm, g, gr);
+            const real dr = gap_axial(&fi_, &fj, rm, L, g, gr);
             real dUdr = (dr == 0) ? 0 : dmorse(p->D_long, p->A_long, dr) / dr;
             if (p->barrier && t->on_tub[b] == 0 && t->on_tub[base + j] == 0)
                 if (dr != 0) dUdr += dbarr(p->a_barr_long, p->r_barr_long, p->w_barr_long, dr) / dr;
@@ -263,7 +277,7 @@ void or_forces(const or_params *p, const or_topo *t, const or_coord *rc, int64_t
             int j, sw;
             decode_lateral(t->lat[b * O_MAXLAT + k], &j, &sw);
             make_frame(&rc[base + j], &fj);
-            const real dr = gap_lateral(&fi_, &fj, sw, g, gr);
+            const real dr = gap_lateral(&fi_, &fj, sw, L, g, gr);
             real dUdr;
             if (dr == 0) dUdr = 0;
             else if (t->mon_type[b] != t->mon_type[base + j]) dUdr = dmorse(p->D_lat / p->seam_coeff, p->A_lat, dr) / dr;
@@ -277,7 +291,9 @@ void or_forces(const or_params *p, const or_topo *t, const or_coord *rc, int64_t
             for (int k = 0; k < t->lj_count[b]; k++) {
                 const int j = t->lj[b * t->max_lj + k];
                 const or_coord *c = &rc[base + j];
-                const real dx = (real)rc[b].x - c->x, dy = (real)rc[b].y - c->y, dz = (real)rc[b].z - c->z;
+                real d[3] = {(real)rc[b].x - c->x, (real)rc[b].y - c->y, (real)rc[b].z - c->z};
+                min_image(d, L);
+                const real dx = d[0], dy = d[1], dz = d[2];
                 const real dr = sqrt(dx * dx + dy * dy + dz * dz);
                 if (dr < O_LJ_CUTOFF) {
                     const real df = 6.0 / pow(dr, 8);
@@ -316,6 +332,7 @@ void or_energies(const or_params *p, const or_topo *t, const or_coord *rc, int64
         if (t->extra[b]) continue;
         const int tr = traj_of(t, b);
         const int64_t base = t->off[tr];
+        const real L = t->box ? t->box[tr] : 0;
         frame fi_, fj;
         make_frame(&rc[b], &fi_);
         real U_harm = 0, U_long = 0, U_lat = 0, U_psi = 0, U_fi = 0, U_teta = 0, U_lj = 0;
@@ -324,7 +341,7 @@ void or_energies(const or_params *p, const or_topo *t, const or_coord *rc, int64
             int j; real rm;
             decode_axial(t->harm[b], &j, &rm);
             make_frame(&rc[base + j], &fj);
-            const real dr = gap_axial(&fi_, &fj, rm, g, gr);
+            const real dr = gap_axial(&fi_, &fj, rm, L, g, gr);
             U_harm += (p->C / 2) * dr * dr;
             if (dr < O_ANGLE_CUTOFF) {
                 const real psiij = fi_.psi - fj.psi, fiij = fi_.fi - fj.fi, thetaji = fj.theta - fi_.theta;
@@ -337,7 +354,7 @@ void or_energies(const or_params *p, const or_topo *t, const or_coord *rc, int64
             int j; real rm;
             decode_axial(t->lon[b * O_MAXLON + k], &j, &rm);
             make_frame(&rc[base + j], &fj);
-            const real dr = gap_axial(&fi_, &fj, rm, g, gr);
+            const real dr = gap_axial(&fi_, &fj, rm, L, g, gr);
             U_long += morse_en(p->D_long, p->A_long, dr);
             if (p->barrier && t->on_tub[b] == 0 && t->on_tub[base + j] == 0)
                 U_long += barr(p->a_barr_long, p->r_barr_long, p->w_barr_long, dr);
@@ -354,7 +371,7 @@ void or_energies(const or_params *p, const or_topo *t, const or_coord *rc, int64
             int j, sw;
             decode_lateral(t->lat[b * O_MAXLAT + k], &j, &sw);
             make_frame(&rc[base + j], &fj);
-            const real dr = gap_lateral(&fi_, &fj, sw, g, gr);
+            const real dr = gap_lateral(&fi_, &fj, sw, L, g, gr);
             if (t->mon_type[b] != t->mon_type[base + j]) U_lat += morse_en(p->D_lat / p->seam_coeff, p->A_lat, dr);
             else U_lat += morse_en(p->D_lat, p->A_lat, dr);
             if (p->barrier && t->on_tub[b] == 0 && t->on_tub[base + j] == 0)
@@ -364,8 +381,9 @@ void or_energies(const or_params *p, const or_topo *t, const or_coord *rc, int64
             for (int k = 0; k < t->lj_count[b]; k++) {
                 const int j = t->lj[b * t->max_lj + k];
                 const or_coord *c = &rc[base + j];
-                const real dx = (real)rc[b].x - c->x, dy = (real)rc[b].y - c->y, dz = (real)rc[b].z - c->z;
-                const real dr = sqrt(dx * dx + dy * dy + dz * dz);
+                real d[3] = {(real)rc[b].x - c->x, (real)rc[b].y - c->y, (real)rc[b].z - c->z};
+                min_image(d, L);
+                const real dr = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
                 if (dr < O_LJ_CUTOFF) U_lj += (real)p->ljscale * (real)p->ljsigma6 / pow(dr, 6);
             }
         }
@@ -392,6 +410,7 @@ int64_t or_pairs(const or_topo *t, const or_coord *rc, int64_t nb)
         const int n = (int)(t->off[tr + 1] - base);
         const int i = (int)(b - base);
         const int32_t h = t->harm[b];
+        const real L = t->box ? t->box[tr] : 0;
         const real rm = (h < 0) ? -O_RMON : O_RMON; /* :548-551 */
         const int hpart = abs(h);
         frame fi_, fj;
@@ -401,7 +420,7 @@ int64_t or_pairs(const or_topo *t, const or_coord *rc, int64_t nb)
         for (int j = 0; j < n; j++) {
             if (t->mon_type[b] != t->mon_type[base + j] && hpart != j) {
                 make_frame(&rc[base + j], &fj);
-                const real dr = gap_axial(&fi_, &fj, rm, g, gr);
+                const real dr = gap_axial(&fi_, &fj, rm, L, g, gr);
                 if (dr < O_PAIR_CUTOFF) {
                     if (t->lon_count[b] < O_MAXLON) t->lon[b * O_MAXLON + t->lon_count[b]++] = (h < 0) ? j : -j;
                     else ovf = 1;
@@ -413,7 +432,7 @@ int64_t or_pairs(const or_topo *t, const or_coord *rc, int64_t nb)
                 make_frame(&rc[base + j], &fj);
                 for (int ind = 0; ind < 2; ind++) {
                     /* ind 0: p1<->p2 swapped, stored negative (:612-619, :646-651) */
-                    const real dr = gap_lateral(&fi_, &fj, ind == 0, g, gr);
+                    const real dr = gap_lateral(&fi_, &fj, ind == 0, L, g, gr);
                     if (dr < O_PAIR_CUTOFF) {
                         int32_t e = (ind == 0) ? (j != 0 ? -j : -O_ZERO) : (j != 0 ? j : O_ZERO);
                         if (t->lat_count[b] < O_MAXLAT) t->lat[b * O_MAXLAT + t->lat_count[b]++] = e;
@@ -440,10 +459,12 @@ int64_t or_lj_list(const or_params *p, const or_topo *t, const or_coord *rc, int
         const int n = (int)(t->off[tr + 1] - base);
         const int i = (int)(b - base);
         int ovf = 0;
+        const real L = t->box ? t->box[tr] : 0;
         for (int j = 0; j < n; j++) {
-            const real dx = (real)rc[b].x - rc[base + j].x, dy = (real)rc[b].y - rc[base + j].y,
-                       dz = (real)rc[b].z - rc[base + j].z;
-            const real dr = sqrt(dx * dx + dy * dy + dz * dz);
+            real d[3] = {(real)rc[b].x - rc[base + j].x, (real)rc[b].y - rc[base + j].y,
+                         (real)rc[b].z - rc[base + j].z};
+            min_image(d, L);
+            const real dr = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
             if (dr < (real)p->ljpairscutoff && i != j) {
                 if (t->lj_count[b] < t->max_lj) t->lj[b * t->max_lj + t->lj_count[b]++] = j;
                 else ovf = 1;
@@ -598,9 +619,11 @@ void or_tea_eps(const or_params *p, const or_topo *t, const or_coord *rc, int64_
         const int tr = traj_of(t, b);
         const int64_t base = t->off[tr], end = t->off[tr + 1];
         real s[4] = {0, 0, 0, 0};
+        const real L = t->box ? t->box[tr] : 0;
         for (int64_t j = base; j < end; j++) {
             if (j == b || t->extra[j] || t->extra[b]) continue;
             real e[3] = {(real)rc[j].x - rc[b].x, (real)rc[j].y - rc[b].y, (real)rc[j].z - rc[b].z};
+            min_image(e, L);
             const real r = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
             for (int k = 0; k < 3; k++) e[k] /= r;
             real D[9];
@@ -684,9 +707,11 @@ void or_tea_integrate(const or_params *p, const or_topo *t, or_coord *rc, const 
             f[k] = F[7 * b + k] + R[3 * b + k] * c[k];
             c[k] *= beta;
         }
+        const real L = t->box ? t->box[tr] : 0;
         for (int64_t j = base; j < end; j++) {
             if (j == b || t->extra[b] || t->extra[j]) continue;
             real e[3] = {pos[3 * j] - pos[3 * b], pos[3 * j + 1] - pos[3 * b + 1], pos[3 * j + 2] - pos[3 * b + 2]};
+            min_image(e, L);
             const real r = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
             for (int k = 0; k < 3; k++) e[k] /= r;
             real D[9], v[3];

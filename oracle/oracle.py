@@ -30,7 +30,7 @@ class _Topo(ctypes.Structure):
                 ("lj_count", ctypes.c_void_p), ("lj", ctypes.c_void_p),
                 ("fixed", ctypes.c_void_p), ("extra", ctypes.c_void_p),
                 ("mon_type", ctypes.c_void_p), ("gtp", ctypes.c_void_p), ("on_tub", ctypes.c_void_p),
-                ("zs", ctypes.c_void_p)]
+                ("zs", ctypes.c_void_p), ("box", ctypes.c_void_p)]
 
 
 def build():
@@ -95,6 +95,9 @@ class Oracle:
         batch.fixed = np.ascontiguousarray(batch.fixed, np.uint8)
         batch.extra = np.ascontiguousarray(batch.extra, np.uint8)
         batch.zs = np.ascontiguousarray(batch.zs, np.float32)
+        if batch.box is None or len(batch.box) != batch.ntraj:
+            batch.box = np.zeros(batch.ntraj, np.float32)
+        batch.box = np.ascontiguousarray(batch.box, np.float32)
         batch.off = np.ascontiguousarray(batch.off, np.int64)
         self.ci = np.zeros((nb, 3))
         self.eps_t = np.zeros(batch.ntraj)
@@ -105,7 +108,8 @@ class Oracle:
         b = self.b
         self.t = _Topo(b.ntraj, self.max_lj, _ptr(b.off), _ptr(b.harm_count), _ptr(b.harm), _ptr(b.lon_count),
                        _ptr(b.lon), _ptr(b.lat_count), _ptr(b.lat), _ptr(self.lj_count), _ptr(self.lj),
-                       _ptr(b.fixed), _ptr(b.extra), _ptr(b.mon_type), _ptr(b.gtp), _ptr(b.on_tub), _ptr(b.zs))
+                       _ptr(b.fixed), _ptr(b.extra), _ptr(b.mon_type), _ptr(b.gtp), _ptr(b.on_tub), _ptr(b.zs),
+                       _ptr(b.box))
 
     @property
     def _pp(self):

@@ -205,3 +205,60 @@ def test_tea_batch_with_extra_beads_matches_oracle():
     g = eng.get_state()
     assert np.abs(g[:, :3] - orc.b.coords[:, :3]).max() < 1e-4
     eng.close()
+
+
+def _small_uneven_periodic(seed=1):
+    from mt_amd import workloads
+    return workloads.uneven(3, dimers=(20, 30), free=(200, 500), conc=(30.0, 50.0), seed=seed)
+
+
+def test_cell_lists_match_all_pairs_periodic():
+    print("case cell lists vs all-pairs, periodic uneven batch", flush=True)
+    from mt_amd.engine import LISTS_CELLS
+    p = base_params()
+    b = _small_uneven_periodic()
+    eng = Engine(p, b.copy(), mode=MODE_STREAMED, list_method=LISTS_CELLS)
+    orc = Oracle(p, b.copy())
+    eng.rebuild()
+    assert orc.rebuild() == 0
+    _lists_equal(eng, orc)
+    f = eng.forces()
+    fo = orc.forces()
+    assert rel_err(f, fo, XYZ) < 1e-5 and rel_err(f, fo, ANG) < 1e-5
+    eng.close()
+
+
+def test_cell_lists_long_open_tube():
+    print("case cell lists (auto) on a 2600-bead open tube", flush=True)
+    p = base_params()
+    b = c1_batch(ntraj=2, n_dimers=100, jitter=0.1)
+    eng = Engine(p, b.copy(), mode=MODE_STREAMED)
+    orc = Oracle(p, b.copy())
+    eng.rebuild()
+    assert orc.rebuild() == 0
+    _lists_equal(eng, orc)
+    eng.close()
+
+
+def test_periodic_dynamics_and_tea():
+    print("case periodic plain + TEA steps vs oracle", flush=True)
+    from mt_amd import params as P
+    from mt_amd.engine import LISTS_CELLS
+    b = _small_uneven_periodic(seed=2)
+    p = base_params()
+    eng, orc = Engine(p, b.copy(), mode=MODE_STREAMED, list_method=LISTS_CELLS), Oracle(p, b.copy(), precision="f32")
+    nz = noise(np.random.default_rng(3), 120, b.nb)
+    eng.set_noise(nz)
+    eng.run(120)
+    assert orc.run(0, 120, noise=nz) == 0
+    assert np.abs(eng.get_state()[:, :6] - orc.b.coords[:, :6]).max() < 1e-3
+    eng.close()
+    pt = P.derive(dict(P.TEA, tea_epsilon_freq=2))
+    eng, orc = Engine(pt, b.copy(), mode=MODE_STREAMED, list_method=LISTS_CELLS), Oracle(pt, b.copy())
+    nz = noise(np.random.default_rng(4), 4, b.nb)
+    eng.set_noise(nz)
+    eng.run(4)
+    assert orc.run(0, 4, noise=nz) == 0
+    np.testing.assert_allclose(eng.tea_stats()[0], orc.eps_t, rtol=1e-5)
+    assert np.abs(eng.get_state()[:, :3] - orc.b.coords[:, :3]).max() < 1e-4
+    eng.close()

@@ -35,5 +35,5 @@ def test_struct_layouts_match_header():
     # mte_params: 36 4-byte fields in declaration order; mte_coord is 7 floats
     from mt_amd.params import Params
     assert ctypes.sizeof(Params) == 4 * len(Params._fields_)
-    assert ctypes.sizeof(engine.Options) == 16
-    assert ctypes.sizeof(engine.Topology) == 8 + 13 * 8
+    assert ctypes.sizeof(engine.Options) == 20
+    assert ctypes.sizeof(engine.Topology) == 8 + 14 * 8

@@ -208,3 +208,23 @@ def test_lists_brute_force_sets_small():
         d = np.sqrt(((x - x[i]) ** 2).sum(1))
         expect = [j for j in range(b.nb) if j != i and d[j] < p.ljpairscutoff]
         assert list(o.lj[i, :o.lj_count[i]]) == expect
+
+
+def test_periodic_image_invariance():
+    # Moving any bead by a box vector leaves every force unchanged (minimum image).
+    from mt_amd import workloads
+    p = _cfg()
+    b = workloads.uneven(2, dimers=(20, 22), free=(100, 150), conc=(40.0, 50.0), seed=5)
+    o = Oracle(p, b)
+    o.rebuild()
+    F0 = o.forces()
+    c = b.coords.copy()
+    rng = np.random.default_rng(0)
+    for t in range(b.ntraj):
+        s, e = int(b.off[t]), int(b.off[t + 1])
+        L = float(b.box[t])
+        idx = rng.choice(np.arange(s, e), 50, replace=False)
+        shift = rng.integers(-1, 2, size=(50, 3)) * L
+        c[idx, :3] = (c[idx, :3].astype(np.float64) + shift).astype(np.float32)
+    F1 = o.forces(c)
+    assert np.abs(F1 - F0).max() < 5e-3 * np.abs(F0).max()
