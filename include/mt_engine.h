@@ -168,6 +168,44 @@ int mte_reset_timing(mte_engine *e);
 /* Which mode create() chose (MTE_MODE_STREAMED or MTE_MODE_RESIDENT). */
 int mte_get_mode(const mte_engine *e, int32_t *mode);
 
+/* ------------------------------------------------------------------ MLP-HI
+ * The learned hydrodynamic correction has no counterpart in the reference.
+ * Per bead it predicts the local mobility A = I + sum_j phi(r_ij) D_hat_ij
+ * (neighbours within 12 nm, cosine taper from 8 nm) from the relative
+ * positions of the 32 nearest neighbours with a 96-128-128-128-9 tanh MLP,
+ * and integrates dr = (dt/gammaR) A F + sqrt(2 kB T dt/gammaR) chol(A) xi.
+ * Weights: MTE_MLP_NPARAM floats, W1 (96x128, input-major), b1, W2, b2, W3,
+ * b3, W4 (128x9), b4.  Requires tea_on parameters (tea_a, gammaR). */
+#define MTE_MLP_NPARAM 46601
+#define MTE_MLP_FEATURES 96
+#define MTE_MLP_OUTPUTS 9
+#define MTE_HI_DEFAULT 0  /* TEA when params.tea_on, else free draining */
+#define MTE_HI_MLP 1      /* MLP-HI, falling back to TEA above the error tolerance */
+
+int mte_mlp_set_weights(mte_engine *e, const float *w, int64_t n);
+int mte_mlp_get_weights(mte_engine *e, float *w, int64_t n);
+/* Features (B x 96) and exact labels A - I (B x 9) of the current state;
+ * either pointer may be NULL. */
+int mte_mlp_local(mte_engine *e, float *features, float *labels);
+/* Inference with the bf16-rounded weights on n feature rows. */
+int mte_mlp_forward(mte_engine *e, const float *features, int64_t n, float *out);
+/* Mean squared error and its gradient (MTE_MLP_NPARAM floats) with the FP32
+ * weights over n samples. */
+int mte_mlp_loss_grad(mte_engine *e, const float *features, const float *labels, int64_t n, double *loss,
+                      float *grad);
+/* Adam training on the device: `iters` steps over consecutive mini-batches
+ * of `batch` samples (cycling through the n given); loss per step into
+ * losses (may be NULL). */
+int mte_mlp_train(mte_engine *e, const float *features, const float *labels, int64_t n, int32_t iters,
+                  int32_t batch, float lr, double *losses);
+/* Select the hydrodynamics of mte_run: MTE_HI_DEFAULT or MTE_HI_MLP, with the
+ * exact-label check every check_every steps and the relative error above
+ * which the engine falls back to TEA for the rest of the run. */
+int mte_set_hi_mode(mte_engine *e, int32_t mode, int32_t check_every, float tolerance);
+/* Relative error of the last check per trajectory, whether the fallback has
+ * fired, and how many bead predictions failed the positivity check. */
+int mte_mlp_status(mte_engine *e, double *last_error, int32_t *fallback, int64_t *bad_beads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmt_engine.so")
-SOURCES = ["engine.hip", "forces.hip", "lists.hip", "cells.hip", "tea.hip", "resident.hip"]
+SOURCES = ["engine.hip", "forces.hip", "lists.hip", "cells.hip", "tea.hip", "mlp.hip", "resident.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics",

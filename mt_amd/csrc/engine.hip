@@ -125,6 +125,9 @@ int mte_destroy(mte_engine *e) {
     flush_timers(e);
     for (auto &kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
     for (void *p : e->allocations) (void)hipFree(p);
+    for (void *p : {(void *)e->mlp.x, (void *)e->mlp.t, (void *)e->mlp.hid, (void *)e->mlp.y, (void *)e->mlp.g4,
+                    (void *)e->mlp.ga, (void *)e->mlp.gb, (void *)e->mlp.partial})
+        if (p) (void)hipFree(p);
     if (e->noise) (void)hipFree(e->noise);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
@@ -316,7 +319,7 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
         }
         e->use_cells = cells_ok && (opt->list_method == MTE_LISTS_CELLS ||
                                     (opt->list_method == MTE_LISTS_AUTO && nmax > 2048));
-        if (e->use_cells) {
+        if (e->use_cells || e->P.tea_on) {   // MLP-HI's neighbour search needs the tables too
             std::vector<int> toff(nt), tsz(nt);
             int total = 0;
             for (int t = 0; t < nt; t++) {
@@ -344,6 +347,23 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
                 return bail(MTE_ERR_HIP);
             }
         }
+    }
+    if (e->P.tea_on) {
+        MlpBuffers &M = e->mlp;
+        if (dalloc(e, &M.w, MLP_NPARAM) != hipSuccess || dalloc(e, &M.wi, MLP_NPARAM) != hipSuccess ||
+            dalloc(e, &M.grad, MLP_NPARAM) != hipSuccess || dalloc(e, &M.m1, MLP_NPARAM) != hipSuccess ||
+            dalloc(e, &M.m2, MLP_NPARAM) != hipSuccess || dalloc(e, &M.feat, (size_t)MLP_IN * nb) != hipSuccess ||
+            dalloc(e, &M.targ, (size_t)MLP_OUT * nb) != hipSuccess ||
+            dalloc(e, &M.ystep, (size_t)MLP_OUT * nb) != hipSuccess || dalloc(e, &M.err, 2 * (size_t)nt) != hipSuccess ||
+            dalloc(e, &M.nbad, 1) != hipSuccess) {
+            e->err = "mte_create: hipMalloc failed for MLP buffers";
+            return bail(MTE_ERR_HIP);
+        }
+        if (hipMemsetAsync(M.nbad, 0, sizeof(int), e->stream) != hipSuccess) {
+            e->err = "mte_create: memset failed";
+            return bail(MTE_ERR_HIP);
+        }
+        e->mlp_last_err.assign(nt, 0.0);
     }
     ResidentPlan plan;
     const bool fits = resident_fits(e, &plan);
@@ -474,6 +494,8 @@ static int check_tea(mte_engine *e) {
     return MTE_OK;
 }
 
+static bool mlp_active(const mte_engine *e) { return e->hi_mode == MTE_HI_MLP && !e->mlp_fallback; }
+
 // Enqueue `len` streamed steps starting at global step s0 (list rebuilds and
 // TEA refreshes at their periods), flipping e->cur per step.  The same
 // sequence is launched directly or recorded into a graph.
@@ -491,7 +513,17 @@ static hipError_t enqueue_streamed(mte_engine *e, int64_t s0, int64_t len, int64
         }
         const float *nz = e->noise ? e->noise + (size_t)6 * e->nb * (noise_cursor + k) : nullptr;
         hipEvent_t ev{};
-        if (e->P.tea_on) {
+        if (mlp_active(e)) {
+            // forces, features (+ exact labels on check steps), fused prediction + integration
+            const bool check = step % e->mlp_check_every == 0;
+            st = launch_forces(e);
+            if (st == hipSuccess) st = launch_mlp_local(e, e->mlp.feat, check ? e->mlp.targ : nullptr);
+            if (st != hipSuccess) break;
+            time_begin(e, "mlp_step", &ev);
+            st = launch_mlp_step(e, nz, check ? e->mlp.ystep : nullptr);
+            time_end(e, "mlp_step", ev);
+            if (st == hipSuccess && check) st = launch_mlp_error(e, e->mlp.ystep, e->mlp.targ);
+        } else if (e->P.tea_on) {
             st = launch_forces_tea(e, nz);
             if (st == hipSuccess && step % e->P.tea_epsilon_freq == 0) st = launch_tea_eps(e);
             if (st != hipSuccess) break;
@@ -508,13 +540,40 @@ static hipError_t enqueue_streamed(mte_engine *e, int64_t s0, int64_t len, int64
     return st;
 }
 
-// Steps from `step` to the next list rebuild or TEA refresh (exclusive), so
-// that a segment only carries events at its first step.
+// Steps from `step` to the next list rebuild, TEA refresh or MLP check
+// (exclusive), so that a segment only carries events at its first step.
 static int64_t steps_to_next_event(const mte_engine *e, int64_t step) {
     const int64_t f = e->P.ljpairsupdatefreq;
     int64_t n = f - (step % f);
     if (e->P.tea_on) n = std::min<int64_t>(n, e->P.tea_epsilon_freq - (step % e->P.tea_epsilon_freq));
+    if (mlp_active(e)) n = std::min<int64_t>(n, e->mlp_check_every - (step % e->mlp_check_every));
     return n;
+}
+
+// Sampled-error judge of MLP-HI: relative error of the predicted A - I
+// against the exact labels per trajectory; above the tolerance the run
+// continues with TEA (the fallback), whose epsilon/beta are refreshed first.
+static int check_mlp(mte_engine *e) {
+    std::vector<double> se(2 * e->ntraj);
+    int bad = 0;
+    HIPCHK(hipMemcpyAsync(se.data(), e->mlp.err, sizeof(double) * se.size(), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(&bad, e->mlp.nbad, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->mlp_bad = bad;
+    double worst = 0.0;
+    for (int t = 0; t < e->ntraj; t++) {
+        const double r = se[2 * t + 1] > 0 ? std::sqrt(se[2 * t] / se[2 * t + 1]) : 0.0;
+        e->mlp_last_err[t] = r;
+        worst = std::max(worst, r);
+    }
+    if (worst > e->mlp_tol || bad > 0) {
+        e->mlp_fallback = true;
+        HIPCHK(launch_tea_eps(e));
+        int rc = check_tea(e);
+        if (rc) return rc;
+        e->tea_valid = true;
+    }
+    return MTE_OK;
 }
 
 int mte_run(mte_engine *e, int64_t nsteps) {
@@ -543,14 +602,18 @@ int mte_run(mte_engine *e, int64_t nsteps) {
             continue;
         }
         const int64_t seg = std::min<int64_t>(nsteps - done, steps_to_next_event(e, e->step));
-        const bool tea_refresh = e->P.tea_on && e->step % e->P.tea_epsilon_freq == 0;
+        const bool mlp_now = mlp_active(e);
+        const bool mlp_check = mlp_now && e->step % e->mlp_check_every == 0;
+        const bool tea_refresh = !mlp_now && e->P.tea_on && e->step % e->P.tea_epsilon_freq == 0;
         const bool rebuilds = need_lists && rebuild_due(e);
         if (e->use_graphs && !e->noise && !e->timing) {
             // The rebuild-at-first-step decision is part of the key: a segment
             // starting on a rebuild step whose lists already exist skips it.
             const int64_t lj_phase = rebuilds ? e->step % freq : (e->step % freq == 0 ? -1 : e->step % freq);
-            const auto key = std::make_tuple(lj_phase, e->P.tea_on ? e->step % e->P.tea_epsilon_freq : 0, seg,
-                                             e->cur);
+            const auto key = std::make_tuple(lj_phase,
+                                             mlp_now ? -1 - e->step % e->mlp_check_every
+                                                     : (e->P.tea_on ? e->step % e->P.tea_epsilon_freq : 0),
+                                             seg, e->cur);
             auto it = e->graphs.find(key);
             const int cur0 = e->cur;
             if (it == e->graphs.end()) {
@@ -574,6 +637,17 @@ int mte_run(mte_engine *e, int64_t nsteps) {
         }
         if (rebuilds) { e->lists_step = e->step; e->lists_valid = true; }
         if (tea_refresh) {
+            int rc = check_tea(e);
+            if (rc) return rc;
+            e->tea_valid = true;
+        }
+        if (mlp_check) {
+            int rc = check_mlp(e);
+            if (rc) return rc;
+        }
+        if (!mlp_now && e->P.tea_on && !e->tea_valid && !tea_refresh) {
+            // first TEA segment not on a refresh step: compute epsilon/beta now
+            HIPCHK(launch_tea_eps(e));
             int rc = check_tea(e);
             if (rc) return rc;
             e->tea_valid = true;
@@ -708,6 +782,187 @@ int mte_tea_stats(mte_engine *e, double *eps, double *beta) {
         if (eps) eps[t] = st[2 * t];
         if (beta) beta[t] = st[2 * t + 1];
     }
+    return MTE_OK;
+}
+
+// ------------------------------------------------------------------ MLP-HI
+static int need_mlp(mte_engine *e) {
+    if (!e) return MTE_ERR_ARG;
+    if (!e->P.tea_on) return fail(e, MTE_ERR_STATE, "MLP-HI needs TEA parameters (tea_on, tea_a)");
+    std::vector<float> box(e->ntraj);
+    if (hipMemcpy(box.data(), e->T.tbox, sizeof(float) * e->ntraj, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(e, MTE_ERR_HIP, "need_mlp: box download failed");
+    for (float L : box)
+        if (L > 0.f && L < 36.f) return fail(e, MTE_ERR_ARG, "MLP-HI needs periodic boxes of at least 36 nm");
+    return MTE_OK;
+}
+
+static int mlp_reserve(mte_engine *e, int n) {
+    MlpBuffers &M = e->mlp;
+    if (n <= M.cap) return MTE_OK;
+    for (void *p : {(void *)M.x, (void *)M.t, (void *)M.hid, (void *)M.y, (void *)M.g4, (void *)M.ga, (void *)M.gb,
+                    (void *)M.partial})
+        if (p) {
+            (void)hipStreamSynchronize(e->stream);
+            (void)hipFree(p);
+        }
+    M.cap = 0;
+    const size_t nn = (size_t)n;
+    if (hipMalloc(&M.x, sizeof(float) * nn * MLP_IN) != hipSuccess ||
+        hipMalloc(&M.t, sizeof(float) * nn * MLP_OUT) != hipSuccess ||
+        hipMalloc(&M.hid, sizeof(float) * 3 * nn * MLP_H) != hipSuccess ||
+        hipMalloc(&M.y, sizeof(float) * nn * MLP_OUT) != hipSuccess ||
+        hipMalloc(&M.g4, sizeof(float) * nn * MLP_OUT) != hipSuccess ||
+        hipMalloc(&M.ga, sizeof(float) * nn * MLP_H) != hipSuccess ||
+        hipMalloc(&M.gb, sizeof(float) * nn * MLP_H) != hipSuccess ||
+        hipMalloc(&M.partial, sizeof(double) * ((nn * MLP_OUT + 255) / 256 + 1)) != hipSuccess)
+        return fail(e, MTE_ERR_HIP, "MLP work space allocation failed");
+    M.cap = n;
+    return MTE_OK;
+}
+
+static double mlp_loss(mte_engine *e, int n) {
+    const int nblk = (n * MLP_OUT + 255) / 256;
+    std::vector<double> part(nblk);
+    if (hipMemcpyAsync(part.data(), e->mlp.partial, sizeof(double) * nblk, hipMemcpyDeviceToHost, e->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(e->stream) != hipSuccess)
+        return -1.0;
+    double s = 0;
+    for (double v : part) s += v;
+    return s / ((double)n * MLP_OUT);
+}
+
+int mte_mlp_set_weights(mte_engine *e, const float *w, int64_t n) {
+    int rc = need_mlp(e);
+    if (rc) return rc;
+    if (!w || n != MLP_NPARAM) return fail(e, MTE_ERR_ARG, "mte_mlp_set_weights: expected MTE_MLP_NPARAM weights");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipMemcpyAsync(e->mlp.w, w, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(launch_mlp_round(e));
+    HIPCHK(hipMemsetAsync(e->mlp.m1, 0, sizeof(float) * MLP_NPARAM, e->stream));
+    HIPCHK(hipMemsetAsync(e->mlp.m2, 0, sizeof(float) * MLP_NPARAM, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->mlp.adam_t = 0;
+    e->mlp.ready = true;
+    return MTE_OK;
+}
+
+int mte_mlp_get_weights(mte_engine *e, float *w, int64_t n) {
+    int rc = need_mlp(e);
+    if (rc) return rc;
+    if (!w || n != MLP_NPARAM) return fail(e, MTE_ERR_ARG, "mte_mlp_get_weights: expected MTE_MLP_NPARAM weights");
+    HIPCHK(hipMemcpyAsync(w, e->mlp.w, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return MTE_OK;
+}
+
+int mte_mlp_local(mte_engine *e, float *features, float *labels) {
+    int rc = need_mlp(e);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(launch_mlp_local(e, e->mlp.feat, e->mlp.targ));
+    if (features)
+        HIPCHK(hipMemcpyAsync(features, e->mlp.feat, sizeof(float) * MLP_IN * e->nb, hipMemcpyDeviceToHost, e->stream));
+    if (labels)
+        HIPCHK(hipMemcpyAsync(labels, e->mlp.targ, sizeof(float) * MLP_OUT * e->nb, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return MTE_OK;
+}
+
+int mte_mlp_forward(mte_engine *e, const float *features, int64_t n, float *out) {
+    int rc = need_mlp(e);
+    if (rc) return rc;
+    if (!features || !out || n <= 0) return fail(e, MTE_ERR_ARG, "mte_mlp_forward: bad arguments");
+    HIPCHK(hipSetDevice(e->device));
+    rc = mlp_reserve(e, (int)n);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(e->mlp.x, features, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(launch_mlp_forward(e, e->mlp.x, (int)n, e->mlp.y));
+    HIPCHK(hipMemcpyAsync(out, e->mlp.y, sizeof(float) * MLP_OUT * n, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return MTE_OK;
+}
+
+int mte_mlp_loss_grad(mte_engine *e, const float *features, const float *labels, int64_t n, double *loss,
+                      float *grad) {
+    int rc = need_mlp(e);
+    if (rc) return rc;
+    if (!features || !labels || n <= 0) return fail(e, MTE_ERR_ARG, "mte_mlp_loss_grad: bad arguments");
+    HIPCHK(hipSetDevice(e->device));
+    rc = mlp_reserve(e, (int)n);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(e->mlp.x, features, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->mlp.t, labels, sizeof(float) * MLP_OUT * n, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(launch_mlp_grad(e, e->mlp.x, e->mlp.t, (int)n));
+    if (grad) HIPCHK(hipMemcpyAsync(grad, e->mlp.grad, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, e->stream));
+    const double l = mlp_loss(e, (int)n);
+    if (loss) *loss = l;
+    return MTE_OK;
+}
+
+int mte_mlp_train(mte_engine *e, const float *features, const float *labels, int64_t n, int32_t iters,
+                  int32_t batch, float lr, double *losses) {
+    int rc = need_mlp(e);
+    if (rc) return rc;
+    if (!features || !labels || n <= 0 || iters < 0 || batch <= 0 || batch > n)
+        return fail(e, MTE_ERR_ARG, "mte_mlp_train: bad arguments");
+    if (!e->mlp.ready) return fail(e, MTE_ERR_STATE, "mte_mlp_train: set initial weights first");
+    HIPCHK(hipSetDevice(e->device));
+    // the whole training set stays on the device; batches are consecutive slices
+    float *X = nullptr, *Y = nullptr;
+    HIPCHK(hipMalloc(&X, sizeof(float) * MLP_IN * n));
+    if (hipMalloc(&Y, sizeof(float) * MLP_OUT * n) != hipSuccess) {
+        (void)hipFree(X);
+        return fail(e, MTE_ERR_HIP, "mte_mlp_train: allocation failed");
+    }
+    rc = mlp_reserve(e, batch);
+    int out = rc;
+    if (!rc) {
+        if (hipMemcpyAsync(X, features, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice, e->stream) != hipSuccess ||
+            hipMemcpyAsync(Y, labels, sizeof(float) * MLP_OUT * n, hipMemcpyHostToDevice, e->stream) != hipSuccess)
+            out = fail(e, MTE_ERR_HIP, "mte_mlp_train: upload failed");
+        int64_t pos = 0;
+        for (int it = 0; it < iters && !out; it++) {
+            if (pos + batch > n) pos = 0;
+            hipError_t s = launch_mlp_grad(e, X + pos * MLP_IN, Y + pos * MLP_OUT, batch);
+            if (s == hipSuccess) s = launch_mlp_adam(e, lr, ++e->mlp.adam_t);
+            if (s != hipSuccess) out = fail(e, MTE_ERR_HIP, std::string("mte_mlp_train: ") + hipGetErrorString(s));
+            if (losses && !out) losses[it] = mlp_loss(e, batch);
+            pos += batch;
+        }
+        (void)hipStreamSynchronize(e->stream);
+    }
+    (void)hipFree(X);
+    (void)hipFree(Y);
+    return out;
+}
+
+int mte_set_hi_mode(mte_engine *e, int32_t mode, int32_t check_every, float tolerance) {
+    if (!e) return MTE_ERR_ARG;
+    if (mode != MTE_HI_DEFAULT && mode != MTE_HI_MLP) return fail(e, MTE_ERR_ARG, "mte_set_hi_mode: unknown mode");
+    if (mode == MTE_HI_MLP) {
+        int rc = need_mlp(e);
+        if (rc) return rc;
+        if (!e->mlp.ready) return fail(e, MTE_ERR_STATE, "mte_set_hi_mode: MLP weights not set");
+        if (e->mode != MTE_MODE_STREAMED) return fail(e, MTE_ERR_STATE, "MLP-HI runs in streamed mode");
+        if (check_every <= 0 || !(tolerance > 0.f)) return fail(e, MTE_ERR_ARG, "mte_set_hi_mode: bad check settings");
+    }
+    e->hi_mode = mode;
+    e->mlp_check_every = check_every > 0 ? check_every : 100;
+    e->mlp_tol = tolerance > 0.f ? tolerance : 0.02f;
+    e->mlp_fallback = false;
+    e->tea_valid = false;
+    return MTE_OK;
+}
+
+int mte_mlp_status(mte_engine *e, double *last_error, int32_t *fallback, int64_t *bad_beads) {
+    int rc = need_mlp(e);
+    if (rc) return rc;
+    if (last_error)
+        for (int t = 0; t < e->ntraj; t++) last_error[t] = e->mlp_last_err[t];
+    if (fallback) *fallback = e->mlp_fallback ? 1 : 0;
+    if (bad_beads) *bad_beads = e->mlp_bad;
     return MTE_OK;
 }
 

@@ -45,6 +45,23 @@ struct CellTables {
     int total = 0;               // buckets over all trajectories
 };
 
+// MLP-HI network and work buffers (mlp.hip).
+constexpr int MLP_NN = 32, MLP_IN = 96, MLP_H = 128, MLP_OUT = 9;
+constexpr int MLP_NPARAM = MLP_IN * MLP_H + MLP_H + 2 * (MLP_H * MLP_H + MLP_H) + MLP_H * MLP_OUT + MLP_OUT;
+struct MlpBuffers {
+    float *w = nullptr, *wi = nullptr;        // FP32 master weights, bf16-rounded copy for inference
+    float *grad = nullptr, *m1 = nullptr, *m2 = nullptr;
+    float *feat = nullptr, *targ = nullptr, *ystep = nullptr;   // per bead
+    double *err = nullptr;                    // per trajectory: squared error, squared label norm
+    int *nbad = nullptr;                      // beads whose prediction failed the Cholesky check
+    // training work space for up to `cap` samples
+    int cap = 0;
+    float *x = nullptr, *t = nullptr, *hid = nullptr, *y = nullptr, *g4 = nullptr, *ga = nullptr, *gb = nullptr;
+    double *partial = nullptr;
+    int adam_t = 0;
+    bool ready = false;                       // weights uploaded or trained
+};
+
 // Resident-mode launch plan: trajectories grouped by workgroup.
 struct ResidentPlan {
     int threads = 0;        // workgroup size
@@ -81,6 +98,13 @@ struct mte_engine {
     mte::TeaBuffers tea;
     mte::CellTables cells;
     bool use_cells = false;
+    mte::MlpBuffers mlp;
+    int hi_mode = 0;            // MTE_HI_*
+    int mlp_check_every = 100;
+    float mlp_tol = 0.02f;
+    bool mlp_fallback = false;
+    std::vector<double> mlp_last_err;
+    int64_t mlp_bad = 0;
     mte::ResidentPlan rplan;
 
     int64_t step = 0;
@@ -108,6 +132,13 @@ hipError_t launch_forces_tea(mte_engine *e, const float *noise_step);
 hipError_t launch_energies(mte_engine *e);
 hipError_t launch_rebuild(mte_engine *e);
 hipError_t launch_rebuild_cells(mte_engine *e);
+hipError_t launch_mlp_local(mte_engine *e, float *feat, float *targ);
+hipError_t launch_mlp_forward(mte_engine *e, const float *feat, int n, float *out);
+hipError_t launch_mlp_step(mte_engine *e, const float *noise_step, float *yout);
+hipError_t launch_mlp_error(mte_engine *e, const float *y, const float *t);
+hipError_t launch_mlp_grad(mte_engine *e, const float *feat, const float *targ, int n);
+hipError_t launch_mlp_adam(mte_engine *e, float lr, int t);
+hipError_t launch_mlp_round(mte_engine *e);
 hipError_t build_cells(mte_engine *e, float radius);
 float candidate_radius(const mte_params &P);
 hipError_t launch_tea_eps(mte_engine *e);

@@ -1,0 +1,480 @@
+// mlp.hip -- the learned hydrodynamic correction (MLP-HI); the reference has
+// no counterpart.
+//
+// Per bead i the engine predicts the local hydrodynamic mobility
+//   A_i = I + sum_{j != i, r_ij < 12 nm} phi(r_ij) D_hat_ij
+// (D_hat the RPY block of src/bdhitea_kernel.cu:38-58, phi = 1 up to 8 nm and
+// a cosine taper to 0 at 12 nm) from the relative positions of its 32 nearest
+// neighbours (96 inputs), with a 96-128-128-128-9 tanh network, and moves the
+// bead by
+//   dr_i = (dt/gammaR) A_i F_i + sqrt(2 kB T dt / gammaR) L_i xi_i,  L_i L_i^T = A_i.
+// For a rigid cluster this gives the Kirkwood centre-of-mass mobility
+// (1/n^2) sum_ij D_hat_ij in both drift and noise.  The exact A_i (the same
+// neighbour sum, all neighbours within 12 nm) is the training label and the
+// sampled-error judge; plain TEA is the fallback.
+//
+// Kernels: neighbour features and exact labels over the hashed cell tables
+// (cs >= 12 nm); inference fused with the integrator, one bead per lane,
+// activations in an LDS row per lane, weights read as wave-uniform operands
+// (bf16-rounded values, FP32 accumulation); training (forward with stored
+// activations, backward, Adam) in FP32.
+#include "engine_internal.h"
+
+namespace mte {
+
+constexpr int MLP_BLOCK = 64;
+constexpr int ROW = MLP_H + 1;   // LDS row pitch (odd: no bank conflicts between lanes)
+constexpr float MLP_RC = 12.f, MLP_RIN = 8.f;
+// flat parameter offsets
+constexpr int OW1 = 0, OB1 = OW1 + MLP_IN * MLP_H, OW2 = OB1 + MLP_H, OB2 = OW2 + MLP_H * MLP_H, OW3 = OB2 + MLP_H,
+              OB3 = OW3 + MLP_H * MLP_H, OW4 = OB3 + MLP_H, OB4 = OW4 + MLP_H * MLP_OUT;
+static_assert(OB4 + MLP_OUT == MLP_NPARAM, "parameter layout");
+
+__device__ __forceinline__ float taper(float r) {
+    if (r <= MLP_RIN) return 1.f;
+    if (r >= MLP_RC) return 0.f;
+    float s, c;
+    sincos_fast(3.14159265358979f * (r - MLP_RIN) / (MLP_RC - MLP_RIN), &s, &c);
+    return 0.5f * (1.f + c);
+}
+
+// ------------------------------------------------------------ features, labels
+// Neighbour scan over the 27 cells of the hashed table (built with cell side
+// >= 12 nm).  Keeps the 32 nearest (distance, then index) and sums the exact
+// label over every neighbour within 12 nm.
+__global__ void __launch_bounds__(MLP_BLOCK)
+    k_mlp_local(mte_params P, DevTopo T, DevState S, CellTables C, float *feat, float *targ) {
+    __shared__ float sd[MLP_BLOCK * MLP_NN];
+    __shared__ int sj[MLP_BLOCK * MLP_NN];
+    const int b = blockIdx.x * MLP_BLOCK + threadIdx.x;
+    if (b >= T.nb) return;
+    float *nd = sd + threadIdx.x * MLP_NN;
+    int *nj = sj + threadIdx.x * MLP_NN;
+    const int t = T.tid[b];
+    const float box = T.tbox[t];
+    const float3 ri = f3(S.xyz[b]);
+    float A[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int cnt = 0;
+    if (!T.extra[b]) {
+        const int dims = box > 0.f ? max(3, (int)floorf(box / MLP_RC)) : 0;
+        const int4 me = C.cell[b];
+        const uint32_t mask = (uint32_t)(C.tsize[t] - 1);
+        const float a = P.tea_a, ia = 1.f / P.tea_a;
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    int cx = me.x + dx, cy = me.y + dy, cz = me.z + dz;
+                    if (dims > 0) {
+                        cx = (cx + dims) % dims;
+                        cy = (cy + dims) % dims;
+                        cz = (cz + dims) % dims;
+                    }
+                    const uint32_t h = ((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^
+                                       ((uint32_t)cz * 83492791u);
+                    const int bucket = C.toff[t] + (int)(h & mask);
+                    const int s0 = C.start[bucket], s1 = s0 + C.count[bucket];
+                    for (int q = s0; q < s1; q++) {
+                        const int j = C.sorted[q];
+                        if (j == b || T.extra[j]) continue;
+                        const int4 cj = C.cell[j];
+                        if (cj.x != cx || cj.y != cy || cj.z != cz) continue;
+                        const float3 d = min_image(sub3(f3(S.xyz[j]), ri), box);
+                        const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+                        if (r2 >= MLP_RC * MLP_RC) continue;
+                        // exact label: phi(r) D_hat(d)
+                        const float r = sqrtf(r2), rinv = 1.f / r;
+                        const float ra = r * ia;
+                        float cii, crr;
+                        if (ra > 2.f) {
+                            const float u = 1.f / ra, u2 = u * u;
+                            cii = 0.75f * u * (1.f + (2.f / 3.f) * u2);
+                            crr = 0.75f * u * (1.f - 2.f * u2);
+                        } else {
+                            cii = 1.f - 9.f * ra / 32.f;
+                            crr = 3.f * ra / 32.f;
+                        }
+                        const float w = taper(r);
+                        const float e[3] = {d.x * rinv, d.y * rinv, d.z * rinv};
+#pragma unroll
+                        for (int x = 0; x < 3; x++)
+#pragma unroll
+                            for (int y = 0; y < 3; y++) A[3 * x + y] += w * (crr * e[x] * e[y] + (x == y ? cii : 0.f));
+                        // 32 nearest, ordered by (r2, j)
+                        const int lj = j;
+                        if (cnt < MLP_NN || r2 < nd[cnt - 1] || (r2 == nd[cnt - 1] && lj < nj[cnt - 1])) {
+                            int k = cnt < MLP_NN ? cnt++ : MLP_NN - 1;
+                            while (k > 0 && (nd[k - 1] > r2 || (nd[k - 1] == r2 && nj[k - 1] > lj))) {
+                                nd[k] = nd[k - 1];
+                                nj[k] = nj[k - 1];
+                                k--;
+                            }
+                            nd[k] = r2;
+                            nj[k] = lj;
+                        }
+                    }
+                }
+        (void)a;
+    }
+    if (feat) {
+        float *f = feat + (size_t)b * MLP_IN;
+        for (int k = 0; k < MLP_NN; k++) {
+            float3 d = make_float3(0.f, 0.f, 0.f);
+            if (k < cnt) {
+                d = min_image(sub3(f3(S.xyz[nj[k]]), ri), box);
+                d = make_float3(d.x / MLP_RC, d.y / MLP_RC, d.z / MLP_RC);
+            }
+            f[3 * k] = d.x;
+            f[3 * k + 1] = d.y;
+            f[3 * k + 2] = d.z;
+        }
+    }
+    if (targ) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) targ[(size_t)b * MLP_OUT + q] = A[q];
+    }
+}
+
+// ------------------------------------------------------------------ network
+// One dense layer for this lane's sample: row (LDS, K inputs) -> row (M
+// outputs), weights W[k * M + m] and bias, tanh when `act`.  Outputs are
+// produced in chunks of 64 accumulators.
+template <int K, int M>
+__device__ __forceinline__ void dense_row(float *row, const float *__restrict__ W, const float *__restrict__ bias,
+                                          bool act, float *out_regs) {
+    constexpr int CH = M < 64 ? M : 64;
+    float res[M];
+#pragma unroll
+    for (int m0 = 0; m0 < M; m0 += CH) {
+        float acc[CH];
+#pragma unroll
+        for (int m = 0; m < CH; m++) acc[m] = bias[m0 + m];
+        for (int k = 0; k < K; k++) {
+            const float a = row[k];
+            const float *w = W + k * M + m0;
+#pragma unroll
+            for (int m = 0; m < CH; m++) acc[m] = fmaf(a, w[m], acc[m]);
+        }
+#pragma unroll
+        for (int m = 0; m < CH; m++) res[m0 + m] = act ? tanhf(acc[m]) : acc[m];
+    }
+    if (out_regs) {
+#pragma unroll
+        for (int m = 0; m < M; m++) out_regs[m] = res[m];
+    } else {
+#pragma unroll
+        for (int m = 0; m < M; m++) row[m] = res[m];
+    }
+}
+
+// Full network on the lane's LDS row (features in, 9 outputs to y).
+// keep != nullptr stores the three hidden activations (training).
+__device__ __forceinline__ void mlp_eval(float *row, const float *__restrict__ w, float y[MLP_OUT], float *keep,
+                                         size_t stride) {
+    dense_row<MLP_IN, MLP_H>(row, w + OW1, w + OB1, true, nullptr);
+    if (keep)
+        for (int m = 0; m < MLP_H; m++) keep[m] = row[m];
+    dense_row<MLP_H, MLP_H>(row, w + OW2, w + OB2, true, nullptr);
+    if (keep)
+        for (int m = 0; m < MLP_H; m++) keep[stride + m] = row[m];
+    dense_row<MLP_H, MLP_H>(row, w + OW3, w + OB3, true, nullptr);
+    if (keep)
+        for (int m = 0; m < MLP_H; m++) keep[2 * stride + m] = row[m];
+    dense_row<MLP_H, MLP_OUT>(row, w + OW4, w + OB4, false, y);
+}
+
+__global__ void __launch_bounds__(MLP_BLOCK)
+    k_mlp_forward(const float *__restrict__ w, const float *feat, int n, float *out) {
+    __shared__ float rows[MLP_BLOCK * ROW];
+    const int s = blockIdx.x * MLP_BLOCK + threadIdx.x;
+    if (s >= n) return;
+    float *row = rows + threadIdx.x * ROW;
+    for (int k = 0; k < MLP_IN; k++) row[k] = feat[(size_t)s * MLP_IN + k];
+    float y[MLP_OUT];
+    mlp_eval(row, w, y, nullptr, 0);
+    for (int q = 0; q < MLP_OUT; q++) out[(size_t)s * MLP_OUT + q] = y[q];
+}
+
+// Symmetrised A = I + (Y + Y^T)/2 and its Cholesky factor; a pivot below
+// 0.05 marks the prediction unusable and falls back to A = I for the bead.
+__device__ __forceinline__ bool mobility_chol(const float y[9], float A[9], float Lc[6]) {
+#pragma unroll
+    for (int x = 0; x < 3; x++)
+#pragma unroll
+        for (int z = 0; z < 3; z++) A[3 * x + z] = 0.5f * (y[3 * x + z] + y[3 * z + x]) + (x == z ? 1.f : 0.f);
+    const float l00s = A[0];
+    bool ok = l00s > 0.05f;
+    const float l00 = sqrtf(fmaxf(l00s, 0.05f));
+    const float l10 = A[3] / l00, l20 = A[6] / l00;
+    const float l11s = A[4] - l10 * l10;
+    ok = ok && l11s > 0.05f;
+    const float l11 = sqrtf(fmaxf(l11s, 0.05f));
+    const float l21 = (A[7] - l20 * l10) / l11;
+    const float l22s = A[8] - l20 * l20 - l21 * l21;
+    ok = ok && l22s > 0.05f;
+    const float l22 = sqrtf(fmaxf(l22s, 0.05f));
+    if (!ok) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) A[q] = (q % 4 == 0) ? 1.f : 0.f;
+        Lc[0] = 1.f; Lc[1] = 0.f; Lc[2] = 1.f; Lc[3] = 0.f; Lc[4] = 0.f; Lc[5] = 1.f;
+        return false;
+    }
+    Lc[0] = l00; Lc[1] = l10; Lc[2] = l11; Lc[3] = l20; Lc[4] = l21; Lc[5] = l22;
+    return true;
+}
+
+// MLP-HI step: prediction fused with the integrator.  Noise as the plain
+// integrator draws it (mobile beads only, xyz then angular stream).
+__global__ void __launch_bounds__(MLP_BLOCK)
+    k_mlp_step(mte_params P, DevTopo T, DevState Sin, DevState Sout, const float4 *fxyz, const float4 *fang,
+               const float *feat, const float *__restrict__ w, uint4 *seeds, const float *noise, float *yout,
+               int *nbad, LatOffsets L) {
+    __shared__ float rows[MLP_BLOCK * ROW];
+    const int b = blockIdx.x * MLP_BLOCK + threadIdx.x;
+    if (b >= T.nb) return;
+    float4 x = Sin.xyz[b], an = Sin.ang[b];
+    float4 n4 = Sin.fn[b], p1 = Sin.fp1[b], p2 = Sin.fp2[b];
+    const bool mobile = !T.fixed[b] && !T.extra[b];
+    if (mobile) {
+        float *row = rows + threadIdx.x * ROW;
+        for (int k = 0; k < MLP_IN; k++) row[k] = feat[(size_t)b * MLP_IN + k];
+        float y[MLP_OUT];
+        mlp_eval(row, w, y, nullptr, 0);
+        if (yout)
+            for (int q = 0; q < MLP_OUT; q++) yout[(size_t)b * MLP_OUT + q] = y[q];
+        float A[9], Lc[6];
+        if (!mobility_chol(y, A, Lc)) atomicAdd(nbad, 1);
+        float4 rx, ra;
+        if (noise) {
+            const float *nz = noise + 6 * (size_t)b;
+            rx = make_float4(nz[0], nz[1], nz[2], 0.f);
+            ra = make_float4(nz[3], nz[4], nz[5], 0.f);
+        } else {
+            uint4 s = seeds[b];
+            rx = normals4(s);
+            seeds[b] = s;
+            s = seeds[b + T.nb];
+            ra = normals4(s);
+            seeds[b + T.nb] = s;
+        }
+        const float4 F = fxyz[b];
+        const float dtg = P.dt / P.gammaR;
+        const float AF0 = A[0] * F.x + A[1] * F.y + A[2] * F.z;
+        const float AF1 = A[3] * F.x + A[4] * F.y + A[5] * F.z;
+        const float AF2 = A[6] * F.x + A[7] * F.y + A[8] * F.z;
+        const float n0 = Lc[0] * rx.x, n1 = Lc[1] * rx.x + Lc[2] * rx.y, n2 = Lc[3] * rx.x + Lc[4] * rx.y + Lc[5] * rx.z;
+        x.x += dtg * AF0 + P.varR * n0;
+        x.y += dtg * AF1 + P.varR * n1;
+        x.z += dtg * AF2 + P.varR * n2;
+        const float4 fa = fang[b];
+        const float dta = P.dt / (P.gammaTheta * P.alpha), dtt = P.dt / P.gammaTheta;
+        const float vta = P.varTheta * sqrtf(P.freeze_temp / P.alpha);
+        an.x += dta * fa.x + vta * ra.x;
+        an.z += dta * fa.z + vta * ra.y;
+        an.y += dtt * fa.y + P.varTheta * ra.z;
+        frame_vectors(an.x, an.y, an.z, L, n4, p1, p2);
+    } else if (yout) {
+        for (int q = 0; q < MLP_OUT; q++) yout[(size_t)b * MLP_OUT + q] = 0.f;
+    }
+    Sout.xyz[b] = x;
+    Sout.ang[b] = an;
+    Sout.fn[b] = n4;
+    Sout.fp1[b] = p1;
+    Sout.fp2[b] = p2;
+}
+
+// Per-trajectory squared error and squared label norm of predictions y
+// against labels t over mobile beads (sampled-error judge), FP64 sums.
+__global__ void __launch_bounds__(256)
+    k_mlp_error(DevTopo T, const float *y, const float *t, double *out) {
+    __shared__ double se[256], sn[256];
+    const int tr = blockIdx.x;
+    const int base = T.tbase[tr], n = T.tsize[tr];
+    double e = 0, m = 0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const int b = base + i;
+        if (T.fixed[b] || T.extra[b]) continue;
+        for (int q = 0; q < MLP_OUT; q++) {
+            const double d = (double)y[(size_t)b * MLP_OUT + q] - (double)t[(size_t)b * MLP_OUT + q];
+            e += d * d;
+            m += (double)t[(size_t)b * MLP_OUT + q] * (double)t[(size_t)b * MLP_OUT + q];
+        }
+    }
+    se[threadIdx.x] = e;
+    sn[threadIdx.x] = m;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) {
+            se[threadIdx.x] += se[threadIdx.x + k];
+            sn[threadIdx.x] += sn[threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * tr] = se[0];
+        out[2 * tr + 1] = sn[0];
+    }
+}
+
+// ----------------------------------------------------------------- training
+__global__ void __launch_bounds__(MLP_BLOCK)
+    k_mlp_train_fwd(const float *__restrict__ w, const float *feat, int n, float *hid, float *y) {
+    __shared__ float rows[MLP_BLOCK * ROW];
+    const int s = blockIdx.x * MLP_BLOCK + threadIdx.x;
+    if (s >= n) return;
+    float *row = rows + threadIdx.x * ROW;
+    for (int k = 0; k < MLP_IN; k++) row[k] = feat[(size_t)s * MLP_IN + k];
+    float out[MLP_OUT];
+    // hidden layers stored sample-major: hid[(layer * n + s) * H + m]
+    mlp_eval(row, w, out, hid + (size_t)s * MLP_H, (size_t)n * MLP_H);
+    for (int q = 0; q < MLP_OUT; q++) y[(size_t)s * MLP_OUT + q] = out[q];
+}
+
+// dL/dy = 2 (y - t) / (n * 9); per-block squared-error partials.
+__global__ void __launch_bounds__(256)
+    k_mlp_out_grad(const float *y, const float *t, int n, float *g, double *partial) {
+    __shared__ double red[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double e = 0;
+    if (i < n * MLP_OUT) {
+        const float d = y[i] - t[i];
+        g[i] = 2.f * d / (float)(n * MLP_OUT);
+        e = (double)d * d;
+    }
+    red[threadIdx.x] = e;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// dW[k][m] = sum_s in[s][k] g[s][m]; row k == K holds db[m] = sum_s g[s][m].
+__global__ void __launch_bounds__(256)
+    k_mlp_dw(const float *in, const float *g, int n, int K, int M, float *dW, float *db) {
+    const int id = blockIdx.x * 256 + threadIdx.x;
+    if (id >= (K + 1) * M) return;
+    const int k = id / M, m = id % M;
+    float acc = 0.f;
+    if (k < K) {
+        for (int s = 0; s < n; s++) acc = fmaf(in[(size_t)s * K + k], g[(size_t)s * M + m], acc);
+        dW[k * M + m] = acc;
+    } else {
+        for (int s = 0; s < n; s++) acc += g[(size_t)s * M + m];
+        db[m] = acc;
+    }
+}
+
+// gin[s][k] = (sum_m g[s][m] W[k][m]) (1 - h[s][k]^2)   (tanh derivative)
+__global__ void __launch_bounds__(256)
+    k_mlp_dx(const float *g, const float *__restrict__ W, const float *h, int n, int K, int M, float *gin) {
+    const int id = blockIdx.x * 256 + threadIdx.x;
+    if (id >= n * K) return;
+    const int s = id / K, k = id % K;
+    float acc = 0.f;
+    for (int m = 0; m < M; m++) acc = fmaf(g[(size_t)s * M + m], W[k * M + m], acc);
+    const float hv = h[(size_t)s * K + k];
+    gin[id] = acc * (1.f - hv * hv);
+}
+
+__device__ __forceinline__ float round_bf16(float v) {
+    uint32_t u = __float_as_uint(v);
+    u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+    return __uint_as_float(u);
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master weights; refreshes the
+// bf16-rounded copy used by inference.
+__global__ void __launch_bounds__(256)
+    k_mlp_adam(float *w, const float *g, float *m1, float *m2, float *wi, int n, float lr, float b1, float b2,
+               float eps, float c1, float c2) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float a = b1 * m1[i] + (1.f - b1) * gi;
+    const float v = b2 * m2[i] + (1.f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = v;
+    const float nw = w[i] - lr * (a / c1) / (sqrtf(v / c2) + eps);
+    w[i] = nw;
+    wi[i] = round_bf16(nw);
+}
+
+__global__ void __launch_bounds__(256) k_mlp_round(const float *w, float *wi, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) wi[i] = round_bf16(w[i]);
+}
+
+// ------------------------------------------------------------------ launchers
+static inline int blocks(int n, int bs) { return (n + bs - 1) / bs; }
+
+hipError_t launch_mlp_local(mte_engine *e, float *feat, float *targ) {
+    hipError_t s = build_cells(e, MLP_RC);
+    if (s != hipSuccess) return s;
+    hipLaunchKernelGGL(k_mlp_local, dim3(blocks(e->nb, MLP_BLOCK)), dim3(MLP_BLOCK), 0, e->stream, e->P, e->T,
+                       e->S[e->cur], e->cells, feat, targ);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_forward(mte_engine *e, const float *feat, int n, float *out) {
+    hipLaunchKernelGGL(k_mlp_forward, dim3(blocks(n, MLP_BLOCK)), dim3(MLP_BLOCK), 0, e->stream, e->mlp.wi, feat, n,
+                       out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_step(mte_engine *e, const float *noise_step, float *yout) {
+    hipLaunchKernelGGL(k_mlp_step, dim3(blocks(e->nb, MLP_BLOCK)), dim3(MLP_BLOCK), 0, e->stream, e->P, e->T,
+                       e->S[e->cur], e->S[1 - e->cur], e->fxyz, e->fang, e->mlp.feat, e->mlp.wi, e->seeds, noise_step,
+                       yout, e->mlp.nbad, e->L);
+    e->cur = 1 - e->cur;
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_error(mte_engine *e, const float *y, const float *t) {
+    hipLaunchKernelGGL(k_mlp_error, dim3(e->ntraj), dim3(256), 0, e->stream, e->T, y, t, e->mlp.err);
+    return hipGetLastError();
+}
+
+// Forward + backward over n samples already on the device; gradient into
+// e->mlp.grad, squared-error partials into e->mlp.partial.
+hipError_t launch_mlp_grad(mte_engine *e, const float *feat, const float *targ, int n) {
+    MlpBuffers &M = e->mlp;
+    const size_t H = MLP_H;
+    float *h1 = M.hid, *h2 = M.hid + (size_t)n * H, *h3 = M.hid + 2 * (size_t)n * H;
+    hipLaunchKernelGGL(k_mlp_train_fwd, dim3(blocks(n, MLP_BLOCK)), dim3(MLP_BLOCK), 0, e->stream, M.w, feat, n,
+                       M.hid, M.y);
+    hipLaunchKernelGGL(k_mlp_out_grad, dim3(blocks(n * MLP_OUT, 256)), dim3(256), 0, e->stream, M.y, targ, n, M.g4,
+                       M.partial);
+    float *gw = M.grad;
+    hipLaunchKernelGGL(k_mlp_dw, dim3(blocks((MLP_H + 1) * MLP_OUT, 256)), dim3(256), 0, e->stream, h3, M.g4, n,
+                       MLP_H, MLP_OUT, gw + OW4, gw + OB4);
+    hipLaunchKernelGGL(k_mlp_dx, dim3(blocks(n * MLP_H, 256)), dim3(256), 0, e->stream, M.g4, M.w + OW4, h3, n,
+                       MLP_H, MLP_OUT, M.ga);
+    hipLaunchKernelGGL(k_mlp_dw, dim3(blocks((MLP_H + 1) * MLP_H, 256)), dim3(256), 0, e->stream, h2, M.ga, n, MLP_H,
+                       MLP_H, gw + OW3, gw + OB3);
+    hipLaunchKernelGGL(k_mlp_dx, dim3(blocks(n * MLP_H, 256)), dim3(256), 0, e->stream, M.ga, M.w + OW3, h2, n,
+                       MLP_H, MLP_H, M.gb);
+    hipLaunchKernelGGL(k_mlp_dw, dim3(blocks((MLP_H + 1) * MLP_H, 256)), dim3(256), 0, e->stream, h1, M.gb, n, MLP_H,
+                       MLP_H, gw + OW2, gw + OB2);
+    hipLaunchKernelGGL(k_mlp_dx, dim3(blocks(n * MLP_H, 256)), dim3(256), 0, e->stream, M.gb, M.w + OW2, h1, n,
+                       MLP_H, MLP_H, M.ga);
+    hipLaunchKernelGGL(k_mlp_dw, dim3(blocks((MLP_IN + 1) * MLP_H, 256)), dim3(256), 0, e->stream, feat, M.ga, n,
+                       MLP_IN, MLP_H, gw + OW1, gw + OB1);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_adam(mte_engine *e, float lr, int t) {
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+    hipLaunchKernelGGL(k_mlp_adam, dim3(blocks(MLP_NPARAM, 256)), dim3(256), 0, e->stream, e->mlp.w, e->mlp.grad,
+                       e->mlp.m1, e->mlp.m2, e->mlp.wi, MLP_NPARAM, lr, b1, b2, 1e-8f, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_round(mte_engine *e) {
+    hipLaunchKernelGGL(k_mlp_round, dim3(blocks(MLP_NPARAM, 256)), dim3(256), 0, e->stream, e->mlp.w, e->mlp.wi,
+                       MLP_NPARAM);
+    return hipGetLastError();
+}
+
+}  // namespace mte

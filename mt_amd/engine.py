@@ -69,7 +69,17 @@ SIGNATURES = {
     "mte_kernel_time": ([_P, ctypes.c_char_p, _P, _P], ctypes.c_int),
     "mte_reset_timing": ([_P], ctypes.c_int),
     "mte_get_mode": ([_P, _P], ctypes.c_int),
+    "mte_mlp_set_weights": ([_P, _P, ctypes.c_int64], ctypes.c_int),
+    "mte_mlp_get_weights": ([_P, _P, ctypes.c_int64], ctypes.c_int),
+    "mte_mlp_local": ([_P, _P, _P], ctypes.c_int),
+    "mte_mlp_forward": ([_P, _P, ctypes.c_int64, _P], ctypes.c_int),
+    "mte_mlp_loss_grad": ([_P, _P, _P, ctypes.c_int64, _P, _P], ctypes.c_int),
+    "mte_mlp_train": ([_P, _P, _P, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, _P], ctypes.c_int),
+    "mte_set_hi_mode": ([_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_float], ctypes.c_int),
+    "mte_mlp_status": ([_P, _P, _P, _P], ctypes.c_int),
 }
+MLP_NPARAM, MLP_IN, MLP_OUT = 46601, 96, 9
+HI_DEFAULT, HI_MLP = 0, 1
 
 _LIB = None
 
@@ -232,3 +242,52 @@ class Engine:
         n = ctypes.c_int64()
         self._chk(self.lib.mte_kernel_time(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    # ---------------------------------------------------------------- MLP-HI
+    def mlp_set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == MLP_NPARAM
+        self._chk(self.lib.mte_mlp_set_weights(self.h, _p(w), w.size))
+
+    def mlp_get_weights(self):
+        w = np.zeros(MLP_NPARAM, np.float32)
+        self._chk(self.lib.mte_mlp_get_weights(self.h, _p(w), w.size))
+        return w
+
+    def mlp_local(self):
+        f = np.zeros((self.nb, MLP_IN), np.float32)
+        t = np.zeros((self.nb, MLP_OUT), np.float32)
+        self._chk(self.lib.mte_mlp_local(self.h, _p(f), _p(t)))
+        return f, t
+
+    def mlp_forward(self, feat):
+        x = np.ascontiguousarray(feat, np.float32)
+        out = np.zeros((len(x), MLP_OUT), np.float32)
+        self._chk(self.lib.mte_mlp_forward(self.h, _p(x), len(x), _p(out)))
+        return out
+
+    def mlp_loss_grad(self, feat, targ):
+        x = np.ascontiguousarray(feat, np.float32)
+        t = np.ascontiguousarray(targ, np.float32)
+        loss = ctypes.c_double()
+        g = np.zeros(MLP_NPARAM, np.float32)
+        self._chk(self.lib.mte_mlp_loss_grad(self.h, _p(x), _p(t), len(x), ctypes.byref(loss), _p(g)))
+        return loss.value, g
+
+    def mlp_train(self, feat, targ, iters, batch, lr):
+        x = np.ascontiguousarray(feat, np.float32)
+        t = np.ascontiguousarray(targ, np.float32)
+        losses = np.zeros(max(iters, 1), np.float64)
+        self._chk(self.lib.mte_mlp_train(self.h, _p(x), _p(t), len(x), int(iters), int(batch), float(lr),
+                                         _p(losses)))
+        return losses[:iters]
+
+    def set_hi_mode(self, mode, check_every=100, tolerance=0.02):
+        self._chk(self.lib.mte_set_hi_mode(self.h, int(mode), int(check_every), float(tolerance)))
+
+    def mlp_status(self):
+        err = np.zeros(self.ntraj)
+        fb = ctypes.c_int32()
+        bad = ctypes.c_int64()
+        self._chk(self.lib.mte_mlp_status(self.h, _p(err), ctypes.byref(fb), ctypes.byref(bad)))
+        return err, bool(fb.value), bad.value
