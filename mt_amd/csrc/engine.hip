@@ -514,9 +514,11 @@ static hipError_t enqueue_streamed(mte_engine *e, int64_t s0, int64_t len, int64
         const float *nz = e->noise ? e->noise + (size_t)6 * e->nb * (noise_cursor + k) : nullptr;
         hipEvent_t ev{};
         if (mlp_active(e)) {
-            // forces, features (+ exact labels on check steps), fused prediction + integration
+            // forces, TEA normalisation refresh, features (+ exact labels on
+            // check steps), fused prediction + integration
             const bool check = step % e->mlp_check_every == 0;
             st = launch_forces(e);
+            if (st == hipSuccess && step % e->P.tea_epsilon_freq == 0) st = launch_tea_eps(e);
             if (st == hipSuccess) st = launch_mlp_local(e, e->mlp.feat, check ? e->mlp.targ : nullptr);
             if (st != hipSuccess) break;
             time_begin(e, "mlp_step", &ev);
@@ -604,16 +606,22 @@ int mte_run(mte_engine *e, int64_t nsteps) {
         const int64_t seg = std::min<int64_t>(nsteps - done, steps_to_next_event(e, e->step));
         const bool mlp_now = mlp_active(e);
         const bool mlp_check = mlp_now && e->step % e->mlp_check_every == 0;
-        const bool tea_refresh = !mlp_now && e->P.tea_on && e->step % e->P.tea_epsilon_freq == 0;
+        const bool tea_refresh = e->P.tea_on && e->step % e->P.tea_epsilon_freq == 0;
+        if (e->P.tea_on && !e->tea_valid && !tea_refresh) {
+            // TEA's epsilon/beta must exist before the first TEA or MLP-HI step
+            HIPCHK(launch_tea_eps(e));
+            int rc = check_tea(e);
+            if (rc) return rc;
+            e->tea_valid = true;
+        }
         const bool rebuilds = need_lists && rebuild_due(e);
         if (e->use_graphs && !e->noise && !e->timing) {
             // The rebuild-at-first-step decision is part of the key: a segment
             // starting on a rebuild step whose lists already exist skips it.
             const int64_t lj_phase = rebuilds ? e->step % freq : (e->step % freq == 0 ? -1 : e->step % freq);
-            const auto key = std::make_tuple(lj_phase,
-                                             mlp_now ? -1 - e->step % e->mlp_check_every
-                                                     : (e->P.tea_on ? e->step % e->P.tea_epsilon_freq : 0),
-                                             seg, e->cur);
+            const int64_t hi_phase = (e->P.tea_on ? e->step % e->P.tea_epsilon_freq : 0) +
+                                     (mlp_now ? 1000003 * (1 + e->step % e->mlp_check_every) : 0);
+            const auto key = std::make_tuple(lj_phase, hi_phase, seg, e->cur);
             auto it = e->graphs.find(key);
             const int cur0 = e->cur;
             if (it == e->graphs.end()) {
@@ -645,13 +653,7 @@ int mte_run(mte_engine *e, int64_t nsteps) {
             int rc = check_mlp(e);
             if (rc) return rc;
         }
-        if (!mlp_now && e->P.tea_on && !e->tea_valid && !tea_refresh) {
-            // first TEA segment not on a refresh step: compute epsilon/beta now
-            HIPCHK(launch_tea_eps(e));
-            int rc = check_tea(e);
-            if (rc) return rc;
-            e->tea_valid = true;
-        }
+
         if (e->noise) e->noise_cursor += seg;
         e->step += seg;
         done += seg;

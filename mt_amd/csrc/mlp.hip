@@ -1,17 +1,15 @@
 // mlp.hip -- the learned hydrodynamic correction (MLP-HI); the reference has
 // no counterpart.
 //
-// Per bead i the engine predicts the local hydrodynamic mobility
-//   A_i = I + sum_{j != i, r_ij < 12 nm} phi(r_ij) D_hat_ij
+// Per bead i the engine predicts the local hydrodynamic tensor
+//   G_i = sum_{j != i, r_ij < 12 nm} phi(r_ij) D_hat_ij
 // (D_hat the RPY block of src/bdhitea_kernel.cu:38-58, phi = 1 up to 8 nm and
 // a cosine taper to 0 at 12 nm) from the relative positions of its 32 nearest
-// neighbours (96 inputs), with a 96-128-128-128-9 tanh network, and moves the
-// bead by
-//   dr_i = (dt/gammaR) A_i F_i + sqrt(2 kB T dt / gammaR) L_i xi_i,  L_i L_i^T = A_i.
-// For a rigid cluster this gives the Kirkwood centre-of-mass mobility
-// (1/n^2) sum_ij D_hat_ij in both drift and noise.  The exact A_i (the same
-// neighbour sum, all neighbours within 12 nm) is the training label and the
-// sampled-error judge; plain TEA is the fallback.
+// neighbours (96 inputs), with a 96-128-128-128-9 tanh network, and replaces
+// TEA's per-step O(N^2) pair sums by it (k_mlp_step), keeping TEA's beta and
+// C_i normalisation.  The exact G_i (the same neighbour sum over every
+// neighbour within 12 nm) is the training label and the sampled-error judge;
+// plain TEA is the fallback.
 //
 // Kernels: neighbour features and exact labels over the hashed cell tables
 // (cs >= 12 nm); inference fused with the integrator, one bead per lane,
@@ -194,40 +192,36 @@ __global__ void __launch_bounds__(MLP_BLOCK)
     for (int q = 0; q < MLP_OUT; q++) out[(size_t)s * MLP_OUT + q] = y[q];
 }
 
-// Symmetrised A = I + (Y + Y^T)/2 and its Cholesky factor; a pivot below
-// 0.05 marks the prediction unusable and falls back to A = I for the bead.
-__device__ __forceinline__ bool mobility_chol(const float y[9], float A[9], float Lc[6]) {
-#pragma unroll
-    for (int x = 0; x < 3; x++)
-#pragma unroll
-        for (int z = 0; z < 3; z++) A[3 * x + z] = 0.5f * (y[3 * x + z] + y[3 * z + x]) + (x == z ? 1.f : 0.f);
-    const float l00s = A[0];
-    bool ok = l00s > 0.05f;
-    const float l00 = sqrtf(fmaxf(l00s, 0.05f));
-    const float l10 = A[3] / l00, l20 = A[6] / l00;
-    const float l11s = A[4] - l10 * l10;
-    ok = ok && l11s > 0.05f;
-    const float l11 = sqrtf(fmaxf(l11s, 0.05f));
-    const float l21 = (A[7] - l20 * l10) / l11;
-    const float l22s = A[8] - l20 * l20 - l21 * l21;
-    ok = ok && l22s > 0.05f;
-    const float l22 = sqrtf(fmaxf(l22s, 0.05f));
-    if (!ok) {
-#pragma unroll
-        for (int q = 0; q < 9; q++) A[q] = (q % 4 == 0) ? 1.f : 0.f;
-        Lc[0] = 1.f; Lc[1] = 0.f; Lc[2] = 1.f; Lc[3] = 0.f; Lc[4] = 0.f; Lc[5] = 1.f;
-        return false;
-    }
-    Lc[0] = l00; Lc[1] = l10; Lc[2] = l11; Lc[3] = l20; Lc[4] = l21; Lc[5] = l22;
+// Cholesky factor of the symmetric 3x3 S (row-major); false when a pivot
+// falls below 1e-3 (not positive definite enough to use).
+__device__ __forceinline__ bool chol3(const float S[9], float Lc[6]) {
+    const float l00s = S[0];
+    if (!(l00s > 1e-3f)) return false;
+    const float l00 = sqrtf(l00s);
+    const float l10 = S[3] / l00, l20 = S[6] / l00;
+    const float l11s = S[4] - l10 * l10;
+    if (!(l11s > 1e-3f)) return false;
+    const float l11 = sqrtf(l11s);
+    const float l21 = (S[7] - l20 * l10) / l11;
+    const float l22s = S[8] - l20 * l20 - l21 * l21;
+    if (!(l22s > 1e-3f)) return false;
+    Lc[0] = l00; Lc[1] = l10; Lc[2] = l11; Lc[3] = l20; Lc[4] = l21; Lc[5] = sqrtf(l22s);
     return true;
 }
 
-// MLP-HI step: prediction fused with the integrator.  Noise as the plain
-// integrator draws it (mobile beads only, xyz then angular stream).
+// MLP-HI step: prediction fused with the integrator.  With G the predicted
+// local HI tensor (symmetrised), TEA's per-trajectory beta and per-bead row
+// normalisation C_i (refreshed by the epsilon pass as in TEA):
+//   drift  (dt/gammaR) (I + G) F_i               local form of F_i + sum_j D_hat_ij F_j
+//   noise  varR L xi,  L L^T = C (I + 2 beta G) C  row sum of TEA's noise covariance
+//                                                  (src/bdhitea_kernel.cu:104-213) to first order in beta
+// so cluster centre-of-mass diffusion follows TEA's.  A prediction whose
+// covariance is not positive definite falls back to C^2 for the bead (counted).
+// Noise draws as the plain integrator (mobile beads only).
 __global__ void __launch_bounds__(MLP_BLOCK)
     k_mlp_step(mte_params P, DevTopo T, DevState Sin, DevState Sout, const float4 *fxyz, const float4 *fang,
-               const float *feat, const float *__restrict__ w, uint4 *seeds, const float *noise, float *yout,
-               int *nbad, LatOffsets L) {
+               const float *feat, const float *__restrict__ w, const float4 *ci, const float *beta, uint4 *seeds,
+               const float *noise, float *yout, int *nbad, LatOffsets L) {
     __shared__ float rows[MLP_BLOCK * ROW];
     const int b = blockIdx.x * MLP_BLOCK + threadIdx.x;
     if (b >= T.nb) return;
@@ -241,8 +235,24 @@ __global__ void __launch_bounds__(MLP_BLOCK)
         mlp_eval(row, w, y, nullptr, 0);
         if (yout)
             for (int q = 0; q < MLP_OUT; q++) yout[(size_t)b * MLP_OUT + q] = y[q];
-        float A[9], Lc[6];
-        if (!mobility_chol(y, A, Lc)) atomicAdd(nbad, 1);
+        float G[9];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) G[3 * r + c] = 0.5f * (y[3 * r + c] + y[3 * c + r]);
+        const float bt = beta[T.tid[b]];
+        const float4 cc = ci[b];
+        const float Cd[3] = {rsq_fast(1.f + bt * bt * cc.x), rsq_fast(1.f + bt * bt * cc.y),
+                             rsq_fast(1.f + bt * bt * cc.z)};
+        float Sg[9], Lc[6];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) Sg[3 * r + c] = Cd[r] * Cd[c] * ((r == c ? 1.f : 0.f) + 2.f * bt * G[3 * r + c]);
+        if (!chol3(Sg, Lc)) {
+            atomicAdd(nbad, 1);
+            Lc[0] = Cd[0]; Lc[1] = 0.f; Lc[2] = Cd[1]; Lc[3] = 0.f; Lc[4] = 0.f; Lc[5] = Cd[2];
+        }
         float4 rx, ra;
         if (noise) {
             const float *nz = noise + 6 * (size_t)b;
@@ -258,13 +268,13 @@ __global__ void __launch_bounds__(MLP_BLOCK)
         }
         const float4 F = fxyz[b];
         const float dtg = P.dt / P.gammaR;
-        const float AF0 = A[0] * F.x + A[1] * F.y + A[2] * F.z;
-        const float AF1 = A[3] * F.x + A[4] * F.y + A[5] * F.z;
-        const float AF2 = A[6] * F.x + A[7] * F.y + A[8] * F.z;
+        const float d0 = F.x + G[0] * F.x + G[1] * F.y + G[2] * F.z;
+        const float d1 = F.y + G[3] * F.x + G[4] * F.y + G[5] * F.z;
+        const float d2 = F.z + G[6] * F.x + G[7] * F.y + G[8] * F.z;
         const float n0 = Lc[0] * rx.x, n1 = Lc[1] * rx.x + Lc[2] * rx.y, n2 = Lc[3] * rx.x + Lc[4] * rx.y + Lc[5] * rx.z;
-        x.x += dtg * AF0 + P.varR * n0;
-        x.y += dtg * AF1 + P.varR * n1;
-        x.z += dtg * AF2 + P.varR * n2;
+        x.x += dtg * d0 + P.varR * n0;
+        x.y += dtg * d1 + P.varR * n1;
+        x.z += dtg * d2 + P.varR * n2;
         const float4 fa = fang[b];
         const float dta = P.dt / (P.gammaTheta * P.alpha), dtt = P.dt / P.gammaTheta;
         const float vta = P.varTheta * sqrtf(P.freeze_temp / P.alpha);
@@ -350,18 +360,23 @@ __global__ void __launch_bounds__(256)
 }
 
 // dW[k][m] = sum_s in[s][k] g[s][m]; row k == K holds db[m] = sum_s g[s][m].
+// blockIdx.y splits the samples into DW_SPLIT slices added atomically into
+// zeroed dW / db.
+constexpr int DW_SPLIT = 32;
 __global__ void __launch_bounds__(256)
     k_mlp_dw(const float *in, const float *g, int n, int K, int M, float *dW, float *db) {
     const int id = blockIdx.x * 256 + threadIdx.x;
     if (id >= (K + 1) * M) return;
     const int k = id / M, m = id % M;
+    const int per = (n + DW_SPLIT - 1) / DW_SPLIT;
+    const int s0 = blockIdx.y * per, s1 = min(n, s0 + per);
     float acc = 0.f;
     if (k < K) {
-        for (int s = 0; s < n; s++) acc = fmaf(in[(size_t)s * K + k], g[(size_t)s * M + m], acc);
-        dW[k * M + m] = acc;
+        for (int s = s0; s < s1; s++) acc = fmaf(in[(size_t)s * K + k], g[(size_t)s * M + m], acc);
+        atomicAdd(&dW[k * M + m], acc);
     } else {
-        for (int s = 0; s < n; s++) acc += g[(size_t)s * M + m];
-        db[m] = acc;
+        for (int s = s0; s < s1; s++) acc += g[(size_t)s * M + m];
+        atomicAdd(&db[m], acc);
     }
 }
 
@@ -400,6 +415,11 @@ __global__ void __launch_bounds__(256)
     wi[i] = round_bf16(nw);
 }
 
+__global__ void __launch_bounds__(256) k_fill_f32(float *p, int n, float v) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 __global__ void __launch_bounds__(256) k_mlp_round(const float *w, float *wi, int n) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) wi[i] = round_bf16(w[i]);
@@ -424,8 +444,8 @@ hipError_t launch_mlp_forward(mte_engine *e, const float *feat, int n, float *ou
 
 hipError_t launch_mlp_step(mte_engine *e, const float *noise_step, float *yout) {
     hipLaunchKernelGGL(k_mlp_step, dim3(blocks(e->nb, MLP_BLOCK)), dim3(MLP_BLOCK), 0, e->stream, e->P, e->T,
-                       e->S[e->cur], e->S[1 - e->cur], e->fxyz, e->fang, e->mlp.feat, e->mlp.wi, e->seeds, noise_step,
-                       yout, e->mlp.nbad, e->L);
+                       e->S[e->cur], e->S[1 - e->cur], e->fxyz, e->fang, e->mlp.feat, e->mlp.wi, e->tea.ci,
+                       e->tea.beta, e->seeds, noise_step, yout, e->mlp.nbad, e->L);
     e->cur = 1 - e->cur;
     return hipGetLastError();
 }
@@ -446,20 +466,21 @@ hipError_t launch_mlp_grad(mte_engine *e, const float *feat, const float *targ, 
     hipLaunchKernelGGL(k_mlp_out_grad, dim3(blocks(n * MLP_OUT, 256)), dim3(256), 0, e->stream, M.y, targ, n, M.g4,
                        M.partial);
     float *gw = M.grad;
-    hipLaunchKernelGGL(k_mlp_dw, dim3(blocks((MLP_H + 1) * MLP_OUT, 256)), dim3(256), 0, e->stream, h3, M.g4, n,
-                       MLP_H, MLP_OUT, gw + OW4, gw + OB4);
+    hipLaunchKernelGGL(k_fill_f32, dim3(blocks(MLP_NPARAM, 256)), dim3(256), 0, e->stream, gw, MLP_NPARAM, 0.f);
+    hipLaunchKernelGGL(k_mlp_dw, dim3(blocks((MLP_H + 1) * MLP_OUT, 256), DW_SPLIT), dim3(256), 0, e->stream, h3,
+                       M.g4, n, MLP_H, MLP_OUT, gw + OW4, gw + OB4);
     hipLaunchKernelGGL(k_mlp_dx, dim3(blocks(n * MLP_H, 256)), dim3(256), 0, e->stream, M.g4, M.w + OW4, h3, n,
                        MLP_H, MLP_OUT, M.ga);
-    hipLaunchKernelGGL(k_mlp_dw, dim3(blocks((MLP_H + 1) * MLP_H, 256)), dim3(256), 0, e->stream, h2, M.ga, n, MLP_H,
-                       MLP_H, gw + OW3, gw + OB3);
+    hipLaunchKernelGGL(k_mlp_dw, dim3(blocks((MLP_H + 1) * MLP_H, 256), DW_SPLIT), dim3(256), 0, e->stream, h2, M.ga,
+                       n, MLP_H, MLP_H, gw + OW3, gw + OB3);
     hipLaunchKernelGGL(k_mlp_dx, dim3(blocks(n * MLP_H, 256)), dim3(256), 0, e->stream, M.ga, M.w + OW3, h2, n,
                        MLP_H, MLP_H, M.gb);
-    hipLaunchKernelGGL(k_mlp_dw, dim3(blocks((MLP_H + 1) * MLP_H, 256)), dim3(256), 0, e->stream, h1, M.gb, n, MLP_H,
-                       MLP_H, gw + OW2, gw + OB2);
+    hipLaunchKernelGGL(k_mlp_dw, dim3(blocks((MLP_H + 1) * MLP_H, 256), DW_SPLIT), dim3(256), 0, e->stream, h1, M.gb,
+                       n, MLP_H, MLP_H, gw + OW2, gw + OB2);
     hipLaunchKernelGGL(k_mlp_dx, dim3(blocks(n * MLP_H, 256)), dim3(256), 0, e->stream, M.gb, M.w + OW2, h1, n,
                        MLP_H, MLP_H, M.ga);
-    hipLaunchKernelGGL(k_mlp_dw, dim3(blocks((MLP_IN + 1) * MLP_H, 256)), dim3(256), 0, e->stream, feat, M.ga, n,
-                       MLP_IN, MLP_H, gw + OW1, gw + OB1);
+    hipLaunchKernelGGL(k_mlp_dw, dim3(blocks((MLP_IN + 1) * MLP_H, 256), DW_SPLIT), dim3(256), 0, e->stream, feat,
+                       M.ga, n, MLP_IN, MLP_H, gw + OW1, gw + OB1);
     return hipGetLastError();
 }
 

@@ -88,28 +88,27 @@ def free_dimers(n: int, box: float, rng: np.random.Generator, min_sep: float = 8
     (random_concentration_w-o_ring.py:39 uses 8 nm), CA below CB along +z.
     Residue numbers start at first_resid so that `fix` (resid <= fix) never
     pins a free dimer."""
+    from scipy.spatial import cKDTree
     cen = np.zeros((0, 3))
-    tries = 0
-    while len(cen) < n:
-        m = max(64, 2 * (n - len(cen)))
-        cand = rng.uniform(0.0, box, size=(m, 3))
-        for p in cand:
-            if len(cen) >= n:
-                break
-            tries += 1
-            if tries > 200 * n + 10000:
-                raise RuntimeError("free_dimers: box too crowded for min_sep")
-            if len(cen):
-                d = cen - p
-                d -= box * np.round(d / box)
-                if np.min(np.einsum("ij,ij->i", d, d)) < min_sep * min_sep:
-                    continue
-            if avoid is not None and len(avoid):
-                d = avoid - p
-                d -= box * np.round(d / box)
-                if np.min(np.einsum("ij,ij->i", d, d)) < min_sep * min_sep:
-                    continue
-            cen = np.vstack([cen, p])
+    for _ in range(50):
+        need = n - len(cen)
+        if need <= 0:
+            break
+        cand = rng.uniform(0.0, box, size=(int(need * 1.3) + 16, 3))
+        pts = np.vstack([cen, cand])
+        tree = cKDTree(pts % box, boxsize=box)
+        drop = np.zeros(len(pts), bool)
+        for i, j in tree.query_pairs(min_sep):
+            # keep the earlier point (accepted ones come first)
+            if not drop[i] and not drop[j]:
+                drop[max(i, j)] = True
+        if avoid is not None and len(avoid):
+            d, _ = cKDTree(avoid % box, boxsize=box).query(pts % box, k=1)
+            drop |= d < min_sep
+        keep = pts[~drop]
+        cen = keep[:n]
+    if len(cen) < n:
+        raise RuntimeError("free_dimers: box too crowded for min_sep")
     nb = 2 * n
     ids = np.arange(first_id, first_id + nb, dtype=np.int32)
     names = ["CA" if k % 2 == 0 else "CB" for k in range(nb)]

@@ -46,6 +46,7 @@ class Batch:
     on_tub: np.ndarray          # (B,) int32
     zs: np.ndarray              # (T,) float32 wall height per trajectory
     box: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))  # (T,) periodic side, 0 = open
+    free_pairs: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))  # first bead of each free dimer
 
     @property
     def nb(self) -> int:

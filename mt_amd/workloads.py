@@ -26,7 +26,8 @@ def plain(ntraj: int, n_dimers: int, seed: int = 20240601, jitter: float = 0.05)
 def uneven(ntraj: int, dimers=(20, 125), free=(2000, 20000), conc=(10.0, 50.0), seed: int = 4242,
            jitter: float = 0.05) -> topology.Batch:
     rng = np.random.default_rng(seed)
-    atoms, coords, boxes = [], [], []
+    atoms, coords, boxes, free_idx = [], [], [], []
+    first = 0
     for _ in range(ntraj):
         nd = int(rng.integers(dimers[0], dimers[1] + 1))
         nf = int(rng.integers(free[0], free[1] + 1))
@@ -42,6 +43,9 @@ def uneven(ntraj: int, dimers=(20, 125), free=(2000, 20000), conc=(10.0, 50.0), 
         atoms.append(a)
         coords.append(lattice.coords7(a, jitter, rng))
         boxes.append(side)
+        free_idx.append(first + mt.n + 2 * np.arange(nf))
+        first += a.n
     b = topology.make_batch(atoms, coords, box=boxes)
     b.coords = lattice.wrap_angles(b.coords)
+    b.free_pairs = np.concatenate(free_idx).astype(np.int64)
     return b

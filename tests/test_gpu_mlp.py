@@ -60,6 +60,9 @@ def test_mlp_step_matches_restatement():
     orc = Oracle(p, b.copy())
     orc.rebuild()
     F = orc.forces()
+    ci, eps = orc.tea_eps()
+    err, _, beta = orc.tea_beta(eps)
+    assert err == 0
     f, _ = eng.mlp_local()
     y = M.forward(M.round_bf16(w), f)
     nz = noise(np.random.default_rng(2), 1, b.nb)
@@ -68,12 +71,16 @@ def test_mlp_step_matches_restatement():
     g = eng.get_state().astype(np.float64)
     c0 = b.coords.astype(np.float64)
     mob = (b.fixed == 0) & (b.extra == 0)
+    tid = np.repeat(np.arange(b.ntraj), b.sizes())
     worst = 0.0
     for i in np.nonzero(mob)[0]:
         Y = y[i].reshape(3, 3)
-        A = np.eye(3) + 0.5 * (Y + Y.T)
-        Lc = np.linalg.cholesky(A)
-        d = p.dt / p.gammaR * A @ F[i, :3] + p.varR * Lc @ nz[0, i, :3]
+        G = 0.5 * (Y + Y.T)
+        bt = beta[tid[i]]
+        C = 1 / np.sqrt(1 + bt * bt * ci[i])
+        S = np.outer(C, C) * (np.eye(3) + 2 * bt * G)
+        Lc = np.linalg.cholesky(S)
+        d = p.dt / p.gammaR * (F[i, :3] + G @ F[i, :3]) + p.varR * Lc @ nz[0, i, :3]
         worst = max(worst, np.abs(g[i, :3] - c0[i, :3] - d).max())
     assert worst < 2e-5
     eng.close()
