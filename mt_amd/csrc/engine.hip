@@ -164,6 +164,11 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     e->ntraj = nt;
     e->device = opt->device;
     if (hipSetDevice(e->device) != hipSuccess) { e->err = "mte_create: hipSetDevice failed"; return bail(MTE_ERR_HIP); }
+    if (hipDeviceGetAttribute(&e->num_cus, hipDeviceAttributeMultiprocessorCount, e->device) != hipSuccess ||
+        e->num_cus < 1) {
+        e->err = "mte_create: cannot query the compute-unit count";
+        return bail(MTE_ERR_HIP);
+    }
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
         e->err = "mte_create: stream creation failed";
         return bail(MTE_ERR_HIP);
@@ -355,7 +360,7 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
             dalloc(e, &M.m2, MLP_NPARAM) != hipSuccess || dalloc(e, &M.feat, (size_t)MLP_IN * nb) != hipSuccess ||
             dalloc(e, &M.targ, (size_t)MLP_OUT * nb) != hipSuccess ||
             dalloc(e, &M.ystep, (size_t)MLP_OUT * nb) != hipSuccess || dalloc(e, &M.err, 2 * (size_t)nt) != hipSuccess ||
-            dalloc(e, &M.nbad, 1) != hipSuccess) {
+            dalloc(e, &M.nbad, 1) != hipSuccess || dalloc(e, &M.img, mlp_image_bytes() / sizeof(uint4)) != hipSuccess) {
             e->err = "mte_create: hipMalloc failed for MLP buffers";
             return bail(MTE_ERR_HIP);
         }
@@ -933,6 +938,7 @@ int mte_mlp_train(mte_engine *e, const float *features, const float *labels, int
             if (losses && !out) losses[it] = mlp_loss(e, batch);
             pos += batch;
         }
+        if (!out && launch_mlp_pack(e) != hipSuccess) out = fail(e, MTE_ERR_HIP, "mte_mlp_train: weight image failed");
         (void)hipStreamSynchronize(e->stream);
     }
     (void)hipFree(X);

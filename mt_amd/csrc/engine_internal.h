@@ -50,6 +50,7 @@ constexpr int MLP_NN = 32, MLP_IN = 96, MLP_H = 128, MLP_OUT = 9;
 constexpr int MLP_NPARAM = MLP_IN * MLP_H + MLP_H + 2 * (MLP_H * MLP_H + MLP_H) + MLP_H * MLP_OUT + MLP_OUT;
 struct MlpBuffers {
     float *w = nullptr, *wi = nullptr;        // FP32 master weights, bf16-rounded copy for inference
+    uint4 *img = nullptr;                     // inference image: bf16 MFMA fragments + biases (mlp_image_bytes())
     float *grad = nullptr, *m1 = nullptr, *m2 = nullptr;
     float *feat = nullptr, *targ = nullptr, *ystep = nullptr;   // per bead
     double *err = nullptr;                    // per trajectory: squared error, squared label norm
@@ -79,6 +80,7 @@ struct mte_engine {
     int nb = 0, ntraj = 0;
     std::vector<int64_t> off;
     int device = 0;
+    int num_cus = 1;
     hipStream_t stream = nullptr;
     int mode = MTE_MODE_STREAMED;
 
@@ -139,6 +141,8 @@ hipError_t launch_mlp_error(mte_engine *e, const float *y, const float *t);
 hipError_t launch_mlp_grad(mte_engine *e, const float *feat, const float *targ, int n);
 hipError_t launch_mlp_adam(mte_engine *e, float lr, int t);
 hipError_t launch_mlp_round(mte_engine *e);
+hipError_t launch_mlp_pack(mte_engine *e);
+size_t mlp_image_bytes();
 hipError_t build_cells(mte_engine *e, float radius);
 float candidate_radius(const mte_params &P);
 hipError_t launch_tea_eps(mte_engine *e);

@@ -12,10 +12,11 @@
 // plain TEA is the fallback.
 //
 // Kernels: neighbour features and exact labels over the hashed cell tables
-// (cs >= 12 nm); inference fused with the integrator, one bead per lane,
-// activations in an LDS row per lane, weights read as wave-uniform operands
-// (bf16-rounded values, FP32 accumulation); training (forward with stored
-// activations, backward, Adam) in FP32.
+// (cs >= 12 nm); inference fused with the integrator on the matrix cores
+// (v_mfma_f32_16x16x32_bf16: bf16 weights, activations carried as bf16
+// hi + lo pairs, FP32 accumulation; the weight fragments sit in LDS and the
+// activations stay in registers from layer to layer); training (forward with
+// stored activations, backward, Adam) in FP32 on the vector ALUs.
 #include "engine_internal.h"
 
 namespace mte {
@@ -39,7 +40,9 @@ __device__ __forceinline__ float taper(float r) {
 // ------------------------------------------------------------ features, labels
 // Neighbour scan over the 27 cells of the hashed table (built with cell side
 // >= 12 nm).  Keeps the 32 nearest (distance, then index) and sums the exact
-// label over every neighbour within 12 nm.
+// label over every neighbour within 12 nm (LABELS: only on check steps and
+// for training data).
+template <bool LABELS>
 __global__ void __launch_bounds__(MLP_BLOCK)
     k_mlp_local(mte_params P, DevTopo T, DevState S, CellTables C, float *feat, float *targ) {
     __shared__ float sd[MLP_BLOCK * MLP_NN];
@@ -79,24 +82,27 @@ __global__ void __launch_bounds__(MLP_BLOCK)
                         const float3 d = min_image(sub3(f3(S.xyz[j]), ri), box);
                         const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
                         if (r2 >= MLP_RC * MLP_RC) continue;
-                        // exact label: phi(r) D_hat(d)
-                        const float r = sqrtf(r2), rinv = 1.f / r;
-                        const float ra = r * ia;
-                        float cii, crr;
-                        if (ra > 2.f) {
-                            const float u = 1.f / ra, u2 = u * u;
-                            cii = 0.75f * u * (1.f + (2.f / 3.f) * u2);
-                            crr = 0.75f * u * (1.f - 2.f * u2);
-                        } else {
-                            cii = 1.f - 9.f * ra / 32.f;
-                            crr = 3.f * ra / 32.f;
+                        if (LABELS) {
+                            // exact label: phi(r) D_hat(d)
+                            const float r = sqrtf(r2), rinv = 1.f / r;
+                            const float ra = r * ia;
+                            float cii, crr;
+                            if (ra > 2.f) {
+                                const float u = 1.f / ra, u2 = u * u;
+                                cii = 0.75f * u * (1.f + (2.f / 3.f) * u2);
+                                crr = 0.75f * u * (1.f - 2.f * u2);
+                            } else {
+                                cii = 1.f - 9.f * ra / 32.f;
+                                crr = 3.f * ra / 32.f;
+                            }
+                            const float w = taper(r);
+                            const float e[3] = {d.x * rinv, d.y * rinv, d.z * rinv};
+#pragma unroll
+                            for (int x = 0; x < 3; x++)
+#pragma unroll
+                                for (int y = 0; y < 3; y++)
+                                    A[3 * x + y] += w * (crr * e[x] * e[y] + (x == y ? cii : 0.f));
                         }
-                        const float w = taper(r);
-                        const float e[3] = {d.x * rinv, d.y * rinv, d.z * rinv};
-#pragma unroll
-                        for (int x = 0; x < 3; x++)
-#pragma unroll
-                            for (int y = 0; y < 3; y++) A[3 * x + y] += w * (crr * e[x] * e[y] + (x == y ? cii : 0.f));
                         // 32 nearest, ordered by (r2, j)
                         const int lj = j;
                         if (cnt < MLP_NN || r2 < nd[cnt - 1] || (r2 == nd[cnt - 1] && lj < nj[cnt - 1])) {
@@ -112,6 +118,7 @@ __global__ void __launch_bounds__(MLP_BLOCK)
                     }
                 }
         (void)a;
+        (void)ia;
     }
     if (feat) {
         float *f = feat + (size_t)b * MLP_IN;
@@ -126,7 +133,7 @@ __global__ void __launch_bounds__(MLP_BLOCK)
             f[3 * k + 2] = d.z;
         }
     }
-    if (targ) {
+    if (LABELS) {
 #pragma unroll
         for (int q = 0; q < 9; q++) targ[(size_t)b * MLP_OUT + q] = A[q];
     }
@@ -180,18 +187,6 @@ __device__ __forceinline__ void mlp_eval(float *row, const float *__restrict__ w
     dense_row<MLP_H, MLP_OUT>(row, w + OW4, w + OB4, false, y);
 }
 
-__global__ void __launch_bounds__(MLP_BLOCK)
-    k_mlp_forward(const float *__restrict__ w, const float *feat, int n, float *out) {
-    __shared__ float rows[MLP_BLOCK * ROW];
-    const int s = blockIdx.x * MLP_BLOCK + threadIdx.x;
-    if (s >= n) return;
-    float *row = rows + threadIdx.x * ROW;
-    for (int k = 0; k < MLP_IN; k++) row[k] = feat[(size_t)s * MLP_IN + k];
-    float y[MLP_OUT];
-    mlp_eval(row, w, y, nullptr, 0);
-    for (int q = 0; q < MLP_OUT; q++) out[(size_t)s * MLP_OUT + q] = y[q];
-}
-
 // Cholesky factor of the symmetric 3x3 S (row-major); false when a pivot
 // falls below 1e-3 (not positive definite enough to use).
 __device__ __forceinline__ bool chol3(const float S[9], float Lc[6]) {
@@ -209,39 +204,240 @@ __device__ __forceinline__ bool chol3(const float S[9], float Lc[6]) {
     return true;
 }
 
-// MLP-HI step: prediction fused with the integrator.  With G the predicted
-// local HI tensor (symmetrised), TEA's per-trajectory beta and per-bead row
-// normalisation C_i (refreshed by the epsilon pass as in TEA):
+// ------------------------------------------------------ inference on MFMA
+// Each layer is computed transposed, D^T = W^T X^T, with
+// v_mfma_f32_16x16x32_bf16: output neurons are the 16 rows (M), 16 beads the
+// columns (N), inputs the reduction (K = 32 per instruction).  Lane l supplies
+// A[l%16][8(l/16)+q] and B[8(l/16)+q][l%16] (q = 0..7) and receives
+// D[4(l/16)+r][l%16] (r = 0..3).  The outputs of two adjacent 16-neuron
+// tiles therefore already sit in B-operand position for the next layer if
+// that layer's K index 8g+q is read as neuron 4g+q (q < 4) or 16+4g+(q-4)
+// (q >= 4) of the 32-neuron block: the weight image is packed in that order
+// and activations never leave registers.  Activations enter as bf16 hi + lo
+// pairs (|x - hi - lo| <= 2^-18 |x|), so with bf16 weights the products are
+// those of the FP32 network with bf16-rounded weights, accumulated in FP32.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int NET_WAVES = 8;                  // waves per workgroup (2 per SIMD)
+constexpr int NET_THREADS = 64 * NET_WAVES;
+constexpr int NET_GROUPS = 2;                 // 16-bead column groups per wave iteration
+constexpr int NET_TILE = 16 * NET_GROUPS;     // beads per wave iteration
+constexpr int MT_H = MLP_H / 16;              // output tiles of a hidden layer
+constexpr int KS_IN = MLP_IN / 32, KS_H = MLP_H / 32;
+// weight image in 16-byte fragments, [layer][m tile][k step][lane]
+constexpr int FR1 = 0, FR2 = FR1 + MT_H * KS_IN * 64, FR3 = FR2 + MT_H * KS_H * 64, FR4 = FR3 + MT_H * KS_H * 64,
+              FR_END = FR4 + KS_H * 64;
+// FP32 biases after the fragments: b1, b2, b3, then b4 padded to 16
+constexpr int BI2 = MLP_H, BI3 = 2 * MLP_H, BI4 = 3 * MLP_H, BI_END = 3 * MLP_H + 16;
+constexpr size_t NET_IMG_BYTES = (size_t)FR_END * 16 + (size_t)BI_END * 4;
+static_assert(MLP_IN % 32 == 0 && MLP_H % 32 == 0 && MLP_OUT <= 16, "network shape vs MFMA tiles");
+static_assert(NET_IMG_BYTES % 16 == 0, "image size");
+
+// Weight image from the bf16-rounded weights (exact bf16 values).
+__global__ void __launch_bounds__(256) k_mlp_pack(const float *__restrict__ wi, u32x4 *img) {
+    const int id = blockIdx.x * 256 + threadIdx.x;
+    if (id < FR_END) {
+        const float *W;
+        int M, ks, mt, local;
+        bool chained = true;
+        if (id < FR2) {
+            local = id - FR1; W = wi + OW1; M = MLP_H; chained = false;
+            mt = (local / 64) / KS_IN; ks = (local / 64) % KS_IN;
+        } else if (id < FR3) {
+            local = id - FR2; W = wi + OW2; M = MLP_H;
+            mt = (local / 64) / KS_H; ks = (local / 64) % KS_H;
+        } else if (id < FR4) {
+            local = id - FR3; W = wi + OW3; M = MLP_H;
+            mt = (local / 64) / KS_H; ks = (local / 64) % KS_H;
+        } else {
+            local = id - FR4; W = wi + OW4; M = MLP_OUT;
+            mt = 0; ks = local / 64;
+        }
+        const int lane = local % 64, g = lane / 16, m = 16 * mt + lane % 16;
+        unsigned int word[4] = {0u, 0u, 0u, 0u};
+        for (int q = 0; q < 8; q++) {
+            const int k = 32 * ks + (chained ? (q < 4 ? 4 * g + q : 16 + 4 * g + (q - 4)) : 8 * g + q);
+            const float v = m < M ? W[k * M + m] : 0.f;
+            word[q / 2] |= (__float_as_uint(v) >> 16) << (16 * (q % 2));
+        }
+        img[id] = u32x4{word[0], word[1], word[2], word[3]};
+    } else if (id < FR_END + BI_END) {
+        const int j = id - FR_END;
+        float v;
+        if (j < BI2) v = wi[OB1 + j];
+        else if (j < BI3) v = wi[OB2 + j - BI2];
+        else if (j < BI4) v = wi[OB3 + j - BI3];
+        else v = j - BI4 < MLP_OUT ? wi[OB4 + j - BI4] : 0.f;
+        reinterpret_cast<float *>(img + FR_END)[j] = v;
+    }
+}
+
+__device__ __forceinline__ float tanh_fast(float x) {
+    // tanh|x| = 1 - 2 / (exp(2|x|) + 1): absolute error a few 1e-7, exact limit 1
+    const float e = __expf(2.f * fabsf(x));
+    return copysignf(1.f - 2.f * rcp_fast(e + 1.f), x);
+}
+
+__device__ __forceinline__ void split8(const float v[8], bf16x8 &hi, bf16x8 &lo) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const __bf16 h = (__bf16)v[q];
+        hi[q] = h;
+        lo[q] = (__bf16)(v[q] - (float)h);
+    }
+}
+
+// One layer over the bead groups: acc[mt][grp] = bias + W^T (hi + lo).
+template <int KS, int MT>
+__device__ __forceinline__ void net_layer(const u32x4 *frag, const float *bias, const bf16x8 (&hi)[KS][NET_GROUPS],
+                                          const bf16x8 (&lo)[KS][NET_GROUPS], f32x4 (&acc)[MT][NET_GROUPS], int lane) {
+    const int g = lane >> 4;
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) {
+        const f32x4 b = *reinterpret_cast<const f32x4 *>(bias + 16 * mt + 4 * g);
+#pragma unroll
+        for (int grp = 0; grp < NET_GROUPS; grp++) acc[mt][grp] = b;
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const bf16x8 a = __builtin_bit_cast(bf16x8, frag[(mt * KS + ks) * 64 + lane]);
+#pragma unroll
+            for (int grp = 0; grp < NET_GROUPS; grp++) {
+                acc[mt][grp] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, hi[ks][grp], acc[mt][grp], 0, 0, 0);
+                acc[mt][grp] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, lo[ks][grp], acc[mt][grp], 0, 0, 0);
+            }
+        }
+    }
+}
+
+// tanh of a hidden layer, repacked as the next layer's B fragments.
+template <int KS>
+__device__ __forceinline__ void hidden_to_frags(f32x4 (&acc)[2 * KS][NET_GROUPS], bf16x8 (&hi)[KS][NET_GROUPS],
+                                                bf16x8 (&lo)[KS][NET_GROUPS]) {
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+        for (int grp = 0; grp < NET_GROUPS; grp++) {
+            float v[8];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                v[r] = tanh_fast(acc[2 * ks][grp][r]);
+                v[4 + r] = tanh_fast(acc[2 * ks + 1][grp][r]);
+            }
+            split8(v, hi[ks][grp], lo[ks][grp]);
+        }
+}
+
+// The network on beads b0 .. b0+NET_TILE-1 (features of beads >= n read as
+// zero); returns in lane t < NET_TILE the 9 outputs of bead b0 + t.
+__device__ __forceinline__ void net_tile(const u32x4 *frag, const float *bias, const float *__restrict__ feat, int n,
+                                         int b0, int lane, float y[MLP_OUT]) {
+    const int g = lane >> 4, col = lane & 15;
+    // The fragments are the same for every tile: keep the compiler from
+    // hoisting all of them (368 registers) out of the caller's tile loop.
+    __asm__ volatile("" ::: "memory");
+    bf16x8 hi1[KS_IN][NET_GROUPS], lo1[KS_IN][NET_GROUPS];
+#pragma unroll
+    for (int grp = 0; grp < NET_GROUPS; grp++) {
+        const int bead = b0 + 16 * grp + col;
+#pragma unroll
+        for (int ks = 0; ks < KS_IN; ks++) {
+            float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            if (bead < n) {
+                const float4 *src = reinterpret_cast<const float4 *>(feat + (size_t)bead * MLP_IN + 32 * ks + 8 * g);
+                const float4 u = src[0], w = src[1];
+                v[0] = u.x; v[1] = u.y; v[2] = u.z; v[3] = u.w;
+                v[4] = w.x; v[5] = w.y; v[6] = w.z; v[7] = w.w;
+            }
+            split8(v, hi1[ks][grp], lo1[ks][grp]);
+        }
+    }
+    f32x4 acc[MT_H][NET_GROUPS];
+    net_layer<KS_IN, MT_H>(frag + FR1, bias, hi1, lo1, acc, lane);
+    bf16x8 hi[KS_H][NET_GROUPS], lo[KS_H][NET_GROUPS];
+    hidden_to_frags<KS_H>(acc, hi, lo);
+    net_layer<KS_H, MT_H>(frag + FR2, bias + BI2, hi, lo, acc, lane);
+    hidden_to_frags<KS_H>(acc, hi, lo);
+    net_layer<KS_H, MT_H>(frag + FR3, bias + BI3, hi, lo, acc, lane);
+    hidden_to_frags<KS_H>(acc, hi, lo);
+    f32x4 out[1][NET_GROUPS];
+    net_layer<KS_H, 1>(frag + FR4, bias + BI4, hi, lo, out, lane);
+    // output q of bead 16 grp + c sits in lane 16 (q / 4) + c, element q % 4
+    const int grp = (lane >> 4) % NET_GROUPS;
+#pragma unroll
+    for (int q = 0; q < MLP_OUT; q++) {
+        const int src = 16 * (q >> 2) + col;
+        float v = 0.f;
+#pragma unroll
+        for (int gg = 0; gg < NET_GROUPS; gg++) {
+            const float vg = __shfl(out[0][gg][q & 3], src);
+            v = gg == grp ? vg : v;
+        }
+        y[q] = v;
+    }
+}
+
+// Weight image into LDS (dynamic, NET_IMG_BYTES).
+__device__ __forceinline__ void load_image(const u32x4 *__restrict__ img, u32x4 *lds) {
+    for (int i = threadIdx.x; i < (int)(NET_IMG_BYTES / 16); i += NET_THREADS) lds[i] = img[i];
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(NET_THREADS)
+    k_mlp_forward(const u32x4 *__restrict__ img, const float *__restrict__ feat, int n, float *out) {
+    extern __shared__ u32x4 net_lds[];
+    load_image(img, net_lds);
+    const float *bias = reinterpret_cast<const float *>(net_lds + FR_END);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ntiles = (n + NET_TILE - 1) / NET_TILE;
+    for (int tile = blockIdx.x * NET_WAVES + wave; tile < ntiles; tile += gridDim.x * NET_WAVES) {
+        float y[MLP_OUT];
+        net_tile(net_lds, bias, feat, n, tile * NET_TILE, lane, y);
+        const int s = tile * NET_TILE + lane;
+        if (lane < NET_TILE && s < n)
+            for (int q = 0; q < MLP_OUT; q++) out[(size_t)s * MLP_OUT + q] = y[q];
+    }
+}
+
+struct MlpStepArgs {
+    mte_params P;
+    DevTopo T;
+    DevState Sin, Sout;
+    const float4 *fxyz, *fang, *ci;
+    const float *beta;
+    uint4 *seeds;
+    const float *noise;
+    float *yout;
+    int *nbad;
+    LatOffsets L;
+};
+
+// MLP-HI integration of bead b from the network outputs y.  With G the
+// predicted local HI tensor (symmetrised), TEA's per-trajectory beta and
+// per-bead row normalisation C_i (refreshed by the epsilon pass as in TEA):
 //   drift  (dt/gammaR) (I + G) F_i               local form of F_i + sum_j D_hat_ij F_j
 //   noise  varR L xi,  L L^T = C (I + 2 beta G) C  row sum of TEA's noise covariance
 //                                                  (src/bdhitea_kernel.cu:104-213) to first order in beta
 // so cluster centre-of-mass diffusion follows TEA's.  A prediction whose
 // covariance is not positive definite falls back to C^2 for the bead (counted).
 // Noise draws as the plain integrator (mobile beads only).
-__global__ void __launch_bounds__(MLP_BLOCK)
-    k_mlp_step(mte_params P, DevTopo T, DevState Sin, DevState Sout, const float4 *fxyz, const float4 *fang,
-               const float *feat, const float *__restrict__ w, const float4 *ci, const float *beta, uint4 *seeds,
-               const float *noise, float *yout, int *nbad, LatOffsets L) {
-    __shared__ float rows[MLP_BLOCK * ROW];
-    const int b = blockIdx.x * MLP_BLOCK + threadIdx.x;
-    if (b >= T.nb) return;
-    float4 x = Sin.xyz[b], an = Sin.ang[b];
-    float4 n4 = Sin.fn[b], p1 = Sin.fp1[b], p2 = Sin.fp2[b];
+__device__ __forceinline__ void mlp_integrate(const MlpStepArgs &A, int b, const float y[MLP_OUT]) {
+    const mte_params &P = A.P;
+    const DevTopo &T = A.T;
+    float4 x = A.Sin.xyz[b], an = A.Sin.ang[b];
+    float4 n4 = A.Sin.fn[b], p1 = A.Sin.fp1[b], p2 = A.Sin.fp2[b];
     const bool mobile = !T.fixed[b] && !T.extra[b];
     if (mobile) {
-        float *row = rows + threadIdx.x * ROW;
-        for (int k = 0; k < MLP_IN; k++) row[k] = feat[(size_t)b * MLP_IN + k];
-        float y[MLP_OUT];
-        mlp_eval(row, w, y, nullptr, 0);
-        if (yout)
-            for (int q = 0; q < MLP_OUT; q++) yout[(size_t)b * MLP_OUT + q] = y[q];
+        if (A.yout)
+            for (int q = 0; q < MLP_OUT; q++) A.yout[(size_t)b * MLP_OUT + q] = y[q];
         float G[9];
 #pragma unroll
         for (int r = 0; r < 3; r++)
 #pragma unroll
             for (int c = 0; c < 3; c++) G[3 * r + c] = 0.5f * (y[3 * r + c] + y[3 * c + r]);
-        const float bt = beta[T.tid[b]];
-        const float4 cc = ci[b];
+        const float bt = A.beta[T.tid[b]];
+        const float4 cc = A.ci[b];
         const float Cd[3] = {rsq_fast(1.f + bt * bt * cc.x), rsq_fast(1.f + bt * bt * cc.y),
                              rsq_fast(1.f + bt * bt * cc.z)};
         float Sg[9], Lc[6];
@@ -250,23 +446,23 @@ __global__ void __launch_bounds__(MLP_BLOCK)
 #pragma unroll
             for (int c = 0; c < 3; c++) Sg[3 * r + c] = Cd[r] * Cd[c] * ((r == c ? 1.f : 0.f) + 2.f * bt * G[3 * r + c]);
         if (!chol3(Sg, Lc)) {
-            atomicAdd(nbad, 1);
+            atomicAdd(A.nbad, 1);
             Lc[0] = Cd[0]; Lc[1] = 0.f; Lc[2] = Cd[1]; Lc[3] = 0.f; Lc[4] = 0.f; Lc[5] = Cd[2];
         }
         float4 rx, ra;
-        if (noise) {
-            const float *nz = noise + 6 * (size_t)b;
+        if (A.noise) {
+            const float *nz = A.noise + 6 * (size_t)b;
             rx = make_float4(nz[0], nz[1], nz[2], 0.f);
             ra = make_float4(nz[3], nz[4], nz[5], 0.f);
         } else {
-            uint4 s = seeds[b];
+            uint4 s = A.seeds[b];
             rx = normals4(s);
-            seeds[b] = s;
-            s = seeds[b + T.nb];
+            A.seeds[b] = s;
+            s = A.seeds[b + T.nb];
             ra = normals4(s);
-            seeds[b + T.nb] = s;
+            A.seeds[b + T.nb] = s;
         }
-        const float4 F = fxyz[b];
+        const float4 F = A.fxyz[b];
         const float dtg = P.dt / P.gammaR;
         const float d0 = F.x + G[0] * F.x + G[1] * F.y + G[2] * F.z;
         const float d1 = F.y + G[3] * F.x + G[4] * F.y + G[5] * F.z;
@@ -275,21 +471,39 @@ __global__ void __launch_bounds__(MLP_BLOCK)
         x.x += dtg * d0 + P.varR * n0;
         x.y += dtg * d1 + P.varR * n1;
         x.z += dtg * d2 + P.varR * n2;
-        const float4 fa = fang[b];
+        const float4 fa = A.fang[b];
         const float dta = P.dt / (P.gammaTheta * P.alpha), dtt = P.dt / P.gammaTheta;
         const float vta = P.varTheta * sqrtf(P.freeze_temp / P.alpha);
         an.x += dta * fa.x + vta * ra.x;
         an.z += dta * fa.z + vta * ra.y;
         an.y += dtt * fa.y + P.varTheta * ra.z;
-        frame_vectors(an.x, an.y, an.z, L, n4, p1, p2);
-    } else if (yout) {
-        for (int q = 0; q < MLP_OUT; q++) yout[(size_t)b * MLP_OUT + q] = 0.f;
+        frame_vectors(an.x, an.y, an.z, A.L, n4, p1, p2);
+    } else if (A.yout) {
+        for (int q = 0; q < MLP_OUT; q++) A.yout[(size_t)b * MLP_OUT + q] = 0.f;
     }
-    Sout.xyz[b] = x;
-    Sout.ang[b] = an;
-    Sout.fn[b] = n4;
-    Sout.fp1[b] = p1;
-    Sout.fp2[b] = p2;
+    A.Sout.xyz[b] = x;
+    A.Sout.ang[b] = an;
+    A.Sout.fn[b] = n4;
+    A.Sout.fp1[b] = p1;
+    A.Sout.fp2[b] = p2;
+}
+
+// MLP-HI step: network on the matrix cores, then lanes 0..NET_TILE-1
+// integrate the tile's beads.
+__global__ void __launch_bounds__(NET_THREADS)
+    k_mlp_step(const u32x4 *__restrict__ img, const float *__restrict__ feat, MlpStepArgs A) {
+    extern __shared__ u32x4 net_lds[];
+    load_image(img, net_lds);
+    const float *bias = reinterpret_cast<const float *>(net_lds + FR_END);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = A.T.nb;
+    const int ntiles = (n + NET_TILE - 1) / NET_TILE;
+    for (int tile = blockIdx.x * NET_WAVES + wave; tile < ntiles; tile += gridDim.x * NET_WAVES) {
+        float y[MLP_OUT];
+        net_tile(net_lds, bias, feat, n, tile * NET_TILE, lane, y);
+        const int b = tile * NET_TILE + lane;
+        if (lane < NET_TILE && b < n) mlp_integrate(A, b, y);
+    }
 }
 
 // Per-trajectory squared error and squared label norm of predictions y
@@ -431,21 +645,41 @@ static inline int blocks(int n, int bs) { return (n + bs - 1) / bs; }
 hipError_t launch_mlp_local(mte_engine *e, float *feat, float *targ) {
     hipError_t s = build_cells(e, MLP_RC);
     if (s != hipSuccess) return s;
-    hipLaunchKernelGGL(k_mlp_local, dim3(blocks(e->nb, MLP_BLOCK)), dim3(MLP_BLOCK), 0, e->stream, e->P, e->T,
-                       e->S[e->cur], e->cells, feat, targ);
+    if (targ)
+        hipLaunchKernelGGL(k_mlp_local<true>, dim3(blocks(e->nb, MLP_BLOCK)), dim3(MLP_BLOCK), 0, e->stream, e->P,
+                           e->T, e->S[e->cur], e->cells, feat, targ);
+    else
+        hipLaunchKernelGGL(k_mlp_local<false>, dim3(blocks(e->nb, MLP_BLOCK)), dim3(MLP_BLOCK), 0, e->stream, e->P,
+                           e->T, e->S[e->cur], e->cells, feat, targ);
     return hipGetLastError();
 }
 
+// Workgroups of the inference kernels: one per CU at most (the weight image
+// takes most of a CU's LDS), each wave looping over bead tiles.
+static int net_blocks(const mte_engine *e, int n) {
+    const int need = blocks(blocks(n, NET_TILE), NET_WAVES);
+    return std::max(1, std::min(need, e->num_cus));
+}
+
+static hipError_t net_lds_attr(const void *k) {
+    return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)NET_IMG_BYTES);
+}
+
 hipError_t launch_mlp_forward(mte_engine *e, const float *feat, int n, float *out) {
-    hipLaunchKernelGGL(k_mlp_forward, dim3(blocks(n, MLP_BLOCK)), dim3(MLP_BLOCK), 0, e->stream, e->mlp.wi, feat, n,
-                       out);
+    hipError_t s = net_lds_attr((const void *)k_mlp_forward);
+    if (s != hipSuccess) return s;
+    hipLaunchKernelGGL(k_mlp_forward, dim3(net_blocks(e, n)), dim3(NET_THREADS), NET_IMG_BYTES, e->stream,
+                       reinterpret_cast<const u32x4 *>(e->mlp.img), feat, n, out);
     return hipGetLastError();
 }
 
 hipError_t launch_mlp_step(mte_engine *e, const float *noise_step, float *yout) {
-    hipLaunchKernelGGL(k_mlp_step, dim3(blocks(e->nb, MLP_BLOCK)), dim3(MLP_BLOCK), 0, e->stream, e->P, e->T,
-                       e->S[e->cur], e->S[1 - e->cur], e->fxyz, e->fang, e->mlp.feat, e->mlp.wi, e->tea.ci,
-                       e->tea.beta, e->seeds, noise_step, yout, e->mlp.nbad, e->L);
+    hipError_t s = net_lds_attr((const void *)k_mlp_step);
+    if (s != hipSuccess) return s;
+    MlpStepArgs A{e->P, e->T, e->S[e->cur], e->S[1 - e->cur], e->fxyz, e->fang, e->tea.ci, e->tea.beta,
+                  e->seeds, noise_step, yout, e->mlp.nbad, e->L};
+    hipLaunchKernelGGL(k_mlp_step, dim3(net_blocks(e, e->nb)), dim3(NET_THREADS), NET_IMG_BYTES, e->stream,
+                       reinterpret_cast<const u32x4 *>(e->mlp.img), e->mlp.feat, A);
     e->cur = 1 - e->cur;
     return hipGetLastError();
 }
@@ -495,7 +729,15 @@ hipError_t launch_mlp_adam(mte_engine *e, float lr, int t) {
 hipError_t launch_mlp_round(mte_engine *e) {
     hipLaunchKernelGGL(k_mlp_round, dim3(blocks(MLP_NPARAM, 256)), dim3(256), 0, e->stream, e->mlp.w, e->mlp.wi,
                        MLP_NPARAM);
+    return launch_mlp_pack(e);
+}
+
+hipError_t launch_mlp_pack(mte_engine *e) {
+    hipLaunchKernelGGL(k_mlp_pack, dim3(blocks(FR_END + BI_END, 256)), dim3(256), 0, e->stream, e->mlp.wi,
+                       reinterpret_cast<u32x4 *>(e->mlp.img));
     return hipGetLastError();
 }
+
+size_t mlp_image_bytes() { return NET_IMG_BYTES; }
 
 }  // namespace mte
