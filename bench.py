@@ -1,4 +1,5 @@
-"""Throughput of the hot path on one MI355X: batched plain Langevin dynamics.
+"""Throughput of the hot path on one MI355X: batched plain Langevin dynamics,
+with TEA-HI and MLP-HI sections.
 
 Workload (batched plain Langevin, FP32, no HI): 256 independent trajectories
 of a 13-protofilament x 40-dimer microtubule (1040 beads each, 266,240 beads
@@ -6,6 +7,11 @@ in total), per-trajectory HybridTaus streams, bottom ring fixed, LJ pair list
 cutoff 8 nm rebuilt every 100 steps, synthetic lattice jittered N(0, 0.05 nm).
 One step = forces + Euler-Maruyama for every bead of the batch (and the list
 rebuild every 100th step).  value = monomer-steps/s (beads x steps / s).
+
+The "tea_hi" object times batched TEA hydrodynamics (64 x 1040 beads) and
+its drift error against the FP64 Ermak-McCammon oracle; the "mlp_hi" object
+trains the learned HI correction on the device from the engine's own TEA runs
+and deploys it on 32 x 43,250-bead systems beside plain TEA.
 
 Prints one JSON line.  Also reports the dominant kernel's HBM roofline
 fraction (algorithmic bytes per monomer-step x monomer-steps per launch /
@@ -156,6 +162,88 @@ def tea_bench(steps, warmup, cpu=True):
     return out
 
 
+MLP_FLOPS_PER_BEAD = 2 * (96 * 128 + 2 * 128 * 128 + 128 * 9)   # 96-128-128-128-9 network, one bead
+BF16_DENSE_PEAK_TFS = 2500.0
+
+
+def mlp_bench(steps, warmup, iters):
+    """MLP-HI: training data from the engine's own TEA runs, device training,
+    then deployment on 32 trajectories of a 125-dimer microtubule (3250 beads)
+    with 20,000 free dimers (40,000 beads) at 30 uM in a periodic box, beside
+    plain TEA and free-draining (no HI) runs of the same length from the same
+    seeds.  Reports throughput, the free-dimer centre-of-mass diffusion
+    coefficient of each, the sampled error the judge saw, and the MFMA
+    roofline of the fused inference + integrator kernel."""
+    from mt_amd import params, workloads, mlp_hi
+    from mt_amd.engine import Engine, HI_MLP, HI_DEFAULT, MODE_STREAMED
+    pt = params.derive(dict(params.TEA))
+    t0 = time.perf_counter()
+    sets = [(workloads.plain(64, 40, seed=11), 5, 20),
+            (workloads.uneven(8, dimers=(40, 60), free=(3000, 5000), conc=(20, 50), seed=3), 6, 20),
+            (workloads.uneven(8, dimers=(100, 125), free=(15000, 20000), conc=(20, 40), seed=17), 6, 25)]
+    Xs, Ys = [], []
+    for bt, frames, every in sets:
+        x, y, _ = mlp_hi.training_data(pt, bt, frames=frames, every=every)
+        Xs.append(x)
+        Ys.append(y)
+    X, Y = np.concatenate(Xs), np.concatenate(Ys)
+    data_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    w, losses = mlp_hi.train(pt, sets[1][0], X, Y, iters=iters, bsize=8192, lr=2e-3)
+    train_s = time.perf_counter() - t0
+    print(f"[bench] mlp training: {len(X)} samples, {iters} iterations, {train_s:.1f} s, "
+          f"loss {losses[:5].mean():.4g} -> {losses[-20:].mean():.4g}", file=sys.stderr, flush=True)
+    b = workloads.uneven(32, dimers=(125, 125), free=(20000, 20000), conc=(30, 30), seed=5)
+    res = {}
+    for mode in ("mlp", "none", "tea"):
+        p = pt if mode != "none" else params.derive({**mlp_hi._cfg_of(pt), "tea_on": False})
+        eng = Engine(p, b, max_lj=48, mode=MODE_STREAMED, rseed=1234567)
+        if mode == "mlp":
+            eng.mlp_set_weights(w)
+            eng.set_hi_mode(HI_MLP, check_every=100, tolerance=0.02)
+        else:
+            eng.set_hi_mode(HI_DEFAULT)
+        eng.run(warmup)
+        s0 = eng.get_state()
+        ta = time.perf_counter()
+        eng.run(steps)
+        el = time.perf_counter() - ta
+        s1 = eng.get_state()
+        D, De = mlp_hi.com_diffusion(b, s0, s1, steps * pt.dt)
+        r = {"value": b.nb * steps / el, "ms_per_step": el * 1e3 / steps, "D_nm2_per_ps": D, "D_stderr": De}
+        if mode == "mlp":
+            err, fb, bad = eng.mlp_status()
+            r.update(sampled_error_max=float(np.max(err)), fallback=bool(fb), bad_beads=int(bad))
+            eng.reset_timing()
+            eng.set_timing(True)
+            eng.run(50)
+            k_ms, k_n = eng.kernel_time("mlp_step")
+            eng.set_timing(False)
+            avg = k_ms / 1e3 / max(k_n, 1)
+            flops = float(MLP_FLOPS_PER_BEAD) * b.nb
+            r["roofline"] = {"bound": "mfma", "achieved": flops / avg / 1e12, "peak": BF16_DENSE_PEAK_TFS,
+                             "unit": "TFLOP/s", "frac": flops / avg / 1e12 / BF16_DENSE_PEAK_TFS, "traffic": None,
+                             "kernel": "k_mlp_step", "avg_launch_ms": avg * 1e3, "alg_flops_per_launch": flops}
+        eng.close()
+        res[mode] = r
+        print(f"[bench] mlp section {mode}: {r['value']:.4g} monomer-steps/s, D {D:.5g}", file=sys.stderr,
+              flush=True)
+    tea = res["tea"]
+    return {"value": res["mlp"]["value"], "unit": "monomer-steps/s", "steps": steps, "warmup": warmup,
+            "ms_per_step": res["mlp"]["ms_per_step"],
+            "config": {"workload": "MLP-HI: 32 trajectories x (125-dimer microtubule, 3250 beads + 20,000 free "
+                                   "dimers, 40,000 beads) at 30 uM, periodic box, a 2 nm, 0.89 cP, dt 1 ps; "
+                                   "judge every 100 steps at 2%", "beads": b.nb},
+            "speedup_vs_gpu_tea": res["mlp"]["value"] / tea["value"],
+            "diffusion_error_vs_tea": res["mlp"]["D_nm2_per_ps"] / tea["D_nm2_per_ps"] - 1.0,
+            "diffusion_error_no_hi_vs_tea": res["none"]["D_nm2_per_ps"] / tea["D_nm2_per_ps"] - 1.0,
+            "runs": res,
+            "training": {"samples": int(len(X)), "iterations": iters, "batch": 8192, "data_s": data_s,
+                         "train_s": train_s, "loss_start": float(losses[:5].mean()),
+                         "loss_end": float(losses[-20:].mean())},
+            "roofline": res["mlp"]["roofline"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +253,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tea", action="store_true")
     ap.add_argument("--tea-steps", type=int, default=2000)
+    ap.add_argument("--no-mlp", action="store_true")
+    ap.add_argument("--mlp-steps", type=int, default=300)
+    ap.add_argument("--mlp-iters", type=int, default=20000)
     ap.add_argument("--cpu-traj", type=int, default=8)
     ap.add_argument("--cpu-steps", type=int, default=1000)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -235,6 +326,8 @@ def main():
     eng.close()
     if not args.no_tea:
         out["tea_hi"] = tea_bench(args.tea_steps, 20, cpu=not args.no_cpu)
+    if not args.no_mlp:
+        out["mlp_hi"] = mlp_bench(args.mlp_steps, 20, args.mlp_iters)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_traj, args.cpu_steps)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

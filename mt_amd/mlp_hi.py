@@ -39,7 +39,9 @@ def training_data(params, batch, frames: int, every: int, max_lj: int = 48):
 
 
 def train(params, batch, X, Y, iters: int, bsize: int = 4096, lr: float = 1e-3, seed: int = 0):
-    """Adam on the device; returns trained weights and the loss curve."""
+    """Adam on the device (learning rate lr, then lr/3 after half the
+    iterations and lr/10 for the last fifth); returns trained weights and the
+    loss curve."""
     rng = np.random.default_rng(seed)
     perm = rng.permutation(len(X))
     X, Y = X[perm], Y[perm]
@@ -49,7 +51,8 @@ def train(params, batch, X, Y, iters: int, bsize: int = 4096, lr: float = 1e-3, 
     done = 0
     while done < iters:
         k = min(500, iters - done)
-        losses.append(eng.mlp_train(X, Y, iters=k, batch=bsize, lr=lr * (0.3 if done > 0.6 * iters else 1.0)))
+        scale = 1.0 if done < 0.5 * iters else (0.3 if done < 0.8 * iters else 0.1)
+        losses.append(eng.mlp_train(X, Y, iters=k, batch=bsize, lr=lr * scale))
         done += k
     w = eng.mlp_get_weights()
     eng.close()
