@@ -170,10 +170,16 @@ int mte_get_mode(const mte_engine *e, int32_t *mode);
 
 /* ------------------------------------------------------------------ MLP-HI
  * The learned hydrodynamic correction has no counterpart in the reference.
- * Per bead it predicts the local mobility A = I + sum_j phi(r_ij) D_hat_ij
- * (neighbours within 12 nm, cosine taper from 8 nm) from the relative
- * positions of the 32 nearest neighbours with a 96-128-128-128-9 tanh MLP,
- * and integrates dr = (dt/gammaR) A F + sqrt(2 kB T dt/gammaR) chol(A) xi.
+ * Per bead it predicts the local HI tensor G_i = sum_j phi(r_ij) D_hat_ij
+ * (RPY blocks of src/bdhitea_kernel.cu:38-58 over neighbours within 12 nm,
+ * cosine taper from 8 nm) from the relative positions of the 32 nearest
+ * neighbours with a 96-128-128-128-9 tanh MLP (bf16 weights, FP32
+ * accumulation, on the matrix cores), and replaces TEA's per-step pair sums:
+ *   dr = (dt/gammaR) (I + G) F + varR L xi,   L L^T = C (I + 2 beta G) C
+ * with TEA's beta and per-bead C_i (src/bdhitea_kernel.cu:104-213),
+ * refreshed every tea_epsilon_freq steps.  Every check_every steps the exact
+ * G_i is computed for all beads; a per-trajectory relative error above the
+ * tolerance hands the run back to TEA.
  * Weights: MTE_MLP_NPARAM floats, W1 (96x128, input-major), b1, W2, b2, W3,
  * b3, W4 (128x9), b4.  Requires tea_on parameters (tea_a, gammaR). */
 #define MTE_MLP_NPARAM 46601

@@ -116,12 +116,21 @@ class Engine:
         self.max_lj = max_lj
         keep = {}
 
+        nb, nt = batch.nb, batch.ntraj
+        sizes = {"off": nt + 1, "harm_count": nb, "harm": nb, "lon_count": nb, "lon": nb * MAXLON, "lat_count": nb,
+                 "lat": nb * MAXLAT, "fixed": nb, "extra": nb, "mon_type": nb, "gtp": nb, "on_tub": nb, "zs": nt,
+                 "box": nt}
+
         def arr(name, dt):
             a = np.ascontiguousarray(getattr(batch, name), dt)
+            if name == "box" and a.size == 0:
+                return None   # open boundaries
+            if a.size != sizes[name]:
+                raise ValueError(f"batch.{name} has {a.size} elements, the batch needs {sizes[name]}")
             keep[name] = a
             return _p(a)
 
-        topo = Topology(batch.ntraj, arr("off", np.int64), arr("harm_count", np.int32), arr("harm", np.int32),
+        topo = Topology(nt, arr("off", np.int64), arr("harm_count", np.int32), arr("harm", np.int32),
                         arr("lon_count", np.int32), arr("lon", np.int32), arr("lat_count", np.int32),
                         arr("lat", np.int32), arr("fixed", np.uint8), arr("extra", np.uint8),
                         arr("mon_type", np.int32), arr("gtp", np.int32), arr("on_tub", np.int32),

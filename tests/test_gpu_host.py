@@ -1,0 +1,92 @@
+"""The C++ host driver on the GPU: a run of the fixture configuration
+(tests/golden/host/run: 2 trajectories of a 13x3-dimer tube with two free
+dimers, 1000 steps, output every 100) through mth_run, checked against the
+engine driven directly over the C ABI with the same parameters, topology and
+seed: every DCD frame equals the engine state at that step, and the printed
+per-trajectory energies equal mte_compute_energies there."""
+import os
+import re
+import shutil
+import struct
+
+import numpy as np
+import pytest
+
+from mt_amd import host
+from mt_amd.engine import Engine, MODE_STREAMED
+from mt_amd.topology import Batch
+
+pytestmark = pytest.mark.gpu
+
+RUN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "host", "run")
+
+
+def read_dcd(path):
+    raw = open(path, "rb").read()
+    n = struct.unpack_from("<i", raw, 268)[0]
+    off, frames = 276, []
+    while off < len(raw):
+        xyz = []
+        for _ in range(3):
+            size = struct.unpack_from("<i", raw, off)[0]
+            assert size == 4 * n
+            xyz.append(np.frombuffer(raw, np.float32, n, off + 4))
+            off += 8 + 4 * n
+        frames.append(np.stack(xyz, 1))
+    hdr = struct.unpack_from("<i4si i i i", raw, 0)
+    return hdr, frames
+
+
+def batch_from_setup(s):
+    n, nt = s.ntot, s.ntr
+    nb = n * nt
+    wrapped = s.coords.copy()
+    two_pi = 2 * np.pi
+    for k in (3, 4, 5):
+        v = wrapped[:, k].astype(np.float64)
+        wrapped[:, k] = (v - two_pi * np.trunc(v / two_pi)).astype(np.float32)
+    return Batch(coords=wrapped, off=np.arange(nt + 1, dtype=np.int64) * n,
+                 harm_count=np.tile(s.harmonic_count, nt), harm=np.tile(s.harmonic, nt),
+                 lon_count=s.lon_count.copy(), lon=s.lon.copy(), lat_count=s.lat_count.copy(), lat=s.lat.copy(),
+                 fixed=np.tile(s.fixed, nt), extra=s.extra.copy(), mon_type=np.tile(s.mon_type, nt),
+                 gtp=np.ones(nb, np.int32), on_tub=np.zeros(nb, np.int32), zs=np.full(nt, 160.0, np.float32))
+
+
+def test_driver_run_matches_engine(tmp_path, capfd):
+    print("case host driver: fixture run, streamed mode, 1000 steps", flush=True)
+    for f in os.listdir(RUN):
+        shutil.copy(os.path.join(RUN, f), tmp_path / f)
+    old = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        s = host.Setup("main.conf")
+        capfd.readouterr()
+        host.run("main.conf", mode=MODE_STREAMED, max_lj=256, quiet=True)
+        out = capfd.readouterr().out
+        (_, cord, nfile, npriv, nsavc, _), frames = read_dcd("traj_0.dcd")
+        _, frames1 = read_dcd("traj_1.dcd")
+        _, angs = read_dcd("traj_0.dcd_ang")
+        assert cord == b"CORD" and (nfile, npriv, nsavc) == (10, 1, 100)
+        assert len(frames) == len(frames1) == len(angs) == 10
+        assert os.path.exists("result_xyz.pdb") and os.path.exists("result_ang.pdb")
+        energies = [list(map(float, l.split("\t")[1:])) for l in out.splitlines() if l.startswith("Energies[")]
+        assert len(energies) == 20
+        eng = Engine(s.params, batch_from_setup(s), max_lj=256, mode=MODE_STREAMED, rseed=1234567)
+        for k in range(10):
+            st = eng.get_state()
+            _, e_traj = eng.energies()
+            np.testing.assert_array_equal(frames[k], st[:s.ntot, :3])
+            np.testing.assert_array_equal(frames1[k], st[s.ntot:, :3])
+            np.testing.assert_array_equal(angs[k], st[:s.ntot, [3, 5, 4]])   # fi, psi, theta
+            for t in range(2):
+                want = np.array([float("%f" % v) for v in e_traj[t]])
+                np.testing.assert_array_equal(np.array(energies[2 * k + t]), want)
+            eng.run(100)
+        # the result PDBs hold the last output step (900), as the reference's
+        res = [l for l in open("result_xyz.pdb").read().splitlines() if l.startswith("ATOM")]
+        xyz = np.array([[float(l[30:38]), float(l[38:46]), float(l[46:54])] for l in res])
+        np.testing.assert_allclose(xyz, frames[-1], atol=6e-4)
+        eng.close()
+        s.close()
+    finally:
+        os.chdir(old)

@@ -37,3 +37,22 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(Params) == 4 * len(Params._fields_)
     assert ctypes.sizeof(engine.Options) == 20
     assert ctypes.sizeof(engine.Topology) == 8 + 14 * 8
+
+
+def test_binding_rejects_mis_sized_batch_arrays():
+    """The ctypes binding checks every per-bead / per-trajectory array against
+    the batch shape before it crosses the ABI (the C side takes plain
+    pointers), and passes an empty box as NULL (open boundaries)."""
+    import numpy as np
+    import pytest
+    from mt_amd import params as P
+    from mt_amd.engine import Engine
+    from tests.helpers import c1_batch
+    b = c1_batch(ntraj=2)
+    b.gtp = b.gtp[:-1]
+    with pytest.raises(ValueError, match="gtp"):
+        Engine(P.derive(P.BASE), b)
+    b = c1_batch(ntraj=2)
+    b.box = np.zeros(3, np.float32)
+    with pytest.raises(ValueError, match="box"):
+        Engine(P.derive(P.BASE), b)
