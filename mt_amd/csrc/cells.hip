@@ -6,8 +6,11 @@
 // :913-940), O(N^2) per trajectory.  Here the rebuild is:
 //   1. assign: cell coordinates floor(x / cs) per bead (wrapped in periodic
 //      boxes), hashed into a per-trajectory table of 2^k buckets; count;
-//   2. scan: one workgroup per trajectory turns bucket counts into offsets;
-//   3. scatter: beads into bucket order (counting sort);
+//   2. scan: one exclusive scan over the buckets of all trajectories (whose
+//      tables and beads are both laid out trajectory after trajectory)
+//      turns bucket counts into first slots;
+//   3. scatter: beads into bucket order (counting sort), with their
+//      positions copied into that order for the searches;
 //   4. search: each bead visits the 27 neighbouring cells, keeps the beads of
 //      exactly those cells within the candidate radius, sorts the candidates
 //      by index and runs the reference's per-pair tests on them in that
@@ -71,38 +74,94 @@ __global__ void __launch_bounds__(256)
     atomicAdd(&C.count[bucket], 1);
 }
 
-// Exclusive scan of one trajectory's bucket counts; bucket starts are global
-// bead slots.  Also clears the scatter cursors.
-__global__ void __launch_bounds__(1024) k_cell_scan(DevTopo T, CellTables C) {
-    __shared__ int part[1024];
-    const int t = blockIdx.x;
-    const int off = C.toff[t], n = C.tsize[t];
-    const int per = (n + 1023) / 1024;
-    const int lo = threadIdx.x * per, hi = min(n, lo + per);
+// Exclusive scan of the bucket counts over ALL trajectories at once.  Each
+// trajectory's buckets follow the previous trajectory's, and its beads follow
+// the previous trajectory's beads, so the global prefix of bucket k is
+// directly the first slot of that bucket (tbase[t] + the local prefix).
+// Three passes: per-block sums, a scan of the block sums, the in-block scan.
+constexpr int SCAN_T = 256, SCAN_PER = CELL_SCAN_SPAN / SCAN_T;
+
+__global__ void __launch_bounds__(SCAN_T) k_scan_partial(const int *count, int n, int *bsum) {
+    __shared__ int red[SCAN_T];
+    const int base = blockIdx.x * CELL_SCAN_SPAN + threadIdx.x * SCAN_PER;
     int s = 0;
-    for (int i = lo; i < hi; i++) s += C.count[off + i];
-    part[threadIdx.x] = s;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; k++)
+        if (base + k < n) s += count[base + k];
+    red[threadIdx.x] = s;
     __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-        const int v = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0;
-        __syncthreads();
-        part[threadIdx.x] += v;
+    for (int d = SCAN_T / 2; d > 0; d >>= 1) {
+        if (threadIdx.x < d) red[threadIdx.x] += red[threadIdx.x + d];
         __syncthreads();
     }
-    int acc = T.tbase[t] + part[threadIdx.x] - s;
-    for (int i = lo; i < hi; i++) {
-        const int v = C.count[off + i];
-        C.start[off + i] = acc;
-        C.fill[off + i] = 0;
-        acc += v;
+    if (threadIdx.x == 0) bsum[blockIdx.x] = red[0];
+}
+
+// In-place exclusive scan of the nblk block sums (one workgroup).
+__global__ void __launch_bounds__(1024) k_scan_top(int *bsum, int nblk) {
+    __shared__ int part[1024];
+    int carry = 0;
+    for (int c0 = 0; c0 < nblk; c0 += 1024) {
+        const int i = c0 + threadIdx.x;
+        const int v = i < nblk ? bsum[i] : 0;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int d = 1; d < 1024; d <<= 1) {
+            const int u = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0;
+            __syncthreads();
+            part[threadIdx.x] += u;
+            __syncthreads();
+        }
+        if (i < nblk) bsum[i] = carry + part[threadIdx.x] - v;
+        const int tot = part[1023];
+        __syncthreads();
+        carry += tot;
     }
 }
 
-__global__ void __launch_bounds__(256) k_cell_scatter(DevTopo T, CellTables C) {
+// Bucket starts (for the scatter), packed (start, count) ranges (for the
+// searches); clears the scatter cursors.
+__global__ void __launch_bounds__(SCAN_T) k_scan_apply(CellTables C) {
+    __shared__ int part[SCAN_T];
+    const int n = C.total;
+    const int base = blockIdx.x * CELL_SCAN_SPAN + threadIdx.x * SCAN_PER;
+    int v[SCAN_PER];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; k++) {
+        v[k] = base + k < n ? C.count[base + k] : 0;
+        s += v[k];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int d = 1; d < SCAN_T; d <<= 1) {
+        const int u = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0;
+        __syncthreads();
+        part[threadIdx.x] += u;
+        __syncthreads();
+    }
+    int acc = C.bsum[blockIdx.x] + part[threadIdx.x] - s;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; k++) {
+        if (base + k < n) {
+            C.start[base + k] = acc;
+            C.fill[base + k] = 0;
+            C.range[base + k] = make_int2(acc, v[k]);
+        }
+        acc += v[k];
+    }
+}
+
+// Counting-sort scatter; also the bucket-ordered positions the searches
+// stream through (w = bead index, bit 31 set for extra beads).
+__global__ void __launch_bounds__(256) k_cell_scatter(DevTopo T, DevState S, CellTables C) {
     const int b = blockIdx.x * 256 + threadIdx.x;
     if (b >= T.nb) return;
     const int bucket = C.cell[b].w;
-    C.sorted[C.start[bucket] + atomicAdd(&C.fill[bucket], 1)] = b;
+    const int slot = C.start[bucket] + atomicAdd(&C.fill[bucket], 1);
+    C.sorted[slot] = b;
+    const float4 x = S.xyz[b];
+    C.spos[slot] = make_float4(x.x, x.y, x.z, __int_as_float(b | (T.extra[b] ? (int)0x80000000 : 0)));
 }
 
 // Candidates of bead b: beads of the 27 neighbouring cells within `radius`
@@ -128,7 +187,8 @@ __device__ int gather_candidates(const DevTopo &T, const DevState &S, const Cell
                     cz = (cz + g.dims) % g.dims;
                 }
                 const int bucket = C.toff[t] + (int)(cell_hash(cx, cy, cz) & mask);
-                const int s0 = C.start[bucket], s1 = s0 + C.count[bucket];
+                const int2 rg = C.range[bucket];
+                const int s0 = rg.x, s1 = rg.x + rg.y;
                 for (int q = s0; q < s1; q++) {
                     const int j = C.sorted[q];
                     if (j == b) continue;
@@ -186,8 +246,11 @@ hipError_t build_cells(mte_engine *e, float radius) {
     const int gb = (e->nb + 255) / 256;
     hipLaunchKernelGGL(k_fill_int, dim3((C.total + 255) / 256), dim3(256), 0, e->stream, C.count, C.total, 0);
     hipLaunchKernelGGL(k_cell_assign, dim3(gb), dim3(256), 0, e->stream, e->T, e->S[e->cur], C, radius);
-    hipLaunchKernelGGL(k_cell_scan, dim3(e->ntraj), dim3(1024), 0, e->stream, e->T, C);
-    hipLaunchKernelGGL(k_cell_scatter, dim3(gb), dim3(256), 0, e->stream, e->T, C);
+    const int nblk = (C.total + CELL_SCAN_SPAN - 1) / CELL_SCAN_SPAN;
+    hipLaunchKernelGGL(k_scan_partial, dim3(nblk), dim3(SCAN_T), 0, e->stream, C.count, C.total, C.bsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, e->stream, C.bsum, nblk);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nblk), dim3(SCAN_T), 0, e->stream, C);
+    hipLaunchKernelGGL(k_cell_scatter, dim3(gb), dim3(256), 0, e->stream, e->T, e->S[e->cur], C);
     return hipGetLastError();
 }
 

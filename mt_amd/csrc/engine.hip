@@ -339,7 +339,9 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
             if (dalloc(e, &p_toff, nt) != hipSuccess || dalloc(e, &p_tsz, nt) != hipSuccess ||
                 dalloc(e, &C.count, total) != hipSuccess || dalloc(e, &C.start, total) != hipSuccess ||
                 dalloc(e, &C.fill, total) != hipSuccess || dalloc(e, &C.cell, nb) != hipSuccess ||
-                dalloc(e, &C.sorted, nb) != hipSuccess) {
+                dalloc(e, &C.sorted, nb) != hipSuccess || dalloc(e, &C.range, total) != hipSuccess ||
+                dalloc(e, &C.spos, nb) != hipSuccess ||
+                dalloc(e, &C.bsum, (total + CELL_SCAN_SPAN - 1) / CELL_SCAN_SPAN) != hipSuccess) {
                 e->err = "mte_create: hipMalloc failed for cell tables";
                 return bail(MTE_ERR_HIP);
             }

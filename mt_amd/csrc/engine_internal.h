@@ -40,10 +40,15 @@ struct CellTables {
     const int *toff = nullptr;   // trajectory -> first bucket
     const int *tsize = nullptr;  // trajectory -> bucket count (power of two)
     int *count = nullptr, *start = nullptr, *fill = nullptr;   // per bucket
+    int2 *range = nullptr;       // per bucket: first slot, bead count
     int4 *cell = nullptr;        // per bead: cell coordinates, bucket
     int *sorted = nullptr;       // beads in bucket order
+    float4 *spos = nullptr;      // per slot: position of the bead there, w = its index (bit 31: extra)
+    int *bsum = nullptr;         // scan partials, one per CELL_SCAN_SPAN buckets
     int total = 0;               // buckets over all trajectories
 };
+// Buckets per block of the global bucket scan (cells.hip).
+constexpr int CELL_SCAN_SPAN = 4096;
 
 // MLP-HI network and work buffers (mlp.hip).
 constexpr int MLP_NN = 32, MLP_IN = 96, MLP_H = 128, MLP_OUT = 9;

@@ -39,9 +39,14 @@ __device__ __forceinline__ float taper(float r) {
 
 // ------------------------------------------------------------ features, labels
 // Neighbour scan over the 27 cells of the hashed table (built with cell side
-// >= 12 nm).  Keeps the 32 nearest (distance, then index) and sums the exact
-// label over every neighbour within 12 nm (LABELS: only on check steps and
-// for training data).
+// >= 12 nm, or 3 cells per side covering the whole box).  The 27 bucket
+// ranges are loaded up front; a bucket that two of the cells hash to is
+// visited once, and the candidates are streamed from the bucket-ordered
+// positions.  Every bead within 12 nm lies in one of the 27 cells, so
+// visiting their buckets (each once) sees it exactly once, and the distance
+// test rejects the beads of other cells sharing a bucket.  Keeps the 32
+// nearest (distance, then index) and sums the exact label over every
+// neighbour within 12 nm (LABELS: only on check steps and for training data).
 template <bool LABELS>
 __global__ void __launch_bounds__(MLP_BLOCK)
     k_mlp_local(mte_params P, DevTopo T, DevState S, CellTables C, float *feat, float *targ) {
@@ -60,65 +65,73 @@ __global__ void __launch_bounds__(MLP_BLOCK)
         const int dims = box > 0.f ? max(3, (int)floorf(box / MLP_RC)) : 0;
         const int4 me = C.cell[b];
         const uint32_t mask = (uint32_t)(C.tsize[t] - 1);
-        const float a = P.tea_a, ia = 1.f / P.tea_a;
-        for (int dz = -1; dz <= 1; dz++)
-            for (int dy = -1; dy <= 1; dy++)
-                for (int dx = -1; dx <= 1; dx++) {
-                    int cx = me.x + dx, cy = me.y + dy, cz = me.z + dz;
-                    if (dims > 0) {
-                        cx = (cx + dims) % dims;
-                        cy = (cy + dims) % dims;
-                        cz = (cz + dims) % dims;
-                    }
-                    const uint32_t h = ((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^
-                                       ((uint32_t)cz * 83492791u);
-                    const int bucket = C.toff[t] + (int)(h & mask);
-                    const int s0 = C.start[bucket], s1 = s0 + C.count[bucket];
-                    for (int q = s0; q < s1; q++) {
-                        const int j = C.sorted[q];
-                        if (j == b || T.extra[j]) continue;
-                        const int4 cj = C.cell[j];
-                        if (cj.x != cx || cj.y != cy || cj.z != cz) continue;
-                        const float3 d = min_image(sub3(f3(S.xyz[j]), ri), box);
-                        const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
-                        if (r2 >= MLP_RC * MLP_RC) continue;
-                        if (LABELS) {
-                            // exact label: phi(r) D_hat(d)
-                            const float r = sqrtf(r2), rinv = 1.f / r;
-                            const float ra = r * ia;
-                            float cii, crr;
-                            if (ra > 2.f) {
-                                const float u = 1.f / ra, u2 = u * u;
-                                cii = 0.75f * u * (1.f + (2.f / 3.f) * u2);
-                                crr = 0.75f * u * (1.f - 2.f * u2);
-                            } else {
-                                cii = 1.f - 9.f * ra / 32.f;
-                                crr = 3.f * ra / 32.f;
-                            }
-                            const float w = taper(r);
-                            const float e[3] = {d.x * rinv, d.y * rinv, d.z * rinv};
+        const int toff = C.toff[t];
+        const float ia = 1.f / P.tea_a;
+        int bk[27];
 #pragma unroll
-                            for (int x = 0; x < 3; x++)
+        for (int c = 0; c < 27; c++) {
+            int cx = me.x + c % 3 - 1, cy = me.y + (c / 3) % 3 - 1, cz = me.z + c / 9 - 1;
+            if (dims > 0) {
+                cx = (cx + dims) % dims;
+                cy = (cy + dims) % dims;
+                cz = (cz + dims) % dims;
+            }
+            const uint32_t h = ((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u);
+            bk[c] = toff + (int)(h & mask);
+        }
+        int2 rg[27];
 #pragma unroll
-                                for (int y = 0; y < 3; y++)
-                                    A[3 * x + y] += w * (crr * e[x] * e[y] + (x == y ? cii : 0.f));
-                        }
-                        // 32 nearest, ordered by (r2, j)
-                        const int lj = j;
-                        if (cnt < MLP_NN || r2 < nd[cnt - 1] || (r2 == nd[cnt - 1] && lj < nj[cnt - 1])) {
-                            int k = cnt < MLP_NN ? cnt++ : MLP_NN - 1;
-                            while (k > 0 && (nd[k - 1] > r2 || (nd[k - 1] == r2 && nj[k - 1] > lj))) {
-                                nd[k] = nd[k - 1];
-                                nj[k] = nj[k - 1];
-                                k--;
-                            }
-                            nd[k] = r2;
-                            nj[k] = lj;
-                        }
+        for (int c = 0; c < 27; c++) rg[c] = C.range[bk[c]];
+#pragma unroll
+        for (int c = 1; c < 27; c++) {
+            bool dup = false;
+#pragma unroll
+            for (int k = 0; k < c; k++) dup |= bk[k] == bk[c];
+            if (dup) rg[c].y = 0;
+        }
+#pragma unroll
+        for (int c = 0; c < 27; c++) {
+            for (int q = rg[c].x; q < rg[c].x + rg[c].y; q++) {
+                const float4 pj = C.spos[q];
+                const int jw = __float_as_int(pj.w);
+                const int j = jw & 0x7fffffff;
+                if (jw < 0 || j == b) continue;   // extra bead, or this bead
+                const float3 d = min_image(make_float3(pj.x - ri.x, pj.y - ri.y, pj.z - ri.z), box);
+                const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+                if (r2 >= MLP_RC * MLP_RC) continue;
+                if (LABELS) {
+                    // exact label: phi(r) D_hat(d)
+                    const float r = sqrtf(r2), rinv = 1.f / r;
+                    const float ra = r * ia;
+                    float cii, crr;
+                    if (ra > 2.f) {
+                        const float u = 1.f / ra, u2 = u * u;
+                        cii = 0.75f * u * (1.f + (2.f / 3.f) * u2);
+                        crr = 0.75f * u * (1.f - 2.f * u2);
+                    } else {
+                        cii = 1.f - 9.f * ra / 32.f;
+                        crr = 3.f * ra / 32.f;
                     }
+                    const float w = taper(r);
+                    const float e[3] = {d.x * rinv, d.y * rinv, d.z * rinv};
+#pragma unroll
+                    for (int x = 0; x < 3; x++)
+#pragma unroll
+                        for (int y = 0; y < 3; y++) A[3 * x + y] += w * (crr * e[x] * e[y] + (x == y ? cii : 0.f));
                 }
-        (void)a;
-        (void)ia;
+                // 32 nearest, ordered by (r2, j)
+                if (cnt < MLP_NN || r2 < nd[cnt - 1] || (r2 == nd[cnt - 1] && j < nj[cnt - 1])) {
+                    int k = cnt < MLP_NN ? cnt++ : MLP_NN - 1;
+                    while (k > 0 && (nd[k - 1] > r2 || (nd[k - 1] == r2 && nj[k - 1] > j))) {
+                        nd[k] = nd[k - 1];
+                        nj[k] = nj[k - 1];
+                        k--;
+                    }
+                    nd[k] = r2;
+                    nj[k] = j;
+                }
+            }
+        }
     }
     if (feat) {
         float *f = feat + (size_t)b * MLP_IN;
