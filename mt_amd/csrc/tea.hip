@@ -25,23 +25,26 @@ constexpr int TEA_CHUNK = TEA_CHUNK_BEADS;
 
 struct RpyCoef { float crr, cii; };
 
-// D_hat coefficients for distance r (src/bdhitea_kernel.cu:38-58):
-// D_hat = cii I + crr e e^T.
-__device__ __forceinline__ RpyCoef rpy_coef(float r, float rinv, float a) {
+// RPY coefficients of D_hat = cii I + crr e e^T at r^2 (rinv = 1/r), both
+// branches of src/bdhitea_kernel.cu:38-58 evaluated and selected (no
+// divergence); ia = 1/a.
+__device__ __forceinline__ RpyCoef rpy_select(float r2, float rinv, float a, float ia) {
+    const float u1 = a * rinv, u2 = u1 * u1;
+    const float ra = r2 * rinv * ia;
+    const bool near = !(ra > 2.f);
     RpyCoef c;
-    const float ra = r / a;
-    if (ra > 2.f) {
-        const float inv = 1.f / ra;
-        const float inv2 = inv * inv;
-        c.crr = 0.75f * inv * (1.f - 2.f * inv2);
-        c.cii = 0.75f * inv * (1.f + (2.f / 3.f) * inv2);
-    } else {
-        c.crr = 3.f * ra / 32.f;
-        c.cii = 1.f - 9.f * ra / 32.f;
-    }
+    c.cii = near ? fmaf(-0.28125f, ra, 1.f) : 0.75f * u1 * fmaf(0.66666666666666667f, u2, 1.f);
+    c.crr = near ? 0.09375f * ra : 0.75f * u1 * fmaf(-2.f, u2, 1.f);
     return c;
 }
 
+// Epsilon partials and ci of bead i over all j of its trajectory
+// (integrateTea_epsilon_unlisted, src/bdhitea_kernel.cu:61-101), with the
+// block sums in closed form (|e| = 1):
+//   sum of the 9 entries of D_hat      = 3 cii + crr (ex + ey + ez)^2
+//   row a sum of squares               = cii^2 + crr (2 cii + crr) e_a^2
+// Extra beads and the j == i slot enter with weight 0 (their distance kept
+// finite), so no branch sits in the pair loop.
 __global__ void __launch_bounds__(TEA_TILE)
     k_tea_eps(mte_params P, DevTopo T, DevState S, const int2 *tiles, float4 *ci, float *eps) {
     __shared__ float4 sx[TEA_TILE];
@@ -57,37 +60,43 @@ __global__ void __launch_bounds__(TEA_TILE)
         skip = T.extra[b] != 0;
     }
     const float box = T.tbox[tl.x];
-    float sxx = 0.f, syy = 0.f, szz = 0.f, sall = 0.f;
+    const float a = P.tea_a, ia = rcp_fast(P.tea_a);
+    float scc = 0.f, sx2 = 0.f, sy2 = 0.f, sz2 = 0.f, sall = 0.f;
     for (int j0 = 0; j0 < n; j0 += TEA_TILE) {
         const int jl = j0 + threadIdx.x;
         if (jl < n) {
             float4 v = S.xyz[base + jl];
-            v.w = T.extra[base + jl] ? 1.f : 0.f;
+            v.w = T.extra[base + jl] ? 0.f : 1.f;
             sx[threadIdx.x] = v;
         }
         __syncthreads();
         const int m = min(TEA_TILE, n - j0);
         if (active && !skip) {
+            const int self = li - j0;
+#pragma unroll 4
             for (int jj = 0; jj < m; jj++) {
                 const float4 xj = sx[jj];
-                if (j0 + jj == li || xj.w != 0.f) continue;
                 const float3 d = min_image(sub3(f3(xj), ri), box);
-                const float dx = d.x, dy = d.y, dz = d.z;
-                const float r = sqrtf(dx * dx + dy * dy + dz * dz);
-                const float ex = dx / r, ey = dy / r, ez = dz / r;
-                const RpyCoef c = rpy_coef(r, 1.f / r, P.tea_a);
-                const float dxx = ex * ex * c.crr + c.cii, dxy = ex * ey * c.crr, dxz = ex * ez * c.crr;
-                const float dyy = ey * ey * c.crr + c.cii, dyz = ey * ez * c.crr, dzz = ez * ez * c.crr + c.cii;
-                sall += dxx + 2.f * dxy + 2.f * dxz + dyy + 2.f * dyz + dzz;
-                sxx += dxx * dxx + dxy * dxy + dxz * dxz;
-                syy += dxy * dxy + dyy * dyy + dyz * dyz;
-                szz += dxz * dxz + dyz * dyz + dzz * dzz;
+                const bool is_self = jj == self;
+                const float wj = is_self ? 0.f : xj.w;
+                const float r2 = is_self ? 1.f : fmaxf(d.x * d.x + d.y * d.y + d.z * d.z, 1e-12f);
+                const float rinv = rsq_fast(r2);
+                const RpyCoef c = rpy_select(r2, rinv, a, ia);
+                const float ex = d.x * rinv, ey = d.y * rinv, ez = d.z * rinv;
+                const float cw = wj * c.cii;
+                const float k = wj * c.crr * fmaf(2.f, c.cii, c.crr);
+                const float se = ex + ey + ez;
+                scc = fmaf(cw, c.cii, scc);
+                sx2 = fmaf(k, ex * ex, sx2);
+                sy2 = fmaf(k, ey * ey, sy2);
+                sz2 = fmaf(k, ez * ez, sz2);
+                sall += fmaf(wj * c.crr, se * se, 3.f * cw);
             }
         }
         __syncthreads();
     }
     if (active) {
-        ci[b] = make_float4(sxx, syy, szz, 0.f);
+        ci[b] = make_float4(scc + sx2, scc + sy2, scc + sz2, 0.f);
         eps[b] = sall;
     }
 }
@@ -142,7 +151,7 @@ __global__ void __launch_bounds__(256)
 // Pair sums of one (trajectory, 256-bead i group, j chunk) work unit:
 //   partial_i = sum_{j in chunk, j != i, non-extra} D_hat_ij (F_j + beta C_i . R_j)
 // The chunk's j data sit in LDS; extra beads enter with zero force and
-// noise at a far position, so they contribute nothing and need no branch.
+// noise, so they contribute nothing and need no branch.
 // Splitting j into chunks gives ~10 work units per SIMD at 64 x 1040 beads,
 // against ~1 per SIMD for one thread per bead over the whole trajectory.
 __global__ void __launch_bounds__(TEA_TILE)
@@ -156,10 +165,7 @@ __global__ void __launch_bounds__(TEA_TILE)
         const int jb = base + u.z + q;
         float4 x = S.xyz[jb];
         float4 f = fxyz[jb], r = R[jb];
-        if (T.extra[jb]) {
-            x = make_float4(3.0e18f, 3.0e18f, 3.0e18f, 0.f);
-            f = r = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
+        if (T.extra[jb]) f = r = make_float4(0.f, 0.f, 0.f, 0.f);
         sx[q] = x;
         sf[q] = f;
         sr[q] = r;
@@ -184,18 +190,12 @@ __global__ void __launch_bounds__(TEA_TILE)
             const float3 d = min_image(sub3(f3(xj), ri), box);
             const float dx = d.x, dy = d.y, dz = d.z;
             const bool is_self = q == self;
-            const float r2 = is_self ? 1.f : dx * dx + dy * dy + dz * dz;   // keeps the j == i slot finite
+            // the j == i slot and coincident beads stay finite (extra beads carry zero F and R)
+            const float r2 = is_self ? 1.f : fmaxf(dx * dx + dy * dy + dz * dz, 1e-12f);
             const float rinv = rsq_fast(r2);
-            const float r = r2 * rinv;
-            // D_hat = cii I + crr e e^T (src/bdhitea_kernel.cu:38-58); both branches, then select
-            const float u1 = a * rinv, u2 = u1 * u1;
-            const float far_ii = 0.75f * u1 * fmaf(0.66666666666666667f, u2, 1.f);
-            const float far_rr = 0.75f * u1 * fmaf(-2.f, u2, 1.f);
-            const float ra = r * ia;
-            const bool near = !(ra > 2.f);
-            float cii = near ? fmaf(-0.28125f, ra, 1.f) : far_ii;
-            float crr = near ? 0.09375f * ra : far_rr;
-            if (is_self) cii = crr = 0.f;
+            // D_hat = cii I + crr e e^T (src/bdhitea_kernel.cu:38-58)
+            const RpyCoef c = rpy_select(r2, rinv, a, ia);
+            const float cii = is_self ? 0.f : c.cii, crr = is_self ? 0.f : c.crr;
             const float vx = fmaf(cx, Rj.x, Fj.x), vy = fmaf(cy, Rj.y, Fj.y), vz = fmaf(cz, Rj.z, Fj.z);
             const float s = (crr * rinv * rinv) * (dx * vx + dy * vy + dz * vz);
             acc.x += fmaf(cii, vx, s * dx);

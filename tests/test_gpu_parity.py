@@ -183,16 +183,20 @@ def test_tea_drift_against_fp64_exact_em():
     eng.close()
 
 
-def test_tea_batch_with_extra_beads_matches_oracle():
-    print("case TEA uneven batch with extra beads", flush=True)
+@pytest.mark.parametrize("periodic", [False, True])
+def test_tea_batch_with_extra_beads_matches_oracle(periodic):
+    """Extra (reservoir) beads take no part in TEA: excluded from epsilon and
+    from every pair sum, also under the minimum image of a periodic box."""
+    print(f"case TEA uneven batch with extra beads, periodic={periodic}", flush=True)
     from mt_amd import params as P
     p = P.derive(dict(P.TEA, tea_epsilon_freq=3))
     b = uneven_batch()
-    b.extra[b.mon_type == 5] = 0
     # mark the free dimers of trajectory 1 as reservoir (extra) beads
     s, e = int(b.off[1]), int(b.off[2])
     n_mt = 13 * 2 * 5
     b.extra[s + n_mt:e] = 1
+    if periodic:
+        b.box = np.array([130.0, 100.0, 180.0], np.float32)   # each larger than its system
     eng, orc = _pair(p, b, precision="f64")
     rng = np.random.default_rng(8)
     nz = noise(rng, 6, b.nb)
