@@ -119,6 +119,8 @@ def tea_bench(steps, warmup, cpu=True):
     b = topology.replicate(lattice.microtubule(40), 64, 0.05, 77)
     b.coords = lattice.wrap_angles(b.coords)
     eng = Engine(p, b, max_lj=48, mode=MODE_STREAMED)
+    tuned, report = eng.autotune(20)
+    print("[bench] tea autotune: " + "; ".join(report), file=sys.stderr, flush=True)
     eng.run(warmup)
     # timed region: graph-replayed step segments
     t0 = time.perf_counter()
@@ -137,7 +139,7 @@ def tea_bench(steps, warmup, cpu=True):
     out = {"value": b.nb * steps / elapsed, "unit": "monomer-steps/s", "steps": steps, "warmup": warmup,
            "ms_per_step": elapsed * 1e3 / steps,
            "config": {"workload": "TEA-HI: 64 trajectories x 1040-bead microtubule, a 2 nm, 0.89 cP, dt 1 ps, "
-                                  "epsilon/beta every 100 steps", "beads": b.nb},
+                                  "epsilon/beta every 100 steps", "beads": b.nb, "tuning": tuned},
            "roofline": {"bound": "mfma", "achieved": flops / avg / 1e12, "peak": 157.3, "unit": "TFLOP/s",
                         "frac": flops / avg / 1e12 / 157.3, "traffic": None, "kernel": "k_tea_pairs",
                         "avg_launch_ms": avg * 1e3, "alg_flops_per_launch": flops,
@@ -198,6 +200,8 @@ def mlp_bench(steps, warmup, iters):
     for mode in ("mlp", "none", "tea"):
         p = pt if mode != "none" else params.derive({**mlp_hi._cfg_of(pt), "tea_on": False})
         eng = Engine(p, b, max_lj=48, mode=MODE_STREAMED, rseed=1234567)
+        tuned, report = eng.autotune(5)
+        print(f"[bench] mlp section {mode} autotune: " + "; ".join(report), file=sys.stderr, flush=True)
         if mode == "mlp":
             eng.mlp_set_weights(w)
             eng.set_hi_mode(HI_MLP, check_every=100, tolerance=0.02)
@@ -210,7 +214,8 @@ def mlp_bench(steps, warmup, iters):
         el = time.perf_counter() - ta
         s1 = eng.get_state()
         D, De = mlp_hi.com_diffusion(b, s0, s1, steps * pt.dt)
-        r = {"value": b.nb * steps / el, "ms_per_step": el * 1e3 / steps, "D_nm2_per_ps": D, "D_stderr": De}
+        r = {"value": b.nb * steps / el, "ms_per_step": el * 1e3 / steps, "D_nm2_per_ps": D, "D_stderr": De,
+             "tuning": tuned}
         if mode == "mlp":
             err, fb, bad = eng.mlp_status()
             r.update(sampled_error_max=float(np.max(err)), fallback=bool(fb), bad_beads=int(bad))
@@ -254,6 +259,7 @@ def main():
     ap.add_argument("--no-tea", action="store_true")
     ap.add_argument("--tea-steps", type=int, default=2000)
     ap.add_argument("--no-mlp", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no mte_autotune)")
     ap.add_argument("--mlp-steps", type=int, default=300)
     ap.add_argument("--mlp-iters", type=int, default=20000)
     ap.add_argument("--cpu-traj", type=int, default=8)
@@ -279,6 +285,10 @@ def main():
     mode = {"auto": MODE_AUTO, "streamed": MODE_STREAMED, "resident": MODE_RESIDENT}[args.mode]
     p, b = build_workload()
     eng = Engine(p, b, max_lj=48, mode=mode)
+    tuned = None
+    if not args.no_tune:
+        tuned, report = eng.autotune(200)
+        print("[bench] autotune: " + "; ".join(report), file=sys.stderr, flush=True)
     print(f"[bench] beads={b.nb} trajectories={b.ntraj} mode={eng.mode} warmup={args.warmup}", file=sys.stderr,
           flush=True)
     eng.run(args.warmup)
@@ -316,7 +326,7 @@ def main():
         "config": {"workload": "batched plain Langevin: 256 trajectories x 1040-bead 13PFx40-dimer microtubule, "
                                "LJ list 8 nm rebuilt every 100 steps, HybridTaus noise",
                    "trajectories": b.ntraj, "beads_per_trajectory": int(b.sizes()[0]), "beads": b.nb,
-                   "mode": "resident" if eng.mode == MODE_RESIDENT else "streamed"},
+                   "mode": "resident" if eng.mode == MODE_RESIDENT else "streamed", "tuning": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kname, "launches": step_n, "avg_launch_ms": avg_launch_s * 1e3,

@@ -168,6 +168,32 @@ int mte_reset_timing(mte_engine *e);
 /* Which mode create() chose (MTE_MODE_STREAMED or MTE_MODE_RESIDENT). */
 int mte_get_mode(const mte_engine *e, int32_t *mode);
 
+/* ---------------------------------------------------------------- autotuning
+ * Launch shapes of the engine's kernels (no counterpart in the reference,
+ * whose launches are fixed at BLOCK_SIZE threads, src/Cuda.h:122-135).
+ * A field set to 0 keeps the current choice. */
+typedef struct {
+    int32_t resident_per_thread;  /* beads per thread of the resident kernel: 1, 2 or 3 */
+    int32_t stream_block;         /* workgroup size of the streamed bead kernels: 64, 128 or 256 */
+    int32_t tea_chunk;            /* j-chunk of the TEA pair kernel: 64, 128 or 256 beads */
+    int32_t cell_stencil;         /* cell-list rebuilds: 1 = 27 cells of side r, 2 = 125 cells of side r/2 */
+    int32_t mode;                 /* MTE_MODE_RESIDENT or MTE_MODE_STREAMED (plain dynamics) */
+    int32_t reserved[3];
+} mte_tuning;
+
+int mte_get_tuning(const mte_engine *e, mte_tuning *t);
+/* Applies the non-zero fields; MTE_ERR_ARG for a shape the batch cannot use. */
+int mte_set_tuning(mte_engine *e, const mte_tuning *t);
+/* Times every candidate shape that applies to the batch (trial_steps steps
+ * each: for plain dynamics every resident plan against every streamed
+ * workgroup size, which also picks the mode; then TEA chunks and cell
+ * stencils one family after the other) and keeps the fastest.  Coordinates, RNG streams and the step counter are restored after
+ * every trial, and lists and TEA's epsilon are recomputed at the next step;
+ * call it where both are due anyway (e.g. right after mte_create, step 0) so
+ * the tuned run is the untuned one.  MTE_ERR_STATE elsewhere, or with
+ * injected noise pending.  `report` (may be NULL) receives one line per trial. */
+int mte_autotune(mte_engine *e, int32_t trial_steps, mte_tuning *chosen, char *report, int64_t report_len);
+
 /* ------------------------------------------------------------------ MLP-HI
  * The learned hydrodynamic correction has no counterpart in the reference.
  * Per bead it predicts the local HI tensor G_i = sum_j phi(r_ij) D_hat_ij

@@ -34,14 +34,17 @@ struct CellGeom {
     int dims;      // cells per side in a periodic box (0: open)
 };
 
-__device__ __forceinline__ CellGeom cell_geom(float box, float radius) {
+// Cells of side >= `side` (searched K cells either way, so side = r / K covers
+// a radius r); a periodic box has at least 2K + 1 cells per side, in which
+// case the stencil spans the whole box.
+__device__ __forceinline__ CellGeom cell_geom(float box, float side, int K) {
     CellGeom g;
     if (box > 0.f) {
-        g.dims = max(3, (int)floorf(box / radius));
+        g.dims = max(2 * K + 1, (int)floorf(box / side));
         g.cs = box / (float)g.dims;
     } else {
         g.dims = 0;
-        g.cs = radius;
+        g.cs = side;
     }
     return g;
 }
@@ -62,12 +65,12 @@ __global__ void __launch_bounds__(256) k_fill_int(int *p, int n, int v) {
 }
 
 __global__ void __launch_bounds__(256)
-    k_cell_assign(DevTopo T, DevState S, CellTables C, float radius) {
+    k_cell_assign(DevTopo T, DevState S, CellTables C, float side, int K) {
     const int b = blockIdx.x * 256 + threadIdx.x;
     if (b >= T.nb) return;
     const int t = T.tid[b];
     const float box = T.tbox[t];
-    const CellGeom g = cell_geom(box, radius);
+    const CellGeom g = cell_geom(box, side, K);
     const int3 c = cell_of(f3(S.xyz[b]), box, g);
     const int bucket = C.toff[t] + (int)(cell_hash(c.x, c.y, c.z) & (uint32_t)(C.tsize[t] - 1));
     C.cell[b] = make_int4(c.x, c.y, c.z, bucket);
@@ -164,22 +167,23 @@ __global__ void __launch_bounds__(256) k_cell_scatter(DevTopo T, DevState S, Cel
     C.spos[slot] = make_float4(x.x, x.y, x.z, __int_as_float(b | (T.extra[b] ? (int)0x80000000 : 0)));
 }
 
-// Candidates of bead b: beads of the 27 neighbouring cells within `radius`
-// (minimum image), as local indices in ascending order.  Returns the count,
-// or -1 when more than CAND_CAP were found.
+// Candidates of bead b: beads of the (2K+1)^3 neighbouring cells (of side
+// >= radius / K) within `radius` (minimum image), as local indices in
+// ascending order.  Returns the count, or -1 when more than CAND_CAP were
+// found.
 __device__ int gather_candidates(const DevTopo &T, const DevState &S, const CellTables &C, int b, float radius,
-                                 int *cand) {
+                                 int K, int *cand) {
     const int t = T.tid[b], base = T.tbase[t];
     const float box = T.tbox[t];
-    const CellGeom g = cell_geom(box, radius);
+    const CellGeom g = cell_geom(box, radius / K, K);
     const int4 me = C.cell[b];
     const float3 ri = f3(S.xyz[b]);
     const float r2max = radius * radius;
     const uint32_t mask = (uint32_t)(C.tsize[t] - 1);
     int n = 0;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dx = -1; dx <= 1; dx++) {
+    for (int dz = -K; dz <= K; dz++)
+        for (int dy = -K; dy <= K; dy++)
+            for (int dx = -K; dx <= K; dx++) {
                 int cx = me.x + dx, cy = me.y + dy, cz = me.z + dz;
                 if (g.dims > 0) {
                     cx = (cx + g.dims) % g.dims;
@@ -211,7 +215,8 @@ __device__ int gather_candidates(const DevTopo &T, const DevState &S, const Cell
 }
 
 __global__ void __launch_bounds__(CELL_BLOCK)
-    k_rebuild_cells(mte_params P, DevTopo T, DevState S, CellTables C, float radius, int do_lj, int do_bonded) {
+    k_rebuild_cells(mte_params P, DevTopo T, DevState S, CellTables C, float radius, int K, int do_lj,
+                    int do_bonded) {
     __shared__ int cand_all[CELL_BLOCK * CAND_CAP];
     const int b = blockIdx.x * CELL_BLOCK + threadIdx.x;
     if (b >= T.nb) return;
@@ -220,7 +225,7 @@ __global__ void __launch_bounds__(CELL_BLOCK)
                                base, T.tbox[t]);
     if (!T.extra[b]) {
         int *cand = cand_all + threadIdx.x * CAND_CAP;
-        const int n = gather_candidates(T, S, C, b, radius, cand);
+        const int n = gather_candidates(T, S, C, b, radius, K, cand);
         if (n < 0) {
             o.ovf = 1;
         } else {
@@ -241,11 +246,11 @@ float candidate_radius(const mte_params &P) {
     return r * 1.0001f + 1e-4f;   // superset of every list test
 }
 
-hipError_t build_cells(mte_engine *e, float radius) {
+hipError_t build_cells(mte_engine *e, float side, int K) {
     const CellTables &C = e->cells;
     const int gb = (e->nb + 255) / 256;
     hipLaunchKernelGGL(k_fill_int, dim3((C.total + 255) / 256), dim3(256), 0, e->stream, C.count, C.total, 0);
-    hipLaunchKernelGGL(k_cell_assign, dim3(gb), dim3(256), 0, e->stream, e->T, e->S[e->cur], C, radius);
+    hipLaunchKernelGGL(k_cell_assign, dim3(gb), dim3(256), 0, e->stream, e->T, e->S[e->cur], C, side, K);
     const int nblk = (C.total + CELL_SCAN_SPAN - 1) / CELL_SCAN_SPAN;
     hipLaunchKernelGGL(k_scan_partial, dim3(nblk), dim3(SCAN_T), 0, e->stream, C.count, C.total, C.bsum);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, e->stream, C.bsum, nblk);
@@ -259,10 +264,11 @@ hipError_t launch_rebuild_cells(mte_engine *e) {
     const int do_bonded = e->P.is_assembly ? 1 : 0;
     if (!do_lj && !do_bonded) return hipSuccess;
     const float radius = candidate_radius(e->P);
-    hipError_t s = build_cells(e, radius);
+    const int K = e->tune.cell_stencil == 2 ? 2 : 1;
+    hipError_t s = build_cells(e, radius / K, K);
     if (s != hipSuccess) return s;
     hipLaunchKernelGGL(k_rebuild_cells, dim3((e->nb + CELL_BLOCK - 1) / CELL_BLOCK), dim3(CELL_BLOCK), 0, e->stream,
-                       e->P, e->T, e->S[e->cur], e->cells, radius, do_lj, do_bonded);
+                       e->P, e->T, e->S[e->cur], e->cells, radius, K, do_lj, do_bonded);
     return hipGetLastError();
 }
 

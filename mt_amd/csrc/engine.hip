@@ -1,6 +1,7 @@
 // engine.hip -- the C ABI of include/mt_engine.h: device state, topology
 // upload, the per-step driver and host<->device exchange.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -36,6 +37,48 @@ static hipError_t dalloc(mte_engine *e, T **p, size_t count) {
 static int fail(mte_engine *e, int code, const std::string &msg) {
     e->err = msg;
     return code;
+}
+
+static void release(mte_engine *e, void *p) {
+    if (!p) return;
+    (void)hipFree(p);
+    e->allocations.erase(std::remove(e->allocations.begin(), e->allocations.end(), p), e->allocations.end());
+}
+
+// Work units (trajectory, 256-bead i group, j chunk) of the TEA pair kernel
+// and its per-chunk partial buffer, for j-chunks of `chunk` beads.
+hipError_t mte::tea_build_units(mte_engine *e, int chunk) {
+    release(e, e->tea.units);
+    release(e, e->tea.nchunks);
+    release(e, e->tea.partial);
+    e->tea.units = nullptr;
+    e->tea.nchunks = nullptr;
+    e->tea.partial = nullptr;
+    std::vector<int4> units;
+    std::vector<int> nchunks(e->ntraj, 0);
+    int maxc = 0;
+    for (int t = 0; t < e->ntraj; t++) {
+        const int n = (int)(e->off[t + 1] - e->off[t]);
+        int c = 0;
+        for (int j0 = 0; j0 < n; j0 += chunk, c++)
+            for (int i0 = 0; i0 < n; i0 += 256) units.push_back(make_int4(t, i0, j0, c));
+        nchunks[t] = c;
+        maxc = std::max(maxc, c);
+    }
+    hipError_t s;
+    if ((s = dalloc(e, &e->tea.units, units.size())) != hipSuccess) return s;
+    if ((s = dalloc(e, &e->tea.nchunks, (size_t)e->ntraj)) != hipSuccess) return s;
+    if ((s = dalloc(e, &e->tea.partial, (size_t)maxc * e->nb)) != hipSuccess) return s;
+    if ((s = hipMemcpyAsync(e->tea.units, units.data(), sizeof(int4) * units.size(), hipMemcpyHostToDevice,
+                            e->stream)) != hipSuccess)
+        return s;
+    if ((s = hipMemcpyAsync(e->tea.nchunks, nchunks.data(), sizeof(int) * nchunks.size(), hipMemcpyHostToDevice,
+                            e->stream)) != hipSuccess)
+        return s;
+    e->tea.nunits = (int)units.size();
+    e->tea.max_chunks = maxc;
+    e->tune.tea_chunk = chunk;
+    return hipStreamSynchronize(e->stream);
 }
 
 // src/ran2.h:18-56 with state in a struct; positive seed skips the table
@@ -162,6 +205,9 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     const int nb = (int)e->off[nt];
     e->nb = nb;
     e->ntraj = nt;
+    e->tune.stream_block = 256;
+    e->tune.tea_chunk = TEA_CHUNK_BEADS;
+    e->tune.cell_stencil = 1;
     e->device = opt->device;
     if (hipSetDevice(e->device) != hipSuccess) { e->err = "mte_create: hipSetDevice failed"; return bail(MTE_ERR_HIP); }
     if (hipDeviceGetAttribute(&e->num_cus, hipDeviceAttributeMultiprocessorCount, e->device) != hipSuccess ||
@@ -247,20 +293,13 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     }
     ALLOC(e->seeds, (size_t)2 * nb) ALLOC(e->fxyz, nb) ALLOC(e->fang, nb) ALLOC(e->en_bead, (size_t)7 * nb)
     ALLOC(e->en_traj, (size_t)7 * nt)
-    std::vector<int4> tea_units;
-    std::vector<int> tea_nchunks(nt, 0);
     if (e->P.tea_on) {
-        for (int t = 0; t < nt; t++) {
-            int c = 0;
-            for (int j0 = 0; j0 < tsize[t]; j0 += TEA_CHUNK_BEADS, c++)
-                for (int i0 = 0; i0 < tsize[t]; i0 += 256) tea_units.push_back(make_int4(t, i0, j0, c));
-            tea_nchunks[t] = c;
-            e->tea.max_chunks = std::max(e->tea.max_chunks, c);
-        }
-        e->tea.nunits = (int)tea_units.size();
         ALLOC(e->tea.ci, nb) ALLOC(e->tea.eps, nb) ALLOC(e->tea.beta, nt) ALLOC(e->tea.stats, 2 * nt)
-        ALLOC(e->tea.status, 1) ALLOC(e->tea.R, nb) ALLOC(e->tea.units, tea_units.size())
-        ALLOC(e->tea.nchunks, nt) ALLOC(e->tea.partial, (size_t)e->tea.max_chunks * nb)
+        ALLOC(e->tea.status, 1) ALLOC(e->tea.R, nb)
+        if (tea_build_units(e, TEA_CHUNK_BEADS) != hipSuccess) {
+            e->err = "mte_create: TEA work units";
+            return bail(MTE_ERR_HIP);
+        }
     }
 #undef ALLOC
     T.tid = p_tid; T.tbase = p_tbase; T.tsize = p_tsize; T.harm = p_harm; T.harm_raw = p_harm_raw;
@@ -297,9 +336,6 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     UP(p_gtp, gtp.data(), nb) UP(p_ont, ont.data(), nb) UP(p_zs, zs.data(), nt)
     UP(p_box, box.data(), nt)
     UP(e->tiles, tiles.data(), tiles.size()) UP(e->seeds, seeds.data(), seeds.size())
-    if (e->P.tea_on) {
-        UP(e->tea.units, tea_units.data(), tea_units.size()) UP(e->tea.nchunks, tea_nchunks.data(), nt)
-    }
 #undef UP
     if (hipMemsetAsync(T.lj_n, 0, sizeof(int) * nb, e->stream) != hipSuccess ||
         hipMemsetAsync(T.overflow, 0, sizeof(int), e->stream) != hipSuccess) {
@@ -380,6 +416,8 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     }
     e->mode = (opt->mode == MTE_MODE_STREAMED || !fits) ? MTE_MODE_STREAMED : MTE_MODE_RESIDENT;
     e->rplan = plan;
+    e->tune.resident_per_thread = fits ? plan.per_thread : 0;
+    e->tune.mode = e->mode;
     if (hipStreamSynchronize(e->stream) != hipSuccess) { e->err = "mte_create: sync failed"; return bail(MTE_ERR_HIP); }
     *out = e;
     return MTE_OK;
@@ -1001,3 +1039,182 @@ int mte_kernel_time(mte_engine *e, const char *name, double *total_ms, int64_t *
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ tuning
+static void clear_graphs(mte_engine *e) {
+    for (auto &kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
+    e->graphs.clear();
+}
+
+int mte_get_tuning(const mte_engine *e, mte_tuning *t) {
+    if (!e || !t) return MTE_ERR_ARG;
+    *t = e->tune;
+    return MTE_OK;
+}
+
+int mte_set_tuning(mte_engine *e, const mte_tuning *t) {
+    if (!e || !t) return MTE_ERR_ARG;
+    HIPCHK(hipSetDevice(e->device));
+    if (t->mode) {
+        ResidentPlan plan;
+        if (t->mode == MTE_MODE_RESIDENT) {
+            if (!resident_fits(e, &plan))
+                return fail(e, MTE_ERR_ARG, "mte_set_tuning: the batch does not fit the resident kernel");
+            if (e->mode != MTE_MODE_RESIDENT) {
+                e->rplan = plan;
+                e->tune.resident_per_thread = plan.per_thread;
+            }
+        } else if (t->mode != MTE_MODE_STREAMED) {
+            return fail(e, MTE_ERR_ARG, "mte_set_tuning: mode must be MTE_MODE_RESIDENT or MTE_MODE_STREAMED");
+        }
+        if (t->mode != e->mode) {
+            // both paths rebuild their lists at the next rebuild step; switch there
+            if ((e->P.lj_on || e->P.is_assembly) && e->step % e->P.ljpairsupdatefreq != 0)
+                return fail(e, MTE_ERR_STATE, "mte_set_tuning: switch modes at a list rebuild step");
+            e->mode = t->mode;
+            e->lists_step = -1;
+            e->lists_valid = false;
+        }
+        e->tune.mode = e->mode;
+    }
+    if (t->resident_per_thread) {
+        ResidentPlan plan;
+        if (e->mode != MTE_MODE_RESIDENT || !resident_plan_for(e, t->resident_per_thread, &plan))
+            return fail(e, MTE_ERR_ARG, "mte_set_tuning: resident_per_thread not usable for this batch or mode");
+        e->rplan = plan;
+        e->tune.resident_per_thread = t->resident_per_thread;
+    }
+    if (t->stream_block) {
+        if (t->stream_block != 64 && t->stream_block != 128 && t->stream_block != 256)
+            return fail(e, MTE_ERR_ARG, "mte_set_tuning: stream_block must be 64, 128 or 256");
+        e->tune.stream_block = t->stream_block;
+    }
+    if (t->tea_chunk) {
+        if (t->tea_chunk != 64 && t->tea_chunk != 128 && t->tea_chunk != 256)
+            return fail(e, MTE_ERR_ARG, "mte_set_tuning: tea_chunk must be 64, 128 or 256");
+        if (!e->P.tea_on) return fail(e, MTE_ERR_ARG, "mte_set_tuning: tea_chunk needs TEA");
+        if (t->tea_chunk != e->tune.tea_chunk) HIPCHK(tea_build_units(e, t->tea_chunk));
+    }
+    if (t->cell_stencil) {
+        if (t->cell_stencil != 1 && t->cell_stencil != 2)
+            return fail(e, MTE_ERR_ARG, "mte_set_tuning: cell_stencil must be 1 or 2");
+        e->tune.cell_stencil = t->cell_stencil;
+    }
+    clear_graphs(e);   // captured launches hold the old shapes
+    return MTE_OK;
+}
+
+int mte_autotune(mte_engine *e, int32_t trial_steps, mte_tuning *chosen, char *report, int64_t report_len) {
+    if (!e || trial_steps <= 0) return MTE_ERR_ARG;
+    if (e->noise) return fail(e, MTE_ERR_STATE, "mte_autotune: injected noise pending");
+    const bool need_lists = e->P.lj_on || e->P.is_assembly;
+    if ((need_lists && e->step % e->P.ljpairsupdatefreq != 0) ||
+        (e->P.tea_on && e->step % e->P.tea_epsilon_freq != 0))
+        return fail(e, MTE_ERR_STATE, "mte_autotune: call where lists and epsilon are due (e.g. step 0)");
+    HIPCHK(hipSetDevice(e->device));
+    std::vector<mte_coord> r(e->nb);
+    std::vector<uint32_t> rng((size_t)8 * e->nb);
+    int rc;
+    if ((rc = mte_get_state(e, r.data())) || (rc = mte_get_rng(e, rng.data()))) return rc;
+    const int64_t step0 = e->step;
+    const bool fb0 = e->mlp_fallback;
+    const bool timing0 = e->timing;
+    e->timing = false;
+    std::string log;
+    auto restore = [&]() -> int {
+        int q;
+        if ((q = mte_set_state(e, r.data())) || (q = mte_set_rng(e, rng.data()))) return q;
+        e->step = step0;
+        e->lists_step = -1;
+        e->lists_valid = false;
+        e->tea_valid = false;
+        e->mlp_fallback = fb0;
+        return MTE_OK;
+    };
+    // one warm-up run (graph capture, first-touch) and one timed run per candidate
+    auto trial_with = [&](const mte_tuning &t, const char *what, int value, double *ms) -> int {
+        int q;
+        if ((q = restore()) || (q = mte_set_tuning(e, &t)) || (q = mte_run(e, trial_steps)) || (q = restore()))
+            return q;
+        const auto t0 = std::chrono::steady_clock::now();
+        if ((q = mte_run(e, trial_steps))) return q;
+        *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / trial_steps;
+        char line[160];
+        std::snprintf(line, sizeof line, "%s %d: %.4f ms/step\n", what, value, *ms);
+        log += line;
+        return MTE_OK;
+    };
+    auto trial = [&](const char *what, int value, double *ms) -> int {
+        mte_tuning t{};
+        if (!strcmp(what, "stream_block")) t.stream_block = value;
+        else if (!strcmp(what, "tea_chunk")) t.tea_chunk = value;
+        else t.cell_stencil = value;
+        return trial_with(t, what, value, ms);
+    };
+    auto family = [&](const char *what, std::initializer_list<int> values, int *field) -> int {
+        int best = *field;
+        double best_ms = 1e300;
+        for (int v : values) {
+            double ms = 0;
+            const int q = trial(what, v, &ms);
+            if (q) return q;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = v;
+            }
+        }
+        mte_tuning t{};
+        if (!strcmp(what, "stream_block")) t.stream_block = best;
+        else if (!strcmp(what, "tea_chunk")) t.tea_chunk = best;
+        else t.cell_stencil = best;
+        *field = best;
+        return mte_set_tuning(e, &t);
+    };
+    // plain dynamics: every resident plan against every streamed workgroup size
+    ResidentPlan plan;
+    const bool can_resident = resident_fits(e, &plan);
+    if (!e->P.tea_on) {
+        mte_tuning best{};
+        double best_ms = 1e300;
+        auto consider = [&](const mte_tuning &t, const char *what, int value) -> int {
+            double ms = 0;
+            int q = trial_with(t, what, value, &ms);
+            if (q) return q;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+            return MTE_OK;
+        };
+        if (can_resident)
+            for (int pt = 1; pt <= 3; pt++)
+                if (resident_plan_for(e, pt, &plan)) {
+                    mte_tuning t{};
+                    t.mode = MTE_MODE_RESIDENT;
+                    t.resident_per_thread = pt;
+                    if ((rc = consider(t, "resident_per_thread", pt))) return rc;
+                }
+        for (int bs : {64, 128, 256}) {
+            mte_tuning t{};
+            t.mode = MTE_MODE_STREAMED;
+            t.stream_block = bs;
+            if ((rc = consider(t, "stream_block", bs))) return rc;
+        }
+        if ((rc = restore()) || (rc = mte_set_tuning(e, &best))) return rc;
+    } else {
+        if ((rc = family("stream_block", {64, 128, 256}, &e->tune.stream_block))) return rc;
+        if ((rc = family("tea_chunk", {64, 128, 256}, &e->tune.tea_chunk))) return rc;
+    }
+    if (e->mode == MTE_MODE_STREAMED && e->use_cells && need_lists &&
+        (rc = family("cell_stencil", {1, 2}, &e->tune.cell_stencil)))
+        return rc;
+    if ((rc = restore())) return rc;
+    e->timing = timing0;
+    if (chosen) *chosen = e->tune;
+    if (report && report_len > 0) {
+        const size_t n = std::min<size_t>((size_t)report_len - 1, log.size());
+        std::memcpy(report, log.data(), n);
+        report[n] = '\0';
+    }
+    return MTE_OK;
+}

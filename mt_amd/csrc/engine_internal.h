@@ -113,6 +113,7 @@ struct mte_engine {
     std::vector<double> mlp_last_err;
     int64_t mlp_bad = 0;
     mte::ResidentPlan rplan;
+    mte_tuning tune{};   // current launch shapes (mte_autotune / mte_set_tuning)
 
     int64_t step = 0;
     int64_t lists_step = -1;
@@ -148,12 +149,16 @@ hipError_t launch_mlp_adam(mte_engine *e, float lr, int t);
 hipError_t launch_mlp_round(mte_engine *e);
 hipError_t launch_mlp_pack(mte_engine *e);
 size_t mlp_image_bytes();
-hipError_t build_cells(mte_engine *e, float radius);
+hipError_t build_cells(mte_engine *e, float side, int K);
 float candidate_radius(const mte_params &P);
 hipError_t launch_tea_eps(mte_engine *e);
 hipError_t launch_tea_pairs(mte_engine *e);
 hipError_t launch_tea_finish(mte_engine *e, const float *noise_step);
 hipError_t launch_resident(mte_engine *e, int nsteps, bool rebuild_first, const float *noise);
 bool resident_fits(const mte_engine *e, ResidentPlan *plan);
+// Resident plan with `pt` beads per thread; false if the batch cannot use it.
+bool resident_plan_for(const mte_engine *e, int pt, ResidentPlan *plan);
+// TEA pair work units and partial buffers for j-chunks of `chunk` beads.
+hipError_t tea_build_units(mte_engine *e, int chunk);
 
 }  // namespace mte

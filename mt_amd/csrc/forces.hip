@@ -6,6 +6,8 @@
 
 namespace mte {
 
+// Workgroup size of k_frames; the per-step bead kernels take theirs from the
+// engine's tuning (64, 128 or 256).
 constexpr int STREAM_BLOCK = 256;
 
 struct GlobalAcc {
@@ -35,10 +37,11 @@ __global__ void __launch_bounds__(STREAM_BLOCK) k_frames(DevState S, int nb, Lat
 // Force + Euler-Maruyama step.  Reads Sin, writes Sout (both positions and
 // rotated offsets), advances the HybridTaus states of mobile beads only
 // (src/compute_cuda.cu:949-953) unless noise is injected.
-__global__ void __launch_bounds__(STREAM_BLOCK)
+template <int BS>
+__global__ void __launch_bounds__(BS)
     k_step_plain(mte_params P, DevTopo T, DevState Sin, DevState Sout, uint4 *seeds, const float *noise,
                  LatOffsets L) {
-    const int b = blockIdx.x * STREAM_BLOCK + threadIdx.x;
+    const int b = blockIdx.x * BS + threadIdx.x;
     if (b >= T.nb) return;
     float4 x = Sin.xyz[b], a = Sin.ang[b];
     const bool extra = T.extra[b];
@@ -89,9 +92,10 @@ struct TeaDraw {
 
 // Forces (compute_kernel semantics; extra beads get zero), optionally with
 // the TEA random forces.
-__global__ void __launch_bounds__(STREAM_BLOCK)
+template <int BS>
+__global__ void __launch_bounds__(BS)
     k_forces(mte_params P, DevTopo T, DevState S, float4 *fxyz, float4 *fang, TeaDraw D) {
-    const int b = blockIdx.x * STREAM_BLOCK + threadIdx.x;
+    const int b = blockIdx.x * BS + threadIdx.x;
     if (b >= T.nb) return;
     float F[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (!T.extra[b]) {
@@ -116,9 +120,10 @@ __global__ void __launch_bounds__(STREAM_BLOCK)
     }
 }
 
-__global__ void __launch_bounds__(STREAM_BLOCK)
+template <int BS>
+__global__ void __launch_bounds__(BS)
     k_energies(mte_params P, DevTopo T, DevState S, double *en) {
-    const int b = blockIdx.x * STREAM_BLOCK + threadIdx.x;
+    const int b = blockIdx.x * BS + threadIdx.x;
     if (b >= T.nb) return;
     float U[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (!T.extra[b]) {
@@ -149,7 +154,19 @@ __global__ void __launch_bounds__(256) k_traj_sum(DevTopo T, const double *en, d
     if (threadIdx.x < 7) out[7 * t + threadIdx.x] = red[threadIdx.x][0];
 }
 
-static inline int grid_for(int n) { return (n + STREAM_BLOCK - 1) / STREAM_BLOCK; }
+static inline int grid_for(int n, int bs = STREAM_BLOCK) { return (n + bs - 1) / bs; }
+
+// Launch of a per-bead kernel template at the tuned workgroup size.
+#define MTE_LAUNCH_BS(KERNEL, e, ...)                                                                              \
+    do {                                                                                                           \
+        const int bs_ = (e)->tune.stream_block;                                                                    \
+        if (bs_ == 64)                                                                                             \
+            hipLaunchKernelGGL(KERNEL<64>, dim3(grid_for((e)->nb, 64)), dim3(64), 0, (e)->stream, __VA_ARGS__);   \
+        else if (bs_ == 128)                                                                                       \
+            hipLaunchKernelGGL(KERNEL<128>, dim3(grid_for((e)->nb, 128)), dim3(128), 0, (e)->stream, __VA_ARGS__); \
+        else                                                                                                       \
+            hipLaunchKernelGGL(KERNEL<256>, dim3(grid_for((e)->nb, 256)), dim3(256), 0, (e)->stream, __VA_ARGS__); \
+    } while (0)
 
 hipError_t launch_frames(mte_engine *e, int buf) {
     hipLaunchKernelGGL(k_frames, dim3(grid_for(e->nb)), dim3(STREAM_BLOCK), 0, e->stream, e->S[buf], e->nb, e->L);
@@ -157,27 +174,23 @@ hipError_t launch_frames(mte_engine *e, int buf) {
 }
 
 hipError_t launch_step_plain(mte_engine *e, const float *noise_step) {
-    hipLaunchKernelGGL(k_step_plain, dim3(grid_for(e->nb)), dim3(STREAM_BLOCK), 0, e->stream, e->P, e->T,
-                       e->S[e->cur], e->S[1 - e->cur], e->seeds, noise_step, e->L);
+    MTE_LAUNCH_BS(k_step_plain, e, e->P, e->T, e->S[e->cur], e->S[1 - e->cur], e->seeds, noise_step, e->L);
     e->cur = 1 - e->cur;
     return hipGetLastError();
 }
 
 hipError_t launch_forces(mte_engine *e) {
-    hipLaunchKernelGGL(k_forces, dim3(grid_for(e->nb)), dim3(STREAM_BLOCK), 0, e->stream, e->P, e->T, e->S[e->cur],
-                       e->fxyz, e->fang, TeaDraw{nullptr, nullptr, nullptr});
+    MTE_LAUNCH_BS(k_forces, e, e->P, e->T, e->S[e->cur], e->fxyz, e->fang, TeaDraw{nullptr, nullptr, nullptr});
     return hipGetLastError();
 }
 
 hipError_t launch_forces_tea(mte_engine *e, const float *noise_step) {
-    hipLaunchKernelGGL(k_forces, dim3(grid_for(e->nb)), dim3(STREAM_BLOCK), 0, e->stream, e->P, e->T, e->S[e->cur],
-                       e->fxyz, e->fang, TeaDraw{e->seeds, noise_step, e->tea.R});
+    MTE_LAUNCH_BS(k_forces, e, e->P, e->T, e->S[e->cur], e->fxyz, e->fang, TeaDraw{e->seeds, noise_step, e->tea.R});
     return hipGetLastError();
 }
 
 hipError_t launch_energies(mte_engine *e) {
-    hipLaunchKernelGGL(k_energies, dim3(grid_for(e->nb)), dim3(STREAM_BLOCK), 0, e->stream, e->P, e->T,
-                       e->S[e->cur], e->en_bead);
+    MTE_LAUNCH_BS(k_energies, e, e->P, e->T, e->S[e->cur], e->en_bead);
     hipLaunchKernelGGL(k_traj_sum, dim3(e->ntraj), dim3(256), 0, e->stream, e->T, e->en_bead, e->en_traj);
     return hipGetLastError();
 }

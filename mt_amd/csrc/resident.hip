@@ -295,9 +295,11 @@ static size_t fixed_lds_bytes(int nmax) {
            (size_t)SCAN_INTS * sizeof(int);
 }
 
-bool resident_fits(const mte_engine *e, ResidentPlan *plan) {
+// Launch plan with pt beads per thread (pt = 1 allows up to 1024 threads,
+// 2 and 3 up to 768 so the multi-bead kernels keep their registers).
+bool resident_plan_for(const mte_engine *e, int pt, ResidentPlan *plan) {
     *plan = ResidentPlan{};
-    if (e->P.tea_on) return false;
+    if (e->P.tea_on || pt < 1 || pt > RES_MAX_PT) return false;
     int nmax = 0;
     for (int t = 0; t < e->ntraj; t++) nmax = std::max<int>(nmax, (int)(e->off[t + 1] - e->off[t]));
     if (nmax >= 32768) return false;   // uint16 list entries
@@ -306,17 +308,22 @@ bool resident_fits(const mte_engine *e, ResidentPlan *plan) {
         return false;
     const size_t fixed = fixed_lds_bytes(nmax);
     if (fixed + (size_t)MIN_ENTRIES_PER_BEAD * 2 * nmax > (size_t)lds_limit) return false;
-    int pt = 1;
-    if (nmax > RES_MAX_THREADS) pt = (nmax + RES_MULTI_THREADS - 1) / RES_MULTI_THREADS;
-    if (pt > RES_MAX_PT) return false;
     int threads = (nmax + pt - 1) / pt;
     threads = ((threads + 63) / 64) * 64;
+    if (threads > (pt == 1 ? RES_MAX_THREADS : RES_MULTI_THREADS)) return false;
     plan->threads = threads;
     plan->per_thread = pt;
     plan->max_beads = nmax;
     plan->lds_bytes = (size_t)lds_limit & ~(size_t)15;
     plan->entry_cap = (int)std::min<size_t>((plan->lds_bytes - fixed) / 2, 1u << 30);
     return true;
+}
+
+// Default plan: the fewest beads per thread the batch allows.
+bool resident_fits(const mte_engine *e, ResidentPlan *plan) {
+    for (int pt = 1; pt <= RES_MAX_PT; pt++)
+        if (resident_plan_for(e, pt, plan)) return true;
+    return false;
 }
 
 template <int PT, int MAXT>

@@ -12,7 +12,7 @@
 // non-extra beads of the trajectory.
 //
 // Work layout of the per-step product: work units of (trajectory, 256-bead i
-// group, 128-bead j chunk), the chunk's positions, forces and random forces in
+// group, j chunk of 64, 128 or 256 beads as tuned), the chunk's positions, forces and random forces in
 // LDS, FP32 partial sums per chunk reduced in fixed order by the finish kernel,
 // which adds the self term and integrates.  Each (i, j) block costs one rsqrt
 // and ~35 FP32 operations: the product is VALU-bound.
@@ -21,7 +21,6 @@
 namespace mte {
 
 constexpr int TEA_TILE = 256;
-constexpr int TEA_CHUNK = TEA_CHUNK_BEADS;
 
 struct RpyCoef { float crr, cii; };
 
@@ -154,13 +153,14 @@ __global__ void __launch_bounds__(256)
 // noise, so they contribute nothing and need no branch.
 // Splitting j into chunks gives ~10 work units per SIMD at 64 x 1040 beads,
 // against ~1 per SIMD for one thread per bead over the whole trajectory.
+template <int CHUNK>
 __global__ void __launch_bounds__(TEA_TILE)
     k_tea_pairs(mte_params P, DevTopo T, DevState S, const float4 *fxyz, const float4 *R, const float4 *ci,
                 const float *beta, const int4 *units, int chunk_stride, float4 *partial) {
-    __shared__ float4 sx[TEA_CHUNK], sf[TEA_CHUNK], sr[TEA_CHUNK];
+    __shared__ float4 sx[CHUNK], sf[CHUNK], sr[CHUNK];
     const int4 u = units[blockIdx.x];   // traj, i0, j0, chunk index
     const int base = T.tbase[u.x], n = T.tsize[u.x];
-    const int jn = min(TEA_CHUNK, n - u.z);
+    const int jn = min(CHUNK, n - u.z);
     for (int q = threadIdx.x; q < jn; q += TEA_TILE) {
         const int jb = base + u.z + q;
         float4 x = S.xyz[jb];
@@ -268,8 +268,23 @@ hipError_t launch_tea_eps(mte_engine *e) {
 }
 
 hipError_t launch_tea_pairs(mte_engine *e) {
-    hipLaunchKernelGGL(k_tea_pairs, dim3(e->tea.nunits), dim3(TEA_TILE), 0, e->stream, e->P, e->T, e->S[e->cur],
-                       e->fxyz, e->tea.R, e->tea.ci, e->tea.beta, e->tea.units, e->nb, e->tea.partial);
+    // the work units were built for this chunk length (tea_build_units)
+    switch (e->tune.tea_chunk) {
+    case 64:
+        hipLaunchKernelGGL(k_tea_pairs<64>, dim3(e->tea.nunits), dim3(TEA_TILE), 0, e->stream, e->P, e->T,
+                           e->S[e->cur], e->fxyz, e->tea.R, e->tea.ci, e->tea.beta, e->tea.units, e->nb,
+                           e->tea.partial);
+        break;
+    case 256:
+        hipLaunchKernelGGL(k_tea_pairs<256>, dim3(e->tea.nunits), dim3(TEA_TILE), 0, e->stream, e->P, e->T,
+                           e->S[e->cur], e->fxyz, e->tea.R, e->tea.ci, e->tea.beta, e->tea.units, e->nb,
+                           e->tea.partial);
+        break;
+    default:
+        hipLaunchKernelGGL(k_tea_pairs<128>, dim3(e->tea.nunits), dim3(TEA_TILE), 0, e->stream, e->P, e->T,
+                           e->S[e->cur], e->fxyz, e->tea.R, e->tea.ci, e->tea.beta, e->tea.units, e->nb,
+                           e->tea.partial);
+    }
     return hipGetLastError();
 }
 

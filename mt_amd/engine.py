@@ -44,6 +44,16 @@ class Options(ctypes.Structure):
                 ("device", ctypes.c_int32), ("list_method", ctypes.c_int32)]
 
 
+class Tuning(ctypes.Structure):
+    """mte_tuning: launch shapes chosen by mte_autotune (0 = keep)."""
+    _fields_ = [("resident_per_thread", ctypes.c_int32), ("stream_block", ctypes.c_int32),
+                ("tea_chunk", ctypes.c_int32), ("cell_stencil", ctypes.c_int32), ("mode", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 3)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 # Every symbol include/mt_engine.h declares, with its ctypes signature.
 _P = ctypes.c_void_p
 SIGNATURES = {
@@ -69,6 +79,9 @@ SIGNATURES = {
     "mte_kernel_time": ([_P, ctypes.c_char_p, _P, _P], ctypes.c_int),
     "mte_reset_timing": ([_P], ctypes.c_int),
     "mte_get_mode": ([_P, _P], ctypes.c_int),
+    "mte_get_tuning": ([_P, _P], ctypes.c_int),
+    "mte_set_tuning": ([_P, _P], ctypes.c_int),
+    "mte_autotune": ([_P, ctypes.c_int32, _P, ctypes.c_char_p, ctypes.c_int64], ctypes.c_int),
     "mte_mlp_set_weights": ([_P, _P, ctypes.c_int64], ctypes.c_int),
     "mte_mlp_get_weights": ([_P, _P, ctypes.c_int64], ctypes.c_int),
     "mte_mlp_local": ([_P, _P, _P], ctypes.c_int),
@@ -239,6 +252,23 @@ class Engine:
         beta = np.zeros(self.ntraj)
         self._chk(self.lib.mte_tea_stats(self.h, _p(eps), _p(beta)))
         return eps, beta
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        self._chk(self.lib.mte_get_tuning(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **fields):
+        t = Tuning(**fields)
+        self._chk(self.lib.mte_set_tuning(self.h, ctypes.byref(t)))
+
+    def autotune(self, trial_steps: int = 100):
+        """Time the candidate launch shapes on this batch and keep the fastest;
+        returns (chosen shapes, per-trial report lines)."""
+        t = Tuning()
+        buf = ctypes.create_string_buffer(8192)
+        self._chk(self.lib.mte_autotune(self.h, trial_steps, ctypes.byref(t), buf, len(buf)))
+        return t.as_dict(), buf.value.decode().splitlines()
 
     def set_timing(self, on: bool):
         self._chk(self.lib.mte_set_timing(self.h, int(on)))

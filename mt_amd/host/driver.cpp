@@ -8,6 +8,7 @@
 //   engine_mode    auto | streamed | resident   (default auto)
 //   engine_max_lj  LJ list capacity per bead     (default 256, the reference's)
 //   engine_quiet   yes | no                      (default no)
+//   engine_autotune  steps per trial of mte_autotune before the run (default 0: off)
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -30,6 +31,7 @@ int main(int argc, char **argv) {
         opt.mode = mode == "resident" ? MTE_MODE_RESIDENT : mode == "streamed" ? MTE_MODE_STREAMED : MTE_MODE_AUTO;
         opt.max_lj = main_cfg.integer("engine_max_lj", 256);
         opt.quiet = main_cfg.yesno("engine_quiet", false);
+        opt.autotune_steps = main_cfg.integer("engine_autotune", 0);
         mth::RunSetup setup = mth::load_run(argv[1], overrides);
         return mth::run(setup, opt) == MTE_OK ? 0 : 1;
     } catch (const std::exception &ex) {

@@ -301,6 +301,12 @@ int run(RunSetup &s, const RunOptions &o) {
     int rc = mte_create(&P, &topo, &opt, &E.e);
     if (rc) return fail(E.e, "mte_create", rc);
     if ((rc = mte_set_state(E.e, s.coords.data()))) return fail(E.e, "mte_set_state", rc);
+    if (o.autotune_steps > 0) {
+        mte_tuning t{};
+        char report[4096];
+        if ((rc = mte_autotune(E.e, o.autotune_steps, &t, report, sizeof report))) return fail(E.e, "mte_autotune", rc);
+        if (!o.quiet) std::printf("Autotune:\n%s", report);
+    }
 
     Updater up(s, o);
     up.create_dcds();

@@ -25,6 +25,7 @@ struct RunOptions {
     int32_t mode = MTE_MODE_AUTO;   // engine execution mode
     int32_t max_lj = 256;           // LJ list capacity (the reference's)
     bool quiet = false;             // suppress the per-event log lines
+    int autotune_steps = 0;         // > 0: mte_autotune with this many steps per trial before the run
     std::string result_xyz = "result_xyz.pdb", result_ang = "result_ang.pdb";
     std::string mt_len_file = "mt_len.dat", hydrolysis_pdb = "dcd/hydrolysis.pdb";
 };

@@ -37,6 +37,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(Params) == 4 * len(Params._fields_)
     assert ctypes.sizeof(engine.Options) == 20
     assert ctypes.sizeof(engine.Topology) == 8 + 14 * 8
+    assert ctypes.sizeof(engine.Tuning) == 8 * 4
 
 
 def test_binding_rejects_mis_sized_batch_arrays():
