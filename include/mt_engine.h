@@ -175,7 +175,7 @@ int mte_get_mode(const mte_engine *e, int32_t *mode);
 typedef struct {
     int32_t resident_per_thread;  /* beads per thread of the resident kernel: 1, 2 or 3 */
     int32_t stream_block;         /* workgroup size of the streamed bead kernels: 64, 128 or 256 */
-    int32_t tea_chunk;            /* j-chunk of the TEA pair kernel: 64, 128 or 256 beads */
+    int32_t tea_chunk;            /* j-chunk of the TEA pair kernel: 64, 128 or 256 beads (0 without TEA) */
     int32_t cell_stencil;         /* cell-list rebuilds: 1 = 27 cells of side r, 2 = 125 cells of side r/2 */
     int32_t mode;                 /* MTE_MODE_RESIDENT or MTE_MODE_STREAMED (plain dynamics) */
     int32_t reserved[3];

@@ -77,6 +77,7 @@ hipError_t mte::tea_build_units(mte_engine *e, int chunk) {
         return s;
     e->tea.nunits = (int)units.size();
     e->tea.max_chunks = maxc;
+    e->tea.unit_chunk = chunk;
     e->tune.tea_chunk = chunk;
     return hipStreamSynchronize(e->stream);
 }
@@ -206,7 +207,7 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     e->nb = nb;
     e->ntraj = nt;
     e->tune.stream_block = 256;
-    e->tune.tea_chunk = TEA_CHUNK_BEADS;
+    e->tune.tea_chunk = 0;   // set when TEA's work units are built
     e->tune.cell_stencil = 1;
     e->device = opt->device;
     if (hipSetDevice(e->device) != hipSuccess) { e->err = "mte_create: hipSetDevice failed"; return bail(MTE_ERR_HIP); }
@@ -1078,10 +1079,11 @@ int mte_set_tuning(mte_engine *e, const mte_tuning *t) {
         e->tune.mode = e->mode;
     }
     if (t->resident_per_thread) {
+        // a plan for the resident kernel; kept but unused while the engine runs streamed
         ResidentPlan plan;
-        if (e->mode != MTE_MODE_RESIDENT || !resident_plan_for(e, t->resident_per_thread, &plan))
-            return fail(e, MTE_ERR_ARG, "mte_set_tuning: resident_per_thread not usable for this batch or mode");
-        e->rplan = plan;
+        if (!resident_plan_for(e, t->resident_per_thread, &plan))
+            return fail(e, MTE_ERR_ARG, "mte_set_tuning: resident_per_thread not usable for this batch");
+        if (e->mode == MTE_MODE_RESIDENT) e->rplan = plan;
         e->tune.resident_per_thread = t->resident_per_thread;
     }
     if (t->stream_block) {
@@ -1089,11 +1091,11 @@ int mte_set_tuning(mte_engine *e, const mte_tuning *t) {
             return fail(e, MTE_ERR_ARG, "mte_set_tuning: stream_block must be 64, 128 or 256");
         e->tune.stream_block = t->stream_block;
     }
-    if (t->tea_chunk) {
+    if (t->tea_chunk && t->tea_chunk != e->tune.tea_chunk) {
         if (t->tea_chunk != 64 && t->tea_chunk != 128 && t->tea_chunk != 256)
             return fail(e, MTE_ERR_ARG, "mte_set_tuning: tea_chunk must be 64, 128 or 256");
         if (!e->P.tea_on) return fail(e, MTE_ERR_ARG, "mte_set_tuning: tea_chunk needs TEA");
-        if (t->tea_chunk != e->tune.tea_chunk) HIPCHK(tea_build_units(e, t->tea_chunk));
+        if (t->tea_chunk != e->tea.unit_chunk) HIPCHK(tea_build_units(e, t->tea_chunk));
     }
     if (t->cell_stencil) {
         if (t->cell_stencil != 1 && t->cell_stencil != 2)
@@ -1163,11 +1165,12 @@ int mte_autotune(mte_engine *e, int32_t trial_steps, mte_tuning *chosen, char *r
                 best = v;
             }
         }
+        // mte_set_tuning applies the choice (and rebuilds what depends on it)
+        (void)field;
         mte_tuning t{};
         if (!strcmp(what, "stream_block")) t.stream_block = best;
         else if (!strcmp(what, "tea_chunk")) t.tea_chunk = best;
         else t.cell_stencil = best;
-        *field = best;
         return mte_set_tuning(e, &t);
     };
     // plain dynamics: every resident plan against every streamed workgroup size

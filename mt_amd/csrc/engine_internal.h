@@ -27,6 +27,7 @@ struct TeaBuffers {
     float4 *R = nullptr;       // per bead: scaled random force
     int4 *units = nullptr;     // pair work units: traj, i0, j0, chunk index
     int nunits = 0;
+    int unit_chunk = 0;        // j-chunk length the units were built for (the pair kernel's CHUNK)
     int *nchunks = nullptr;    // per trajectory
     int max_chunks = 0;
     float4 *partial = nullptr; // max_chunks x nb pair partial sums

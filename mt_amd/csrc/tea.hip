@@ -268,8 +268,8 @@ hipError_t launch_tea_eps(mte_engine *e) {
 }
 
 hipError_t launch_tea_pairs(mte_engine *e) {
-    // the work units were built for this chunk length (tea_build_units)
-    switch (e->tune.tea_chunk) {
+    // the kernel's chunk must be the one the work units were built for
+    switch (e->tea.unit_chunk) {
     case 64:
         hipLaunchKernelGGL(k_tea_pairs<64>, dim3(e->tea.nunits), dim3(TEA_TILE), 0, e->stream, e->P, e->T,
                            e->S[e->cur], e->fxyz, e->tea.R, e->tea.ci, e->tea.beta, e->tea.units, e->nb,
@@ -280,10 +280,13 @@ hipError_t launch_tea_pairs(mte_engine *e) {
                            e->S[e->cur], e->fxyz, e->tea.R, e->tea.ci, e->tea.beta, e->tea.units, e->nb,
                            e->tea.partial);
         break;
-    default:
+    case 128:
         hipLaunchKernelGGL(k_tea_pairs<128>, dim3(e->tea.nunits), dim3(TEA_TILE), 0, e->stream, e->P, e->T,
                            e->S[e->cur], e->fxyz, e->tea.R, e->tea.ci, e->tea.beta, e->tea.units, e->nb,
                            e->tea.partial);
+        break;
+    default:
+        return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
