@@ -64,6 +64,27 @@ def read_traffic(name):
                      + vals["write_kib_per_dispatch"] * vals.get("write_calibration", 1.0))
 
 
+def read_tea_traffic(chunk):
+    """HBM bytes per k_tea_pairs dispatch for j-chunks of `chunk` beads, from
+    the committed rocprofv3 --pmc summary (FETCH_SIZE, WRITE_SIZE per chunk
+    length, scaled by the copy-kernel calibration recorded there)."""
+    path = os.path.join(ROOT, "profiles", "k_tea_pairs_c3.txt")
+    if not os.path.exists(path):
+        return None
+    cal = {"fetch_calibration": 1.0, "write_calibration": 1.0}
+    found = None
+    for line in open(path):
+        m = re.match(r"chunk (\d+): fetch_kib_per_dispatch = ([0-9.]+), write_kib_per_dispatch = ([0-9.]+)", line)
+        if m and int(m.group(1)) == chunk:
+            found = (float(m.group(2)), float(m.group(3)))
+        m = re.match(r"(fetch_calibration|write_calibration) = ([0-9.]+)", line)
+        if m:
+            cal[m.group(1)] = float(m.group(2))
+    if found is None:
+        return None
+    return 1024.0 * (found[0] * cal["fetch_calibration"] + found[1] * cal["write_calibration"])
+
+
 def cpu_baseline(ntraj_sample, steps_sample):
     """The FP32 CPU oracle (OpenMP over beads, all host cores) on a bounded
     sample: ntraj_sample trajectories of the same 1040-bead microtubule."""
@@ -141,10 +162,11 @@ def tea_bench(steps, warmup, cpu=True):
            "config": {"workload": "TEA-HI: 64 trajectories x 1040-bead microtubule, a 2 nm, 0.89 cP, dt 1 ps, "
                                   "epsilon/beta every 100 steps", "beads": b.nb, "tuning": tuned},
            "roofline": {"bound": "mfma", "achieved": flops / avg / 1e12, "peak": 157.3, "unit": "TFLOP/s",
-                        "frac": flops / avg / 1e12 / 157.3, "traffic": None, "kernel": "k_tea_pairs",
-                        "avg_launch_ms": avg * 1e3, "alg_flops_per_launch": flops,
+                        "frac": flops / avg / 1e12 / 157.3, "traffic": read_tea_traffic(tuned["tea_chunk"]),
+                        "kernel": "k_tea_pairs", "avg_launch_ms": avg * 1e3, "alg_flops_per_launch": flops,
                         "note": "algorithmic work 36 B(B-1) flops per trajectory-step; the kernel evaluates it "
-                                "on VALU (FP32), the peak quoted is the FP32 matrix/vector figure"},
+                                "on VALU (FP32, ~48 instructions per block, at the non-packed issue rate); the "
+                                "peak quoted is the FP32 figure, which counts packed FMAs"},
            "force_error_vs_fp64": tea_error_vs_fp64()}
     if cpu:
         from oracle.oracle import Oracle, generate_seeds, max_threads
