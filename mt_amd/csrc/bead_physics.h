@@ -58,13 +58,13 @@ __device__ __forceinline__ void bending(const mte_params &P, float rm, float th0
                                         float F[6]) {
     const float psiji = aj.z - I.psi, thetaji = aj.y - I.theta, fiji = aj.x - I.fi;
     if (rm > 0.f) {
-        F[5] += P.B_psi * sin_fast(psiji - P.psi_0);
-        F[3] += P.B_fi * sin_fast(fiji - P.fi_0);
-        F[4] += P.B_theta * sin_fast(thetaji - th0);
+        F[5] += P.B_psi * sin_fast_near0(psiji - P.psi_0);
+        F[3] += P.B_fi * sin_fast_near0(fiji - P.fi_0);
+        F[4] += P.B_theta * sin_fast_near0(thetaji - th0);
     } else {
-        F[5] -= P.B_psi * sin_fast(-psiji - P.psi_0);
-        F[3] -= P.B_fi * sin_fast(-fiji - P.fi_0);
-        F[4] -= P.B_theta * sin_fast(-thetaji - th0);
+        F[5] -= P.B_psi * sin_fast_near0(-psiji - P.psi_0);
+        F[3] -= P.B_fi * sin_fast_near0(-fiji - P.fi_0);
+        F[4] -= P.B_theta * sin_fast_near0(-thetaji - th0);
     }
 }
 
@@ -128,10 +128,9 @@ __device__ __forceinline__ void bead_force(const mte_params &P, const DevTopo &T
         const float3 a = sw ? I.a1 : I.a2;
         const float3 bj = f3(sw ? A.fp2(j) : A.fp1(j));
         const Grad G = gap_grad(min_image(sub3(I.r, f3(A.xyz(j))), I.box), a, bj, I.uphi, I.uth);
-        float dUdr;
-        if (G.dr == 0.f) dUdr = 0.f;
-        else if (A.mon(b) != A.mon(j)) dUdr = dmorse(P.D_lat / P.seam_coeff, P.A_lat, G.dr) * G.inv;
-        else dUdr = dmorse(P.D_lat, P.A_lat, G.dr) * G.inv;
+        // seam bonds (monomer types differ) are weakened by seam_coeff (:447-452)
+        const float Dl = A.mon(b) != A.mon(j) ? P.D_lat / P.seam_coeff : P.D_lat;
+        float dUdr = (G.dr == 0.f) ? 0.f : dmorse(Dl, P.A_lat, G.dr) * G.inv;
         if (P.barrier && A.offtub(b) && A.offtub(j) && G.dr != 0.f)
             dUdr += dbarr(P.a_barr_lat, P.r_barr_lat, P.w_barr_lat, G.dr) * G.inv;
 #pragma unroll

@@ -108,6 +108,17 @@ __device__ __forceinline__ float sin_fast(float x) {
     sincos_fast(x, &s, &c);
     return s;
 }
+// sin_fast for arguments that usually need no reduction (bonded angle
+// differences): when x rounds to quadrant 0 the reduction is the identity
+// and only the sine polynomial is evaluated -- the same value sin_fast
+// returns -- and a wave takes the full path only if one of its lanes needs it.
+__device__ __forceinline__ float sin_fast_near0(float x) {
+    if (rintf(x * 0.63661977236758134f) == 0.f) {
+        const float z = x * x;
+        return fmaf(z * x, fmaf(z, fmaf(z, -1.9515295891e-4f, 8.3321608736e-3f), -1.6666654611e-1f), x);
+    }
+    return sin_fast(x);
+}
 __device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float rsq_fast(float x) { return __builtin_amdgcn_rsqf(x); }
 
