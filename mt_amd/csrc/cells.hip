@@ -240,10 +240,7 @@ __global__ void __launch_bounds__(CELL_BLOCK)
 }
 
 float candidate_radius(const mte_params &P) {
-    float r = 0.f;
-    if (P.lj_on) r = fmaxf(r, P.ljpairscutoff);
-    if (P.is_assembly) r = fmaxf(r, BOND_PREFILTER);
-    return r * 1.0001f + 1e-4f;   // superset of every list test
+    return rebuild_radius(P.ljpairscutoff, P.lj_on != 0, P.is_assembly != 0);   // superset of every list test
 }
 
 hipError_t build_cells(mte_engine *e, float side, int K) {

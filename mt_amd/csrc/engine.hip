@@ -637,13 +637,16 @@ int mte_run(mte_engine *e, int64_t nsteps) {
             // One launch covers every step up to the next rebuild boundary.
             const int64_t to_boundary = freq - (e->step % freq);
             const int64_t chunk = std::min<int64_t>(nsteps - done, to_boundary);
-            const bool rb = need_lists && rebuild_due(e);
+            // lists due at this boundary are rebuilt by the streamed kernel first (same stream)
+            if (need_lists && rebuild_due(e)) {
+                const int rc = rebuild_now(e);
+                if (rc) return rc;
+            }
             const float *nz = e->noise ? e->noise + (size_t)6 * e->nb * e->noise_cursor : nullptr;
             hipEvent_t ev{};
             time_begin(e, "step", &ev);
-            HIPCHK(launch_resident(e, (int)chunk, rb, nz));
+            HIPCHK(launch_resident(e, (int)chunk, nz));
             time_end(e, "step", ev);
-            if (rb) { e->lists_step = e->step; e->lists_valid = true; }
             if (e->noise) e->noise_cursor += chunk;
             e->step += chunk;
             done += chunk;

@@ -155,7 +155,7 @@ float candidate_radius(const mte_params &P);
 hipError_t launch_tea_eps(mte_engine *e);
 hipError_t launch_tea_pairs(mte_engine *e);
 hipError_t launch_tea_finish(mte_engine *e, const float *noise_step);
-hipError_t launch_resident(mte_engine *e, int nsteps, bool rebuild_first, const float *noise);
+hipError_t launch_resident(mte_engine *e, int nsteps, const float *noise);
 bool resident_fits(const mte_engine *e, ResidentPlan *plan);
 // Resident plan with `pt` beads per thread; false if the batch cannot use it.
 bool resident_plan_for(const mte_engine *e, int pt, ResidentPlan *plan);

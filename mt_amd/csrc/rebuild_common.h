@@ -9,6 +9,23 @@
 
 namespace mte {
 
+// Radius beyond which no pair enters any list: the LJ list cutoff and the
+// bonded prefilter, widened so float rounding of a distance cannot cross it.
+__host__ __device__ inline float rebuild_radius(float ljcut, bool do_lj, bool do_bonded) {
+    float r = 0.f;
+    if (do_lj) r = fmaxf(r, ljcut);
+    if (do_bonded) r = fmaxf(r, BOND_PREFILTER);
+    return r * 1.0001f + 1e-4f;
+}
+
+// True when the box [lo, hi] comes within r of point p (open boundaries).
+__device__ __forceinline__ bool tile_within(float3 p, float4 lo, float4 hi, float r) {
+    const float ex = fmaxf(fmaxf(lo.x - p.x, p.x - hi.x), 0.f);
+    const float ey = fmaxf(fmaxf(lo.y - p.y, p.y - hi.y), 0.f);
+    const float ez = fmaxf(fmaxf(lo.z - p.z, p.z - hi.z), 0.f);
+    return ex * ex + ey * ey + ez * ez < r * r;
+}
+
 struct RebuildOwn {
     float3 ri, ani, a1, a2;   // position, R_MON n_i, R p1, R p2
     float rp;                  // R_MON used by pairs_kernel

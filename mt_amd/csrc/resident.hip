@@ -7,8 +7,9 @@
 // (local index << 1 | flag).  Each thread owns PT beads and keeps their two
 // HybridTaus states in registers.  A launch runs every step up to the next
 // list-rebuild boundary (LJPairsUpdateFreq steps; 100 in the batched
-// workload), doing the rebuild itself first when it is due, so HBM sees the
-// state once per launch instead of twice per step.  Per step:
+// workload), after the rebuild kernel of lists.hip has refreshed the lists
+// when one is due, so HBM sees the state once per launch instead of twice
+// per step.  Per step:
 //   forces + Euler-Maruyama for own beads, every operand from LDS
 //   -> barrier -> own new state to LDS -> barrier.
 // The arithmetic is bead_force / em_update, the code the streamed step runs.
@@ -16,7 +17,6 @@
 // them from the global k-major arrays instead (same results, slower).
 #include "bead_physics.h"
 #include "engine_internal.h"
-#include "rebuild_common.h"
 
 namespace mte {
 
@@ -137,8 +137,8 @@ __device__ __forceinline__ void run_steps(const mte_params &P, const DevTopo &T,
 
 template <int PT, int MAXT>
 __global__ void __launch_bounds__(MAXT)
-    k_resident(mte_params P, DevTopo T, DevState S, uint4 *seeds, const float *noise, int nsteps, int rebuild,
-               int do_lj, int do_bonded, LatOffsets L, int stride, int entry_cap) {
+    k_resident(mte_params P, DevTopo T, DevState S, uint4 *seeds, const float *noise, int nsteps, LatOffsets L,
+               int stride, int entry_cap) {
     extern __shared__ float4 lds[];
     float4 *lx = lds, *ln = lds + stride, *l1 = lds + 2 * stride, *l2 = lds + 3 * stride;
     int *hoff = reinterpret_cast<int *>(lds + STATE_ARRAYS * stride);
@@ -184,20 +184,6 @@ __global__ void __launch_bounds__(MAXT)
     }
     __syncthreads();
     const LdsAcc A{lx, ln, l1, l2, base};
-    if (rebuild) {
-#pragma unroll
-        for (int k = 0; k < PT; k++) {
-            const int i = threadIdx.x + k * blockDim.x;
-            if (i >= n) continue;
-            const int b = base + i;
-            RebuildOwn o = rebuild_own(lx[i], ln[i], l1[i], l2[i], A.mon(b), T.harm_raw[b], i, b, base, T.tbox[t]);
-            if (!T.extra[b])
-                for (int j = 0; j < n; j++)
-                    rebuild_pair(T, o, j, lx[j], ln[j], l1[j], l2[j], __float_as_int(lx[j].w) & FL_MON,
-                                 P.ljpairscutoff, do_lj, do_bonded);
-            rebuild_finish(T, o, do_lj, do_bonded);
-        }
-    }
     // Compact this trajectory's lists into LDS: totals -> block scan -> copy.
     int mytot[PT];
 #pragma unroll
@@ -327,26 +313,22 @@ bool resident_fits(const mte_engine *e, ResidentPlan *plan) {
 }
 
 template <int PT, int MAXT>
-static hipError_t launch_pt(mte_engine *e, int nsteps, int rb, int do_lj, int do_bonded, const float *noise) {
+static hipError_t launch_pt(mte_engine *e, int nsteps, const float *noise) {
     const ResidentPlan &pl = e->rplan;
     hipError_t s = hipFuncSetAttribute((const void *)k_resident<PT, MAXT>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds_bytes);
     if (s != hipSuccess) return s;
     hipLaunchKernelGGL((k_resident<PT, MAXT>), dim3(e->ntraj), dim3(pl.threads), pl.lds_bytes, e->stream, e->P,
-                       e->T, e->S[e->cur], e->seeds, noise, nsteps, rb, do_lj, do_bonded, e->L, pl.max_beads,
-                       pl.entry_cap);
+                       e->T, e->S[e->cur], e->seeds, noise, nsteps, e->L, pl.max_beads, pl.entry_cap);
     return hipGetLastError();
 }
 
-hipError_t launch_resident(mte_engine *e, int nsteps, bool rebuild_first, const float *noise) {
+hipError_t launch_resident(mte_engine *e, int nsteps, const float *noise) {
     const ResidentPlan &pl = e->rplan;
-    const int do_lj = e->P.lj_on ? 1 : 0;
-    const int do_bonded = e->P.is_assembly ? 1 : 0;
-    const int rb = (rebuild_first && (do_lj || do_bonded)) ? 1 : 0;
     switch (pl.per_thread) {
-    case 1: return launch_pt<1, RES_MAX_THREADS>(e, nsteps, rb, do_lj, do_bonded, noise);
-    case 2: return launch_pt<2, RES_MULTI_THREADS>(e, nsteps, rb, do_lj, do_bonded, noise);
-    case 3: return launch_pt<3, RES_MULTI_THREADS>(e, nsteps, rb, do_lj, do_bonded, noise);
+    case 1: return launch_pt<1, RES_MAX_THREADS>(e, nsteps, noise);
+    case 2: return launch_pt<2, RES_MULTI_THREADS>(e, nsteps, noise);
+    case 3: return launch_pt<3, RES_MULTI_THREADS>(e, nsteps, noise);
     default: return hipErrorInvalidValue;
     }
 }
