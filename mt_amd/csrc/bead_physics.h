@@ -68,22 +68,53 @@ __device__ __forceinline__ void bending(const mte_params &P, float rm, float th0
     }
 }
 
-// Neighbour lists held in the k-major global arrays (packed global entries).
+// Neighbour lists held in the k-major global arrays of bead b, with entry
+// indices taken relative to `base` (0: global indices, the streamed kernels;
+// the trajectory's first bead: local indices, the resident kernel).
 struct GlobalLists {
     const DevTopo &T;
-    int b;
-    __device__ __forceinline__ int harm() const { return T.harm[b]; }
+    int b, base;
+    __device__ __forceinline__ int harm() const {
+        const int h = T.harm[b];
+        return h >= 0 ? h - 2 * base : -1;
+    }
     __device__ __forceinline__ int nlon() const { return T.lon_n[b]; }
-    __device__ __forceinline__ int lon(int k) const { return T.lon[k * T.nb + b]; }
+    __device__ __forceinline__ int lon(int k) const { return T.lon[k * T.nb + b] - 2 * base; }
     __device__ __forceinline__ int nlat() const { return T.lat_n[b]; }
-    __device__ __forceinline__ int lat(int k) const { return T.lat[k * T.nb + b]; }
+    __device__ __forceinline__ int lat(int k) const { return T.lat[k * T.nb + b] - 2 * base; }
     __device__ __forceinline__ int nlj() const { return T.lj_n[b]; }
-    __device__ __forceinline__ int lj(int k) const { return T.lj[k * T.nb + b]; }
+    __device__ __forceinline__ int lj(int k) const { return T.lj[k * T.nb + b] - base; }
 };
 
+// LJ repulsion over the list; the cutoff test selects the force (no branch
+// around the accumulation), and periodic and open boxes get separate loops.
+template <bool PERIODIC, class Acc, class Lists>
+__device__ __forceinline__ void lj_forces(const mte_params &P, const Acc &A, const Lists &Ls, const Own &I,
+                                          float F[6]) {
+    const int nlj = Ls.nlj();
+    const float s0 = P.ljscale * P.ljsigma6;
+    float fx = F[0], fy = F[1], fz = F[2];   // same accumulation order, scalar registers
+    for (int k = 0; k < nlj; k++) {
+        const int j = Ls.lj(k);
+        float3 d = sub3(I.r, f3(A.xyz(j)));
+        if (PERIODIC) d = min_image(d, I.box);
+        const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+        const float inv = rcp_fast(r2);
+        const float s = r2 < LJ_CUT * LJ_CUT ? s0 * (6.f * (inv * inv) * (inv * inv)) : 0.f;
+        fx += s * d.x;
+        fy += s * d.y;
+        fz += s * d.z;
+    }
+    F[0] = fx;
+    F[1] = fy;
+    F[2] = fz;
+}
+
+// Force and torque on bead b (an accessor index, as are the list entries);
+// wall_z is Parameters.zs of the bead's trajectory.
 template <class Acc, class Lists>
-__device__ __forceinline__ void bead_force(const mte_params &P, const DevTopo &T, const Acc &A, const Lists &Ls,
-                                           int b, const Own &I, float F[6]) {
+__device__ __forceinline__ void bead_force(const mte_params &P, const Acc &A, const Lists &Ls, int b, float wall_z,
+                                           const Own &I, float F[6]) {
 #pragma unroll
     for (int q = 0; q < 6; q++) F[q] = 0.f;
     // harmonic (intra-dimer spring + bending), :70-182
@@ -138,26 +169,12 @@ __device__ __forceinline__ void bead_force(const mte_params &P, const DevTopo &T
     }
     // LJ repulsion r^-6 below the hard 6 nm cutoff, :470-495
     if (P.lj_on) {
-        const int nlj = Ls.nlj();
-        const float s0 = P.ljscale * P.ljsigma6;
-        for (int k = 0; k < nlj; k++) {
-            const int j = Ls.lj(k);
-            const float3 d = min_image(sub3(I.r, f3(A.xyz(j))), I.box);
-            const float dx = d.x, dy = d.y, dz = d.z;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 < LJ_CUT * LJ_CUT) {
-                const float inv = rcp_fast(r2);
-                const float df = 6.f * (inv * inv) * (inv * inv);
-                const float s = s0 * df;
-                F[0] += s * dx;
-                F[1] += s * dy;
-                F[2] += s * dz;
-            }
-        }
+        if (I.box > 0.f) lj_forces<true>(P, A, Ls, I, F);
+        else lj_forces<false>(P, A, Ls, I, F);
     }
     // cylindrical and end walls, :497-517
     if (P.is_wall) {
-        const float z = I.r.z, lb = P.rep_leftborder, zt = T.zs[T.tid[b]] + lb;
+        const float z = I.r.z, lb = P.rep_leftborder, zt = wall_z + lb;
         if (z < lb) F[2] += P.rep_eps * fabsf(z - lb);
         else if (z > zt) F[2] += -P.rep_eps * fabsf(z - zt);
         const float rad2 = I.r.x * I.r.x + I.r.y * I.r.y;

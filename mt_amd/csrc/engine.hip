@@ -288,6 +288,7 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     ALLOC(T.lon_n, nb) ALLOC(T.lon, (size_t)MAXLON * nb) ALLOC(T.lat_n, nb) ALLOC(T.lat, (size_t)MAXLAT * nb)
     ALLOC(T.lj_n, nb) ALLOC(T.lj, (size_t)T.max_lj * nb) ALLOC(p_fixed, nb) ALLOC(p_extra, nb) ALLOC(p_mon, nb)
     ALLOC(p_gtp, nb) ALLOC(p_ont, nb) ALLOC(p_zs, nt) ALLOC(p_box, nt) ALLOC(T.overflow, 1) ALLOC(e->tiles, tiles.size())
+    ALLOC(e->rb_order, nb)
     for (int k = 0; k < 2; k++) {
         ALLOC(e->S[k].xyz, nb) ALLOC(e->S[k].ang, nb) ALLOC(e->S[k].fn, nb) ALLOC(e->S[k].fp1, nb)
         ALLOC(e->S[k].fp2, nb)
@@ -337,6 +338,10 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     UP(p_gtp, gtp.data(), nb) UP(p_ont, ont.data(), nb) UP(p_zs, zs.data(), nt)
     UP(p_box, box.data(), nt)
     UP(e->tiles, tiles.data(), tiles.size()) UP(e->seeds, seeds.data(), seeds.size())
+    std::vector<int> order(nb);   // rebuild order: index order until a state arrives (mte_set_state)
+    for (int t = 0; t < nt; t++)
+        for (int i = 0; i < tsize[t]; i++) order[tbase[t] + i] = i;
+    UP(e->rb_order, order.data(), nb)
 #undef UP
     if (hipMemsetAsync(T.lj_n, 0, sizeof(int) * nb, e->stream) != hipSuccess ||
         hipMemsetAsync(T.overflow, 0, sizeof(int), e->stream) != hipSuccess) {
@@ -448,6 +453,7 @@ int mte_set_state(mte_engine *e, const mte_coord *r) {
     HIPCHK(hipMemcpyAsync(e->S[e->cur].xyz, xyz.data(), sizeof(float4) * e->nb, hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(e->S[e->cur].ang, ang.data(), sizeof(float4) * e->nb, hipMemcpyHostToDevice, e->stream));
     HIPCHK(launch_frames(e, e->cur));
+    HIPCHK(launch_rebuild_order(e));
     HIPCHK(hipStreamSynchronize(e->stream));
     return MTE_OK;
 }

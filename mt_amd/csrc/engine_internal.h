@@ -102,6 +102,7 @@ struct mte_engine {
     double *en_bead = nullptr, *en_traj = nullptr;
     int2 *tiles = nullptr;
     int ntiles = 0;
+    int *rb_order = nullptr;   // per trajectory: local bead handled at each rebuild position (spatial order)
     int *d_overflow = nullptr;
     mte::TeaBuffers tea;
     mte::CellTables cells;
@@ -140,6 +141,7 @@ hipError_t launch_forces(mte_engine *e);
 hipError_t launch_forces_tea(mte_engine *e, const float *noise_step);
 hipError_t launch_energies(mte_engine *e);
 hipError_t launch_rebuild(mte_engine *e);
+hipError_t launch_rebuild_order(mte_engine *e);
 hipError_t launch_rebuild_cells(mte_engine *e);
 hipError_t launch_mlp_local(mte_engine *e, float *feat, float *targ);
 hipError_t launch_mlp_forward(mte_engine *e, const float *feat, int n, float *out);

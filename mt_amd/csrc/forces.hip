@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(BS)
     Own I = make_own(x, a, Sin.fn[b], Sin.fp1[b], Sin.fp2[b]);
     I.box = T.tbox[T.tid[b]];
     float F[6];
-    bead_force(P, T, GlobalAcc{Sin, T}, GlobalLists{T, b}, b, I, F);
+    bead_force(P, GlobalAcc{Sin, T}, GlobalLists{T, b, 0}, b, P.is_wall ? T.zs[T.tid[b]] : 0.f, I, F);
     float4 rx, ra;
     if (noise) {
         const float *nz = noise + 6 * (size_t)b;
@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(BS)
     if (!T.extra[b]) {
         Own I = make_own(S.xyz[b], S.ang[b], S.fn[b], S.fp1[b], S.fp2[b]);
         I.box = T.tbox[T.tid[b]];
-        bead_force(P, T, GlobalAcc{S, T}, GlobalLists{T, b}, b, I, F);
+        bead_force(P, GlobalAcc{S, T}, GlobalLists{T, b, 0}, b, P.is_wall ? T.zs[T.tid[b]] : 0.f, I, F);
     }
     fxyz[b] = make_float4(F[0], F[1], F[2], 0.f);
     fang[b] = make_float4(F[3], F[4], F[5], 0.f);   // fi, theta, psi

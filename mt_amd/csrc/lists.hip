@@ -8,7 +8,11 @@
 // the partner, either orientation's lateral gap < 2.5 nm.  Entries come out
 // in the reference's order (ascending j; lateral orientation 0 before 1).
 //
-// Work layout: one workgroup per (trajectory, 256-bead tile of i); the j
+// Work layout: one workgroup per (trajectory, 256 rebuild positions); a
+// position stands for the bead k_rebuild_order put there when the state was
+// last uploaded, which orders each trajectory's beads along a Morton curve of
+// cells so that the 64 beads of a wave sit close together and skip far
+// neighbour groups together.  The j
 // range of the trajectory streams through LDS in 256-bead tiles, so each
 // neighbour's position and rotated offsets are read from HBM once per tile
 // instead of once per (i, j).  Within a tile, j is visited in groups of 8
@@ -29,10 +33,91 @@ namespace mte {
 constexpr int LIST_TILE = 256;
 constexpr int LIST_GROUP = 8;                        // beads per bounding box
 constexpr int LIST_NGROUPS = LIST_TILE / LIST_GROUP;
-constexpr int LIST_CAND = 24;                        // buffered candidates per lane before a flush
+constexpr int LIST_CAND = 32;                        // candidate slots per lane
+constexpr int ORDER_MAX = 2048;                      // larger trajectories keep the index order
+constexpr int ORDER_THREADS = 1024;
+
+// 7-bit coordinate spread to every third bit (Morton interleave).
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+    v &= 0x7fu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// Rebuild order of each trajectory: beads sorted by the Morton code of their
+// cell (side `side`, coordinates taken modulo 128 cells), ties by index.
+// One workgroup per trajectory; keys (code << 11 | bead) bitonic-sorted in
+// LDS.  Any order gives the same lists; this one only makes the bounding-box
+// skips of k_rebuild coherent across a wave.
+__global__ void __launch_bounds__(ORDER_THREADS) k_rebuild_order(DevTopo T, DevState S, float side, int *order) {
+    __shared__ uint32_t key[ORDER_MAX];
+    const int t = blockIdx.x;
+    const int base = T.tbase[t], n = T.tsize[t];
+    if (n > ORDER_MAX) {
+        for (int p = threadIdx.x; p < n; p += blockDim.x) order[base + p] = p;
+        return;
+    }
+    int N = 1;
+    while (N < n) N <<= 1;
+    const float inv = 1.f / side;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        uint32_t k = 0xffffffffu;   // padding sorts last
+        if (i < n) {
+            const float4 x = S.xyz[base + i];
+            const uint32_t cx = (uint32_t)(int)floorf(x.x * inv), cy = (uint32_t)(int)floorf(x.y * inv),
+                           cz = (uint32_t)(int)floorf(x.z * inv);
+            k = ((spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2)) << 11) | (uint32_t)i;
+        }
+        key[i] = k;
+    }
+    __syncthreads();
+    for (int k = 2; k <= N; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int q = threadIdx.x; q < N / 2; q += blockDim.x) {
+                const int a = 2 * q - (q & (j - 1)), c = a + j;   // a has bit j clear
+                const uint32_t ka = key[a], kc = key[c];
+                if ((ka > kc) == ((a & k) == 0)) {
+                    key[a] = kc;
+                    key[c] = ka;
+                }
+            }
+            __syncthreads();
+        }
+    for (int p = threadIdx.x; p < n; p += blockDim.x) order[base + p] = (int)(key[p] & (ORDER_MAX - 1));
+}
+
+// Candidates of the bead at ri among the m beads of the j tile in LDS: the
+// beads of every group whose box comes within rc, whose centre distance is
+// below rc, appended in ascending order to the lane's buffer (each slot is
+// written and kept only if the test passes: no branch per bead); the buffer
+// is flushed through the full tests whenever another group might not fit.
+template <bool PERIODIC, class Flush>
+__device__ __forceinline__ void scan_tile(const float4 *sx, const float4 *glo, const float4 *ghi, int m, float3 ri,
+                                          float box, float rc, float rc2, uint8_t *mine, int &nc, const Flush &flush) {
+    for (int g = 0; g * LIST_GROUP < m; g++) {
+        if (!PERIODIC && !tile_within(ri, glo[g], ghi[g], rc)) continue;
+        const int g0 = g * LIST_GROUP;
+        auto test = [&](int jj) {
+            float3 d = sub3(ri, f3(sx[jj]));
+            if (PERIODIC) d = min_image(d, box);
+            mine[nc * LIST_TILE] = (uint8_t)jj;
+            nc += (d.x * d.x + d.y * d.y + d.z * d.z < rc2) ? 1 : 0;
+        };
+        if (g0 + LIST_GROUP <= m) {
+#pragma unroll
+            for (int u = 0; u < LIST_GROUP; u++) test(g0 + u);
+        } else {
+            for (int jj = g0; jj < m; jj++) test(jj);
+        }
+        if (nc > LIST_CAND - LIST_GROUP) flush();
+    }
+}
 
 __global__ void __launch_bounds__(LIST_TILE)
-    k_rebuild(mte_params P, DevTopo T, DevState S, const int2 *tiles, int do_lj, int do_bonded) {
+    k_rebuild(mte_params P, DevTopo T, DevState S, const int2 *tiles, const int *order, int do_lj, int do_bonded) {
     __shared__ float4 sx[LIST_TILE], sn[LIST_TILE], s1[LIST_TILE], s2[LIST_TILE];
     __shared__ int smt[LIST_TILE];
     __shared__ float4 glo[LIST_NGROUPS], ghi[LIST_NGROUPS];
@@ -40,8 +125,9 @@ __global__ void __launch_bounds__(LIST_TILE)
     const int2 tl = tiles[blockIdx.x];
     const int base = T.tbase[tl.x], n = T.tsize[tl.x];
     const float box = T.tbox[tl.x];
-    const int li = tl.y + threadIdx.x;
-    const bool active = li < n;
+    const int pos = tl.y + threadIdx.x;
+    const bool active = pos < n;
+    const int li = active ? order[base + pos] : 0;
     const int b = base + li;
     RebuildOwn o{};
     bool skip = true;
@@ -85,18 +171,9 @@ __global__ void __launch_bounds__(LIST_TILE)
                 }
                 nc = 0;
             };
-            for (int g = 0; g * LIST_GROUP < m; g++) {
-                // periodic trajectories test every group (the boxes are not minimum-imaged)
-                if (box == 0.f && !tile_within(o.ri, glo[g], ghi[g], rc)) continue;
-                const int g1 = min(m, (g + 1) * LIST_GROUP);
-                for (int jj = g * LIST_GROUP; jj < g1; jj++) {
-                    const float3 d = min_image(sub3(o.ri, f3(sx[jj])), box);
-                    if (d.x * d.x + d.y * d.y + d.z * d.z < rc2) {
-                        mine[nc * LIST_TILE] = (uint8_t)jj;
-                        if (++nc == LIST_CAND) flush();
-                    }
-                }
-            }
+            // periodic trajectories test every group (the boxes are not minimum-imaged)
+            if (box > 0.f) scan_tile<true>(sx, glo, ghi, m, o.ri, box, rc, rc2, mine, nc, flush);
+            else scan_tile<false>(sx, glo, ghi, m, o.ri, box, rc, rc2, mine, nc, flush);
             flush();
         }
         __syncthreads();
@@ -110,7 +187,17 @@ hipError_t launch_rebuild(mte_engine *e) {
     const int do_bonded = e->P.is_assembly ? 1 : 0;
     if (!do_lj && !do_bonded) return hipSuccess;
     hipLaunchKernelGGL(k_rebuild, dim3(e->ntiles), dim3(LIST_TILE), 0, e->stream, e->P, e->T, e->S[e->cur],
-                       e->tiles, do_lj, do_bonded);
+                       e->tiles, e->rb_order, do_lj, do_bonded);
+    return hipGetLastError();
+}
+
+// Spatial rebuild order from the current state.  Run when a state is set, not
+// per rebuild: beads move little between uploads, and a stale order costs
+// only skips, never entries.
+hipError_t launch_rebuild_order(mte_engine *e) {
+    const float side = rebuild_radius(e->P.ljpairscutoff, true, true);
+    hipLaunchKernelGGL(k_rebuild_order, dim3(e->ntraj), dim3(ORDER_THREADS), 0, e->stream, e->T, e->S[e->cur], side,
+                       e->rb_order);
     return hipGetLastError();
 }
 

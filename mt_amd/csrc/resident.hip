@@ -38,36 +38,33 @@ __device__ __forceinline__ int pack_counts(int h, int nlon, int nlat, int nlj) {
     return (h ? 1 : 0) | (nlon << 1) | (nlat << 5) | (nlj << 10);
 }
 
+// Bead data in LDS by local index.
 struct LdsAcc {
     const float4 *x, *n, *p1, *p2;
-    int base;
-    __device__ __forceinline__ float4 xyz(int j) const { return x[j - base]; }
-    __device__ __forceinline__ float4 ang(int j) const {
-        return make_float4(n[j - base].w, p1[j - base].w, p2[j - base].w, 0.f);
-    }
-    __device__ __forceinline__ float4 fn(int j) const { return n[j - base]; }
-    __device__ __forceinline__ float4 fp1(int j) const { return p1[j - base]; }
-    __device__ __forceinline__ float4 fp2(int j) const { return p2[j - base]; }
-    __device__ __forceinline__ int flags(int j) const { return __float_as_int(x[j - base].w); }
+    __device__ __forceinline__ float4 xyz(int j) const { return x[j]; }
+    __device__ __forceinline__ float4 ang(int j) const { return make_float4(n[j].w, p1[j].w, p2[j].w, 0.f); }
+    __device__ __forceinline__ float4 fn(int j) const { return n[j]; }
+    __device__ __forceinline__ float4 fp1(int j) const { return p1[j]; }
+    __device__ __forceinline__ float4 fp2(int j) const { return p2[j]; }
+    __device__ __forceinline__ int flags(int j) const { return __float_as_int(x[j].w); }
     __device__ __forceinline__ int mon(int j) const { return flags(j) & FL_MON; }
     __device__ __forceinline__ bool gtp1(int j) const { return (flags(j) & FL_GTP1) != 0; }
     __device__ __forceinline__ bool offtub(int j) const { return (flags(j) & FL_OFFTUB) != 0; }
 };
 
-// Lists compacted in LDS; entries carry local indices, returned as packed
-// global entries so bead_force sees the same encoding as GlobalLists.
+// Lists compacted in LDS: packed entries with local indices, the encoding
+// bead_force reads through LdsAcc.
 struct LdsLists {
     const uint16_t *e;
-    int off, cnt, base;
-    __device__ __forceinline__ int conv(int v) const { return ((base + (v >> 1)) << 1) | (v & 1); }
+    int off, cnt;
     __device__ __forceinline__ int h() const { return cnt & 1; }
-    __device__ __forceinline__ int harm() const { return h() ? conv(e[off]) : -1; }
+    __device__ __forceinline__ int harm() const { return h() ? (int)e[off] : -1; }
     __device__ __forceinline__ int nlon() const { return (cnt >> 1) & 15; }
-    __device__ __forceinline__ int lon(int k) const { return conv(e[off + h() + k]); }
+    __device__ __forceinline__ int lon(int k) const { return e[off + h() + k]; }
     __device__ __forceinline__ int nlat() const { return (cnt >> 5) & 31; }
-    __device__ __forceinline__ int lat(int k) const { return conv(e[off + h() + nlon() + k]); }
+    __device__ __forceinline__ int lat(int k) const { return e[off + h() + nlon() + k]; }
     __device__ __forceinline__ int nlj() const { return (cnt >> 10) & 1023; }
-    __device__ __forceinline__ int lj(int k) const { return base + e[off + h() + nlon() + nlat() + k]; }
+    __device__ __forceinline__ int lj(int k) const { return e[off + h() + nlon() + nlat() + k]; }
 };
 
 // Writes a bead's new state to LDS; returns (sin theta, cos theta, sin psi,
@@ -92,12 +89,13 @@ __device__ __forceinline__ float4 store_state(float4 *lx, float4 *ln, float4 *l1
 }
 
 template <int PT, class Lists, class MakeLists>
-__device__ __forceinline__ void run_steps(const mte_params &P, const DevTopo &T, const LdsAcc &A, float4 *lx,
+__device__ __forceinline__ void run_steps(const mte_params &P, const DevTopo &T, const LdsAcc &A, int t, float4 *lx,
                                           float4 *ln, float4 *l1, float4 *l2, const bool mob[PT], uint4 s0[PT],
                                           uint4 s1[PT], float4 trig[PT], const float *noise, int nsteps, const LatOffsets &L,
                                           const MakeLists &make_lists) {
-    const int base = A.base, nb = T.nb;
-    const float box = T.tbox[T.tid[base]];
+    const int base = T.tbase[t], nb = T.nb;
+    const float box = T.tbox[t];
+    const float wall_z = P.is_wall ? T.zs[t] : 0.f;
     for (int s = 0; s < nsteps; s++) {
         float4 nx[PT], na[PT];
 #pragma unroll
@@ -111,7 +109,7 @@ __device__ __forceinline__ void run_steps(const mte_params &P, const DevTopo &T,
             I.box = box;
             float F[6];
             const Lists Ls = make_lists(i, b);
-            bead_force(P, T, A, Ls, b, I, F);
+            bead_force(P, A, Ls, i, wall_z, I, F);
             float4 rx, ra;
             if (noise) {
                 const float *nz = noise + (size_t)s * 6 * nb + 6 * (size_t)b;
@@ -183,7 +181,7 @@ __global__ void __launch_bounds__(MAXT)
         }
     }
     __syncthreads();
-    const LdsAcc A{lx, ln, l1, l2, base};
+    const LdsAcc A{lx, ln, l1, l2};
     // Compact this trajectory's lists into LDS: totals -> block scan -> copy.
     int mytot[PT];
 #pragma unroll
@@ -249,11 +247,11 @@ __global__ void __launch_bounds__(MAXT)
             for (int q = 0; q < ((hc >> 10) & 1023); q++) ent[o++] = (uint16_t)(T.lj[q * nb + b] - base);
         }
         __syncthreads();
-        auto mk = [&](int i, int) { return LdsLists{ent, hoff[i], hcnt[i], base}; };
-        run_steps<PT, LdsLists>(P, T, A, lx, ln, l1, l2, mob, s0, s1, trig, noise, nsteps, L, mk);
+        auto mk = [&](int i, int) { return LdsLists{ent, hoff[i], hcnt[i]}; };
+        run_steps<PT, LdsLists>(P, T, A, t, lx, ln, l1, l2, mob, s0, s1, trig, noise, nsteps, L, mk);
     } else {
-        auto mk = [&](int, int b) { return GlobalLists{T, b}; };
-        run_steps<PT, GlobalLists>(P, T, A, lx, ln, l1, l2, mob, s0, s1, trig, noise, nsteps, L, mk);
+        auto mk = [&](int, int b) { return GlobalLists{T, b, base}; };
+        run_steps<PT, GlobalLists>(P, T, A, t, lx, ln, l1, l2, mob, s0, s1, trig, noise, nsteps, L, mk);
     }
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int b = base + i;
