@@ -206,15 +206,22 @@ __device__ __forceinline__ float u01(uint32_t u) {
     float f = __uint_as_float(0x3f800000u | (0x007fffffu & u)) - 1.0f;
     return f == 0.f ? 1.0e-8f : f;
 }
+// Box-Muller radius sqrt(-2 ln u) = sqrt(-2 ln2 log2 u) from the hardware
+// log2 and square root (1 ulp each).  u lies in [2^-23, 1) or is 1e-8, so
+// both arguments are normal floats and need none of the denormal scaling and
+// rounding fix-ups that logf and sqrtf expand to.
+__device__ __forceinline__ float bm_radius(float u) {
+    return __builtin_amdgcn_sqrtf(-1.38629436111989061883f * __builtin_amdgcn_logf(u));
+}
 // One draw of four normals (two Box-Muller pairs), advancing the state four
 // times as src/HybridTaus.cu:85-98 does.
 __device__ __forceinline__ float4 normals4(uint4 &s) {
     float4 o;
-    float r = sqrtf(-2.0f * __logf(u01(hybrid_taus(s))));
+    float r = bm_radius(u01(hybrid_taus(s)));
     float th = 6.28318530717958647692f * u01(hybrid_taus(s));
     o.x = r * __sinf(th);
     o.y = r * __cosf(th);
-    r = sqrtf(-2.0f * __logf(u01(hybrid_taus(s))));
+    r = bm_radius(u01(hybrid_taus(s)));
     th = 6.28318530717958647692f * u01(hybrid_taus(s));
     o.z = r * __sinf(th);
     o.w = r * __cosf(th);
