@@ -26,6 +26,12 @@ constexpr int RES_MAX_PT = 3;
 constexpr int STATE_ARRAYS = 4;         // xyz|flags, n|fi, p1|theta, p2|psi
 constexpr int HDR_INTS = 2;             // per bead: list offset, packed counts
 constexpr int SCAN_INTS = RES_MAX_THREADS;
+constexpr int FLAG_INTS = 2;            // per-step "a bead left its skin" flags, alternating by step parity
+// LJ entries copied to LDS are those within LJ_CUT + 2 SKIN_HALF (+ margin)
+// of the bead at the launch's first step; while every bead stays within
+// SKIN_HALF of where it was then, no dropped pair can come below LJ_CUT.
+constexpr float SKIN_HALF = 0.75f;
+constexpr float FILTER_R = LJ_CUT + 2.f * SKIN_HALF + 0.01f;
 // Minimum room for list entries per bead when sizing the plan.
 constexpr int MIN_ENTRIES_PER_BEAD = 24;
 
@@ -88,16 +94,27 @@ __device__ __forceinline__ float4 store_state(float4 *lx, float4 *ln, float4 *l1
     return make_float4(R.st, R.ct, R.sp, R.cp);
 }
 
-template <int PT, class Lists, class MakeLists>
+// The step loop.  With `filtered` the LJ lists in LDS hold only the pairs
+// within FILTER_R at the first step: each mobile bead's displacement from
+// its first-step position is checked every step, and once any bead of the
+// trajectory has moved SKIN_HALF or more, `refill` restores the full lists
+// before the next force evaluation.  Dropped pairs contribute exactly zero
+// force until then, so the sums are those over the full lists.
+template <int PT, class Lists, class MakeLists, class Refill>
 __device__ __forceinline__ void run_steps(const mte_params &P, const DevTopo &T, const LdsAcc &A, int t, float4 *lx,
                                           float4 *ln, float4 *l1, float4 *l2, const bool mob[PT], uint4 s0[PT],
                                           uint4 s1[PT], float4 trig[PT], const float *noise, int nsteps, const LatOffsets &L,
-                                          const MakeLists &make_lists) {
+                                          const MakeLists &make_lists, bool filtered, int *flags,
+                                          const Refill &refill) {
     const int base = T.tbase[t], nb = T.nb;
     const float box = T.tbox[t];
     const float wall_z = P.is_wall ? T.zs[t] : 0.f;
+    float3 xr[PT];
+#pragma unroll
+    for (int k = 0; k < PT; k++) xr[k] = mob[k] ? f3(lx[threadIdx.x + k * blockDim.x]) : make_float3(0.f, 0.f, 0.f);
     for (int s = 0; s < nsteps; s++) {
         float4 nx[PT], na[PT];
+        bool moved = false;
 #pragma unroll
         for (int k = 0; k < PT; k++) {
             if (!mob[k]) continue;
@@ -122,14 +139,25 @@ __device__ __forceinline__ void run_steps(const mte_params &P, const DevTopo &T,
             em_update(P, F, rx, ra, x, a);
             nx[k] = x;   // w keeps the packed flags
             na[k] = a;
+            if (filtered) {
+                const float dx = x.x - xr[k].x, dy = x.y - xr[k].y, dz = x.z - xr[k].z;
+                moved |= dx * dx + dy * dy + dz * dz >= SKIN_HALF * SKIN_HALF;
+            }
         }
+        if (filtered && moved) flags[s & 1] = 1;
         __syncthreads();
+        const bool left = filtered && flags[s & 1] != 0;
+        if (filtered && threadIdx.x == 0) flags[(s + 1) & 1] = 0;   // read at the next step, after its barrier
 #pragma unroll
         for (int k = 0; k < PT; k++) {
             if (!mob[k]) continue;
             trig[k] = store_state(lx, ln, l1, l2, threadIdx.x + k * blockDim.x, nx[k], na[k], L);
         }
         __syncthreads();
+        if (left) {   // uniform: every thread read the same flag
+            refill();
+            filtered = false;
+        }
     }
 }
 
@@ -142,7 +170,8 @@ __global__ void __launch_bounds__(MAXT)
     int *hoff = reinterpret_cast<int *>(lds + STATE_ARRAYS * stride);
     int *hcnt = hoff + stride;
     int *scan = hcnt + stride;
-    uint16_t *ent = reinterpret_cast<uint16_t *>(scan + SCAN_INTS);
+    int *flags = scan + SCAN_INTS;
+    uint16_t *ent = reinterpret_cast<uint16_t *>(flags + FLAG_INTS);
     const int t = blockIdx.x;
     const int base = T.tbase[t], n = T.tsize[t];
     const int nb = T.nb;
@@ -182,76 +211,112 @@ __global__ void __launch_bounds__(MAXT)
     }
     __syncthreads();
     const LdsAcc A{lx, ln, l1, l2};
-    // Compact this trajectory's lists into LDS: totals -> block scan -> copy.
-    int mytot[PT];
-#pragma unroll
-    for (int k = 0; k < PT; k++) {
-        const int i = threadIdx.x + k * blockDim.x;
-        mytot[k] = 0;
-        if (i >= n) continue;
-        const int b = base + i;
-        const int h = T.harm[b] >= 0 ? 1 : 0;
-        const int nlon = T.lon_n[b], nlat = T.lat_n[b], nlj = P.lj_on ? T.lj_n[b] : 0;
-        hcnt[i] = pack_counts(h, nlon, nlat, nlj);
-        mytot[k] = h + nlon + nlat + nlj;
-    }
-    // Chunked exclusive scan: thread t owns beads [t*c, (t+1)*c).
-    const int c = (n + blockDim.x - 1) / blockDim.x;
-    // totals go through hoff, indexed by bead
-#pragma unroll
-    for (int k = 0; k < PT; k++) {
-        const int i = threadIdx.x + k * blockDim.x;
-        if (i < n) hoff[i] = mytot[k];
-    }
-    __syncthreads();
-    int run = 0;
-    const int lo = threadIdx.x * c, hi = min(n, lo + c);
-    for (int i = lo; i < hi; i++) run += hoff[i];
-    scan[threadIdx.x] = run;
-    __syncthreads();
-    for (int d = 1; d < (int)blockDim.x; d <<= 1) {
-        const int v = threadIdx.x >= (unsigned)d ? scan[threadIdx.x - d] : 0;
-        __syncthreads();
-        scan[threadIdx.x] += v;
-        __syncthreads();
-    }
-    const int total = scan[blockDim.x - 1];
-    int acc = scan[threadIdx.x] - run;   // exclusive prefix of this chunk
-    for (int i = lo; i < hi; i++) {
-        const int v = hoff[i];
-        hoff[i] = acc;
-        acc += v;
-    }
-    __syncthreads();
-    const bool use_lds = total <= entry_cap;
-    if (use_lds) {
+    const float box = T.tbox[t];
+    // LJ entries of bead i kept in LDS: all of them (rf2 == 0), or those
+    // within sqrt(rf2) of it now.
+    auto lj_keep = [&](int i, int j, float rf2) {
+        if (rf2 == 0.f) return true;
+        const float3 d = min_image(sub3(f3(lx[i]), f3(lx[j])), box);
+        return d.x * d.x + d.y * d.y + d.z * d.z < rf2;
+    };
+    // Compact this trajectory's lists into LDS: counts -> block scan of the
+    // per-bead totals into offsets -> copy (when `copy` and they fit).
+    // Returns the number of entries.
+    auto compact = [&](float rf2, bool copy) {
+        int mytot[PT];
 #pragma unroll
         for (int k = 0; k < PT; k++) {
             const int i = threadIdx.x + k * blockDim.x;
+            mytot[k] = 0;
             if (i >= n) continue;
             const int b = base + i;
-            int o = hoff[i];
-            const int hc = hcnt[i];
-            if (hc & 1) {
-                const int e = T.harm[b];
-                ent[o++] = (uint16_t)((((e >> 1) - base) << 1) | (e & 1));
+            const int h = T.harm[b] >= 0 ? 1 : 0;
+            const int nlon = T.lon_n[b], nlat = T.lat_n[b];
+            int nlj = 0;
+            if (P.lj_on) {
+                const int m = T.lj_n[b];
+                if (rf2 == 0.f) nlj = m;
+                else
+                    for (int q = 0; q < m; q++) nlj += lj_keep(i, T.lj[q * nb + b] - base, rf2) ? 1 : 0;
             }
-            for (int q = 0; q < ((hc >> 1) & 15); q++) {
-                const int e = T.lon[q * nb + b];
-                ent[o++] = (uint16_t)((((e >> 1) - base) << 1) | (e & 1));
-            }
-            for (int q = 0; q < ((hc >> 5) & 31); q++) {
-                const int e = T.lat[q * nb + b];
-                ent[o++] = (uint16_t)((((e >> 1) - base) << 1) | (e & 1));
-            }
-            for (int q = 0; q < ((hc >> 10) & 1023); q++) ent[o++] = (uint16_t)(T.lj[q * nb + b] - base);
+            hcnt[i] = pack_counts(h, nlon, nlat, nlj);
+            mytot[k] = h + nlon + nlat + nlj;
+        }
+        // Chunked exclusive scan: thread t owns beads [t*c, (t+1)*c); the
+        // totals go through hoff, indexed by bead.
+        const int c = (n + blockDim.x - 1) / blockDim.x;
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            const int i = threadIdx.x + k * blockDim.x;
+            if (i < n) hoff[i] = mytot[k];
         }
         __syncthreads();
+        int run = 0;
+        const int lo = threadIdx.x * c, hi = min(n, lo + c);
+        for (int i = lo; i < hi; i++) run += hoff[i];
+        scan[threadIdx.x] = run;
+        __syncthreads();
+        for (int d = 1; d < (int)blockDim.x; d <<= 1) {
+            const int v = threadIdx.x >= (unsigned)d ? scan[threadIdx.x - d] : 0;
+            __syncthreads();
+            scan[threadIdx.x] += v;
+            __syncthreads();
+        }
+        const int total = scan[blockDim.x - 1];
+        int acc = scan[threadIdx.x] - run;   // exclusive prefix of this chunk
+        for (int i = lo; i < hi; i++) {
+            const int v = hoff[i];
+            hoff[i] = acc;
+            acc += v;
+        }
+        __syncthreads();
+        if (copy && total <= entry_cap) {
+#pragma unroll
+            for (int k = 0; k < PT; k++) {
+                const int i = threadIdx.x + k * blockDim.x;
+                if (i >= n) continue;
+                const int b = base + i;
+                int o = hoff[i];
+                const int hc = hcnt[i];
+                if (hc & 1) {
+                    const int e = T.harm[b];
+                    ent[o++] = (uint16_t)((((e >> 1) - base) << 1) | (e & 1));
+                }
+                for (int q = 0; q < ((hc >> 1) & 15); q++) {
+                    const int e = T.lon[q * nb + b];
+                    ent[o++] = (uint16_t)((((e >> 1) - base) << 1) | (e & 1));
+                }
+                for (int q = 0; q < ((hc >> 5) & 31); q++) {
+                    const int e = T.lat[q * nb + b];
+                    ent[o++] = (uint16_t)((((e >> 1) - base) << 1) | (e & 1));
+                }
+                if (P.lj_on)
+                    for (int q = 0, m = T.lj_n[b]; q < m; q++) {
+                        const int j = T.lj[q * nb + b] - base;
+                        if (lj_keep(i, j, rf2)) ent[o++] = (uint16_t)j;
+                    }
+            }
+        }
+        if (threadIdx.x == 0) flags[0] = flags[1] = 0;
+        __syncthreads();
+        return total;
+    };
+    // The full lists decide between LDS and global lists; in LDS, the LJ
+    // lists are filtered by distance when the list cutoff reaches beyond
+    // FILTER_R (the full lists fit, so a refill always fits too).
+    const bool use_lds = compact(0.f, false) <= entry_cap;
+    if (use_lds) {
+        const bool filter = P.lj_on && P.ljpairscutoff > FILTER_R;
+        compact(filter ? FILTER_R * FILTER_R : 0.f, true);
         auto mk = [&](int i, int) { return LdsLists{ent, hoff[i], hcnt[i]}; };
-        run_steps<PT, LdsLists>(P, T, A, t, lx, ln, l1, l2, mob, s0, s1, trig, noise, nsteps, L, mk);
+        auto refill = [&]() { compact(0.f, true); };
+        run_steps<PT, LdsLists>(P, T, A, t, lx, ln, l1, l2, mob, s0, s1, trig, noise, nsteps, L, mk, filter, flags,
+                                refill);
     } else {
         auto mk = [&](int, int b) { return GlobalLists{T, b, base}; };
-        run_steps<PT, GlobalLists>(P, T, A, t, lx, ln, l1, l2, mob, s0, s1, trig, noise, nsteps, L, mk);
+        auto refill = [&]() {};
+        run_steps<PT, GlobalLists>(P, T, A, t, lx, ln, l1, l2, mob, s0, s1, trig, noise, nsteps, L, mk, false, flags,
+                                   refill);
     }
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int b = base + i;
@@ -276,7 +341,7 @@ __global__ void __launch_bounds__(MAXT)
 
 static size_t fixed_lds_bytes(int nmax) {
     return (size_t)STATE_ARRAYS * sizeof(float4) * nmax + (size_t)HDR_INTS * sizeof(int) * nmax +
-           (size_t)SCAN_INTS * sizeof(int);
+           (size_t)(SCAN_INTS + FLAG_INTS) * sizeof(int);
 }
 
 // Launch plan with pt beads per thread (pt = 1 allows up to 1024 threads,
