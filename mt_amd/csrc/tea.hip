@@ -184,23 +184,33 @@ __global__ void __launch_bounds__(TEA_TILE)
                     cz = bt * rsq_fast(1.f + w * cc.z);
         const float a = P.tea_a, ia = rcp_fast(P.tea_a);
         const float box = T.tbox[u.x];
-        const int self = li - u.z;   // position of i in this chunk, if any
         for (int q = 0; q < jn; q++) {
             const float4 xj = sx[q], Fj = sf[q], Rj = sr[q];
             const float3 d = min_image(sub3(f3(xj), ri), box);
             const float dx = d.x, dy = d.y, dz = d.z;
-            const bool is_self = q == self;
-            // the j == i slot and coincident beads stay finite (extra beads carry zero F and R)
-            const float r2 = is_self ? 1.f : fmaxf(dx * dx + dy * dy + dz * dz, 1e-12f);
+            // the j == i slot (d = 0) and coincident beads stay finite through the clamp
+            // (extra beads carry zero F and R)
+            const float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-12f);
             const float rinv = rsq_fast(r2);
             // D_hat = cii I + crr e e^T (src/bdhitea_kernel.cu:38-58)
             const RpyCoef c = rpy_select(r2, rinv, a, ia);
-            const float cii = is_self ? 0.f : c.cii, crr = is_self ? 0.f : c.crr;
             const float vx = fmaf(cx, Rj.x, Fj.x), vy = fmaf(cy, Rj.y, Fj.y), vz = fmaf(cz, Rj.z, Fj.z);
-            const float s = (crr * rinv * rinv) * (dx * vx + dy * vy + dz * vz);
-            acc.x += fmaf(cii, vx, s * dx);
-            acc.y += fmaf(cii, vy, s * dy);
-            acc.z += fmaf(cii, vz, s * dz);
+            const float s = (c.crr * rinv * rinv) * (dx * vx + dy * vy + dz * vz);
+            acc.x = fmaf(s, dx, fmaf(c.cii, vx, acc.x));
+            acc.y = fmaf(s, dy, fmaf(c.cii, vy, acc.y));
+            acc.z = fmaf(s, dz, fmaf(c.cii, vz, acc.z));
+        }
+        const int self = li - u.z;   // position of i in this chunk, if any
+        if (self >= 0 && self < jn) {
+            // The loop took the j == i slot as a coincident bead: d = 0 leaves
+            // only its cii (v_i) term, recomputed here from the same operands
+            // and taken out again.
+            const float4 Fi = sf[self], Ri = sr[self];
+            const float r2 = 1e-12f, rinv = rsq_fast(r2);
+            const float cii = rpy_select(r2, rinv, a, ia).cii;
+            acc.x = fmaf(-cii, fmaf(cx, Ri.x, Fi.x), acc.x);
+            acc.y = fmaf(-cii, fmaf(cy, Ri.y, Fi.y), acc.y);
+            acc.z = fmaf(-cii, fmaf(cz, Ri.z, Fi.z), acc.z);
         }
     }
     partial[(size_t)u.w * chunk_stride + b] = make_float4(acc.x, acc.y, acc.z, 0.f);
