@@ -186,9 +186,9 @@ __device__ int gather_candidates(const DevTopo &T, const DevState &S, const Cell
             for (int dx = -K; dx <= K; dx++) {
                 int cx = me.x + dx, cy = me.y + dy, cz = me.z + dz;
                 if (g.dims > 0) {
-                    cx = (cx + g.dims) % g.dims;
-                    cy = (cy + g.dims) % g.dims;
-                    cz = (cz + g.dims) % g.dims;
+                    cx = wrap_cell(cx, g.dims);
+                    cy = wrap_cell(cy, g.dims);
+                    cz = wrap_cell(cz, g.dims);
                 }
                 const int bucket = C.toff[t] + (int)(cell_hash(cx, cy, cz) & mask);
                 const int2 rg = C.range[bucket];

@@ -79,6 +79,10 @@ __device__ __forceinline__ float3 min_image(float3 d, float L) {
 }
 __device__ __forceinline__ float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
 
+// Periodic cell index: i in [-dims, 2 dims) wrapped into [0, dims) (a
+// neighbour stencil never reaches further), without an integer division.
+__device__ __forceinline__ int wrap_cell(int i, int dims) { return i < 0 ? i + dims : (i >= dims ? i - dims : i); }
+
 // sin and cos by Cody-Waite reduction modulo pi/2 (two-part pi/2, exact
 // with FMA for |x| < 8192) and the single-precision minimax polynomials of
 // the Cephes library (S. L. Moshier, sinf.c / cosf.c) on [-pi/4, pi/4];

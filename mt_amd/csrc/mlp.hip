@@ -67,38 +67,54 @@ __global__ void __launch_bounds__(MLP_BLOCK)
         const uint32_t mask = (uint32_t)(C.tsize[t] - 1);
         const int toff = C.toff[t];
         const float ia = 1.f / P.tea_a;
+        // own cell first, then faces, edges, corners: the nearest neighbours
+        // come early and later candidates mostly fail the 32nd-distance test
+        constexpr int ORD[27] = {13, 4, 10, 12, 14, 16, 22, 1, 3, 5, 7, 9, 11, 15, 17, 19, 21, 23, 25,
+                                 0, 2, 6, 8, 18, 20, 24, 26};
         int bk[27];
 #pragma unroll
-        for (int c = 0; c < 27; c++) {
+        for (int o = 0; o < 27; o++) {
+            const int c = ORD[o];
             int cx = me.x + c % 3 - 1, cy = me.y + (c / 3) % 3 - 1, cz = me.z + c / 9 - 1;
             if (dims > 0) {
-                cx = (cx + dims) % dims;
-                cy = (cy + dims) % dims;
-                cz = (cz + dims) % dims;
+                cx = wrap_cell(cx, dims);
+                cy = wrap_cell(cy, dims);
+                cz = wrap_cell(cz, dims);
             }
             const uint32_t h = ((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u);
-            bk[c] = toff + (int)(h & mask);
+            bk[o] = toff + (int)(h & mask);
         }
         int2 rg[27];
 #pragma unroll
-        for (int c = 0; c < 27; c++) rg[c] = C.range[bk[c]];
+        for (int o = 0; o < 27; o++) rg[o] = C.range[bk[o]];
+        // a bucket two cells hash to is visited once: a 64-bit filter on the
+        // low bucket bits, and the full comparison only when it hits
+        uint64_t seen = 1ull << (bk[0] & 63);
 #pragma unroll
-        for (int c = 1; c < 27; c++) {
-            bool dup = false;
+        for (int o = 1; o < 27; o++) {
+            const uint64_t bit = 1ull << (bk[o] & 63);
+            if (seen & bit) {
+                bool dup = false;
 #pragma unroll
-            for (int k = 0; k < c; k++) dup |= bk[k] == bk[c];
-            if (dup) rg[c].y = 0;
+                for (int k = 0; k < o; k++) dup |= bk[k] == bk[o];
+                if (dup) rg[o].y = 0;
+            }
+            seen |= bit;
         }
 #pragma unroll
         for (int c = 0; c < 27; c++) {
-            for (int q = rg[c].x; q < rg[c].x + rg[c].y; q++) {
-                const float4 pj = C.spos[q];
+            // each candidate is one 16-byte load, issued one candidate ahead
+            const int q0 = rg[c].x, q1 = q0 + rg[c].y;
+            float4 pn = q0 < q1 ? C.spos[q0] : make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int q = q0; q < q1; q++) {
+                const float4 pj = pn;
+                if (q + 1 < q1) pn = C.spos[q + 1];
                 const int jw = __float_as_int(pj.w);
                 const int j = jw & 0x7fffffff;
-                if (jw < 0 || j == b) continue;   // extra bead, or this bead
                 const float3 d = min_image(make_float3(pj.x - ri.x, pj.y - ri.y, pj.z - ri.z), box);
                 const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
-                if (r2 >= MLP_RC * MLP_RC) continue;
+                // extra beads, this bead and beads beyond the radius drop out
+                if (jw < 0 || j == b || r2 >= MLP_RC * MLP_RC) continue;
                 if (LABELS) {
                     // exact label: phi(r) D_hat(d)
                     const float r = sqrtf(r2), rinv = 1.f / r;
