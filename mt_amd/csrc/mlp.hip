@@ -106,9 +106,11 @@ __global__ void __launch_bounds__(MLP_BLOCK)
             // each candidate is one 16-byte load, issued one candidate ahead
             const int q0 = rg[c].x, q1 = q0 + rg[c].y;
             float4 pn = q0 < q1 ? C.spos[q0] : make_float4(0.f, 0.f, 0.f, 0.f);
+            float4 pm = q0 + 1 < q1 ? C.spos[q0 + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
             for (int q = q0; q < q1; q++) {
                 const float4 pj = pn;
-                if (q + 1 < q1) pn = C.spos[q + 1];
+                pn = pm;
+                if (q + 2 < q1) pm = C.spos[q + 2];
                 const int jw = __float_as_int(pj.w);
                 const int j = jw & 0x7fffffff;
                 const float3 d = min_image(make_float3(pj.x - ri.x, pj.y - ri.y, pj.z - ri.z), box);
