@@ -165,7 +165,7 @@ def tea_bench(steps, warmup, cpu=True):
                         "frac": flops / avg / 1e12 / 157.3, "traffic": read_tea_traffic(tuned["tea_chunk"]),
                         "kernel": "k_tea_pairs", "avg_launch_ms": avg * 1e3, "alg_flops_per_launch": flops,
                         "note": "algorithmic work 36 B(B-1) flops per trajectory-step; the kernel evaluates it "
-                                "on VALU (FP32, ~48 instructions per block, at the non-packed issue rate); the "
+                                "on VALU (FP32, ~41 instructions per block, at the non-packed issue rate); the "
                                 "peak quoted is the FP32 figure, which counts packed FMAs"},
            "force_error_vs_fp64": tea_error_vs_fp64()}
     if cpu:

@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(MAXT)
     const float box = T.tbox[t];
     // LJ entries of bead i kept in LDS: all of them (rf2 == 0), or those
     // within sqrt(rf2) of it now.
-    auto lj_keep = [&](int i, int j, float rf2) {
+    auto lj_keep = [lx, box](int i, int j, float rf2) {
         if (rf2 == 0.f) return true;
         const float3 d = min_image(sub3(f3(lx[i]), f3(lx[j])), box);
         return d.x * d.x + d.y * d.y + d.z * d.z < rf2;
