@@ -414,6 +414,13 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
         }
         e->mlp_last_err.assign(nt, 0.0);
     }
+    {   // list rebuilds with one workgroup per trajectory when the largest one fits in LDS
+        int nmax = 0, lds_limit = 0;
+        for (int t = 0; t < nt; t++) nmax = std::max(nmax, tsize[t]);
+        if (hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device) != hipSuccess ||
+            !rebuild_traj_plan(nmax, lds_limit, &e->rb_traj_threads, &e->rb_traj_lds))
+            e->rb_traj_threads = 0;
+    }
     ResidentPlan plan;
     const bool fits = resident_fits(e, &plan);
     if (opt->mode == MTE_MODE_RESIDENT && !fits) {

@@ -33,7 +33,10 @@ namespace mte {
 constexpr int LIST_TILE = 256;
 constexpr int LIST_GROUP = 8;                        // beads per bounding box
 constexpr int LIST_NGROUPS = LIST_TILE / LIST_GROUP;
+constexpr int LIST_SUPER = 64;                       // beads per super-group box
 constexpr int LIST_CAND = 32;                        // candidate slots per lane
+constexpr int TRAJ_THREADS = 1024;                   // k_rebuild_traj workgroup (at most)
+constexpr int TRAJ_CAND = 24;                        // its candidate slots per lane
 constexpr int ORDER_MAX = 2048;                      // larger trajectories keep the index order
 constexpr int ORDER_THREADS = 1024;
 
@@ -89,30 +92,70 @@ __global__ void __launch_bounds__(ORDER_THREADS) k_rebuild_order(DevTopo T, DevS
     for (int p = threadIdx.x; p < n; p += blockDim.x) order[base + p] = (int)(key[p] & (ORDER_MAX - 1));
 }
 
-// Candidates of the bead at ri among the m beads of the j tile in LDS: the
-// beads of every group whose box comes within rc, whose centre distance is
-// below rc, appended in ascending order to the lane's buffer (each slot is
-// written and kept only if the test passes: no branch per bead); the buffer
-// is flushed through the full tests whenever another group might not fit.
-template <bool PERIODIC, class Flush>
-__device__ __forceinline__ void scan_tile(const float4 *sx, const float4 *glo, const float4 *ghi, int m, float3 ri,
-                                          float box, float rc, float rc2, uint8_t *mine, int &nc, const Flush &flush) {
-    for (int g = 0; g * LIST_GROUP < m; g++) {
-        if (!PERIODIC && !tile_within(ri, glo[g], ghi[g], rc)) continue;
-        const int g0 = g * LIST_GROUP;
-        auto test = [&](int jj) {
-            float3 d = sub3(ri, f3(sx[jj]));
-            if (PERIODIC) d = min_image(d, box);
-            mine[nc * LIST_TILE] = (uint8_t)jj;
-            nc += (d.x * d.x + d.y * d.y + d.z * d.z < rc2) ? 1 : 0;
-        };
-        if (g0 + LIST_GROUP <= m) {
-#pragma unroll
-            for (int u = 0; u < LIST_GROUP; u++) test(g0 + u);
-        } else {
-            for (int jj = g0; jj < m; jj++) test(jj);
+// Bounding boxes of the m beads in LDS: groups of LIST_GROUP consecutive
+// beads, and super-groups of LIST_SUPER (the union of their groups' boxes).
+// Ends with a barrier.
+__device__ __forceinline__ void group_boxes(const float4 *sx, int m, float4 *glo, float4 *ghi, float4 *slo,
+                                            float4 *shi) {
+    const int ng = (m + LIST_GROUP - 1) / LIST_GROUP, nsg = (m + LIST_SUPER - 1) / LIST_SUPER;
+    for (int g = threadIdx.x; g < ng; g += blockDim.x) {
+        const int g0 = g * LIST_GROUP, g1 = min(m, g0 + LIST_GROUP);
+        float3 lo = f3(sx[g0]), hi = lo;
+        for (int q = g0 + 1; q < g1; q++) {
+            const float3 x = f3(sx[q]);
+            lo = make_float3(fminf(lo.x, x.x), fminf(lo.y, x.y), fminf(lo.z, x.z));
+            hi = make_float3(fmaxf(hi.x, x.x), fmaxf(hi.y, x.y), fmaxf(hi.z, x.z));
         }
-        if (nc > LIST_CAND - LIST_GROUP) flush();
+        glo[g] = make_float4(lo.x, lo.y, lo.z, 0.f);
+        ghi[g] = make_float4(hi.x, hi.y, hi.z, 0.f);
+    }
+    __syncthreads();
+    constexpr int GPS = LIST_SUPER / LIST_GROUP;
+    for (int sg = threadIdx.x; sg < nsg; sg += blockDim.x) {
+        float4 lo = glo[sg * GPS], hi = ghi[sg * GPS];
+        for (int g = sg * GPS + 1; g < min(ng, (sg + 1) * GPS); g++) {
+            const float4 l = glo[g], h = ghi[g];
+            lo = make_float4(fminf(lo.x, l.x), fminf(lo.y, l.y), fminf(lo.z, l.z), 0.f);
+            hi = make_float4(fmaxf(hi.x, h.x), fmaxf(hi.y, h.y), fmaxf(hi.z, h.z), 0.f);
+        }
+        slo[sg] = lo;
+        shi[sg] = hi;
+    }
+    __syncthreads();
+}
+
+// Candidates of the bead at ri among the m beads in LDS: the beads of every
+// group (inside a super-group) whose box comes within rc and whose centre
+// distance is below rc, appended in ascending order to the lane's buffer
+// mine[slot * stride] (each slot is written and kept only if the test
+// passes: no branch per bead); the buffer is flushed through the full tests
+// whenever another group might not fit in its `cap` slots.
+template <bool PERIODIC, class CandT, class Flush>
+__device__ __forceinline__ void scan_groups(const float4 *sx, const float4 *glo, const float4 *ghi, const float4 *slo,
+                                            const float4 *shi, int m, float3 ri, float box, float rc, float rc2,
+                                            CandT *mine, int stride, int cap, int &nc, const Flush &flush) {
+    constexpr int GPS = LIST_SUPER / LIST_GROUP;
+    const int ng = (m + LIST_GROUP - 1) / LIST_GROUP;
+    for (int sg = 0; sg * LIST_SUPER < m; sg++) {
+        if (!PERIODIC && !tile_within(ri, slo[sg], shi[sg], rc)) continue;
+        const int gend = min(ng, (sg + 1) * GPS);
+        for (int g = sg * GPS; g < gend; g++) {
+            if (!PERIODIC && !tile_within(ri, glo[g], ghi[g], rc)) continue;
+            const int g0 = g * LIST_GROUP;
+            auto test = [&](int jj) {
+                float3 d = sub3(ri, f3(sx[jj]));
+                if (PERIODIC) d = min_image(d, box);
+                mine[nc * stride] = (CandT)jj;
+                nc += (d.x * d.x + d.y * d.y + d.z * d.z < rc2) ? 1 : 0;
+            };
+            if (g0 + LIST_GROUP <= m) {
+#pragma unroll
+                for (int u = 0; u < LIST_GROUP; u++) test(g0 + u);
+            } else {
+                for (int jj = g0; jj < m; jj++) test(jj);
+            }
+            if (nc > cap - LIST_GROUP) flush();
+        }
     }
 }
 
@@ -120,7 +163,7 @@ __global__ void __launch_bounds__(LIST_TILE)
     k_rebuild(mte_params P, DevTopo T, DevState S, const int2 *tiles, const int *order, int do_lj, int do_bonded) {
     __shared__ float4 sx[LIST_TILE], sn[LIST_TILE], s1[LIST_TILE], s2[LIST_TILE];
     __shared__ int smt[LIST_TILE];
-    __shared__ float4 glo[LIST_NGROUPS], ghi[LIST_NGROUPS];
+    __shared__ float4 glo[LIST_NGROUPS], ghi[LIST_NGROUPS], slo[LIST_TILE / LIST_SUPER], shi[LIST_TILE / LIST_SUPER];
     __shared__ uint8_t cand[LIST_CAND * LIST_TILE];   // [slot][lane]: positions within the j tile
     const int2 tl = tiles[blockIdx.x];
     const int base = T.tbase[tl.x], n = T.tsize[tl.x];
@@ -149,18 +192,7 @@ __global__ void __launch_bounds__(LIST_TILE)
         }
         __syncthreads();
         const int m = min(LIST_TILE, n - j0);
-        if (threadIdx.x < LIST_NGROUPS && threadIdx.x * LIST_GROUP < m) {
-            const int g0 = threadIdx.x * LIST_GROUP, g1 = min(m, g0 + LIST_GROUP);
-            float3 lo = f3(sx[g0]), hi = lo;
-            for (int q = g0 + 1; q < g1; q++) {
-                const float3 x = f3(sx[q]);
-                lo = make_float3(fminf(lo.x, x.x), fminf(lo.y, x.y), fminf(lo.z, x.z));
-                hi = make_float3(fmaxf(hi.x, x.x), fmaxf(hi.y, x.y), fmaxf(hi.z, x.z));
-            }
-            glo[threadIdx.x] = make_float4(lo.x, lo.y, lo.z, 0.f);
-            ghi[threadIdx.x] = make_float4(hi.x, hi.y, hi.z, 0.f);
-        }
-        __syncthreads();
+        group_boxes(sx, m, glo, ghi, slo, shi);
         if (active && !skip) {
             int nc = 0;
             auto flush = [&]() {
@@ -172,8 +204,10 @@ __global__ void __launch_bounds__(LIST_TILE)
                 nc = 0;
             };
             // periodic trajectories test every group (the boxes are not minimum-imaged)
-            if (box > 0.f) scan_tile<true>(sx, glo, ghi, m, o.ri, box, rc, rc2, mine, nc, flush);
-            else scan_tile<false>(sx, glo, ghi, m, o.ri, box, rc, rc2, mine, nc, flush);
+            if (box > 0.f)
+                scan_groups<true>(sx, glo, ghi, slo, shi, m, o.ri, box, rc, rc2, mine, LIST_TILE, LIST_CAND, nc, flush);
+            else
+                scan_groups<false>(sx, glo, ghi, slo, shi, m, o.ri, box, rc, rc2, mine, LIST_TILE, LIST_CAND, nc, flush);
             flush();
         }
         __syncthreads();
@@ -181,13 +215,88 @@ __global__ void __launch_bounds__(LIST_TILE)
     if (active) rebuild_finish(T, o, do_lj, do_bonded);
 }
 
+// The same rebuild with one workgroup per trajectory and the whole
+// trajectory in LDS (positions, rotated offsets, monomer types, group and
+// super-group boxes, a uint16 candidate buffer per lane): every bead's data
+// crosses from L2 once instead of once per 256-bead i tile.  Used when the
+// largest trajectory fits (rebuild_traj_plan).
+__global__ void __launch_bounds__(TRAJ_THREADS)
+    k_rebuild_traj(mte_params P, DevTopo T, DevState S, const int *order, int do_lj, int do_bonded) {
+    extern __shared__ float4 lds[];
+    const int t = blockIdx.x;
+    const int base = T.tbase[t], n = T.tsize[t];
+    const int ng = (n + LIST_GROUP - 1) / LIST_GROUP, nsg = (n + LIST_SUPER - 1) / LIST_SUPER;
+    float4 *sx = lds, *sn = sx + n, *s1 = sn + n, *s2 = s1 + n;
+    float4 *glo = s2 + n, *ghi = glo + ng, *slo = ghi + ng, *shi = slo + nsg;
+    int *smt = reinterpret_cast<int *>(shi + nsg);
+    uint16_t *cand = reinterpret_cast<uint16_t *>(smt + n);   // [slot][thread]
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        sx[q] = S.xyz[base + q];
+        sn[q] = S.fn[base + q];
+        s1[q] = S.fp1[base + q];
+        s2[q] = S.fp2[base + q];
+        smt[q] = T.mon_type[base + q];
+    }
+    __syncthreads();
+    group_boxes(sx, n, glo, ghi, slo, shi);
+    const float box = T.tbox[t];
+    const float rc = rebuild_radius(P.ljpairscutoff, do_lj, do_bonded);
+    const float rc2 = rc * rc;
+    uint16_t *mine = cand + threadIdx.x;
+    for (int pos = threadIdx.x; pos < n; pos += blockDim.x) {
+        const int li = order[base + pos];
+        const int b = base + li;
+        RebuildOwn o = rebuild_own(sx[li], sn[li], s1[li], s2[li], smt[li], T.harm_raw[b], li, b, base, box);
+        if (!T.extra[b]) {
+            int nc = 0;
+            auto flush = [&]() {
+                for (int q = 0; q < nc; q++) {
+                    const int jj = mine[q * blockDim.x];
+                    rebuild_pair(T, o, jj, sx[jj], sn[jj], s1[jj], s2[jj], smt[jj], P.ljpairscutoff, do_lj, do_bonded);
+                }
+                nc = 0;
+            };
+            if (box > 0.f)
+                scan_groups<true>(sx, glo, ghi, slo, shi, n, o.ri, box, rc, rc2, mine, (int)blockDim.x, TRAJ_CAND, nc,
+                                  flush);
+            else
+                scan_groups<false>(sx, glo, ghi, slo, shi, n, o.ri, box, rc, rc2, mine, (int)blockDim.x, TRAJ_CAND, nc,
+                                   flush);
+            flush();
+        }
+        rebuild_finish(T, o, do_lj, do_bonded);
+    }
+}
+
+// Workgroup size and dynamic LDS of k_rebuild_traj for trajectories of up to
+// nmax beads; false when they do not fit (the tiled kernel is used).
+bool rebuild_traj_plan(int nmax, int lds_limit, int *threads, size_t *lds) {
+    if (nmax <= 0 || nmax > 65535) return false;
+    const int th = std::min(TRAJ_THREADS, ((nmax + 63) / 64) * 64);
+    const size_t ng = (nmax + LIST_GROUP - 1) / LIST_GROUP, nsg = (nmax + LIST_SUPER - 1) / LIST_SUPER;
+    const size_t bytes = sizeof(float4) * (4 * (size_t)nmax + 2 * ng + 2 * nsg) + sizeof(int) * (size_t)nmax +
+                         sizeof(uint16_t) * (size_t)TRAJ_CAND * th;
+    if (bytes > (size_t)lds_limit) return false;
+    *threads = th;
+    *lds = (bytes + 15) & ~(size_t)15;
+    return true;
+}
+
 hipError_t launch_rebuild(mte_engine *e) {
     if (e->use_cells) return launch_rebuild_cells(e);
     const int do_lj = e->P.lj_on ? 1 : 0;
     const int do_bonded = e->P.is_assembly ? 1 : 0;
     if (!do_lj && !do_bonded) return hipSuccess;
-    hipLaunchKernelGGL(k_rebuild, dim3(e->ntiles), dim3(LIST_TILE), 0, e->stream, e->P, e->T, e->S[e->cur],
-                       e->tiles, e->rb_order, do_lj, do_bonded);
+    if (e->rb_traj_threads > 0) {
+        const hipError_t s = hipFuncSetAttribute((const void *)k_rebuild_traj,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->rb_traj_lds);
+        if (s != hipSuccess) return s;
+        hipLaunchKernelGGL(k_rebuild_traj, dim3(e->ntraj), dim3(e->rb_traj_threads), e->rb_traj_lds, e->stream, e->P,
+                           e->T, e->S[e->cur], e->rb_order, do_lj, do_bonded);
+    } else {
+        hipLaunchKernelGGL(k_rebuild, dim3(e->ntiles), dim3(LIST_TILE), 0, e->stream, e->P, e->T, e->S[e->cur],
+                           e->tiles, e->rb_order, do_lj, do_bonded);
+    }
     return hipGetLastError();
 }
 

@@ -1,6 +1,6 @@
 // rebuild_common.h -- the per-(i, j) candidate tests of a list rebuild,
-// shared by the streamed rebuild kernel (LDS tiles) and the resident kernel
-// (whole trajectory in LDS).
+// shared by the rebuild kernels of lists.hip (j tiles in LDS, or the whole
+// trajectory in LDS) and the cell-list rebuild of cells.hip.
 //
 // Restates pairs_kernel (src/compute_cuda.cu:527-674) and LJ_kernel
 // (:913-940); see lists.hip for the semantics.
@@ -16,6 +16,17 @@ __host__ __device__ inline float rebuild_radius(float ljcut, bool do_lj, bool do
     if (do_lj) r = fmaxf(r, ljcut);
     if (do_bonded) r = fmaxf(r, BOND_PREFILTER);
     return r * 1.0001f + 1e-4f;
+}
+
+// sqrtf(d2) < cut, the list test of the reference, with the square root
+// evaluated only inside a relative 2^-20 band around cut: outside it the
+// comparison of d2 with the band's squared ends decides the same way,
+// sqrtf being correctly rounded and monotonic.
+__device__ __forceinline__ bool below_cut(float d2, float cut) {
+    const float lo = cut * (1.f - 0x1p-20f), hi = cut * (1.f + 0x1p-20f);
+    if (d2 < lo * lo) return true;
+    if (d2 >= hi * hi) return false;
+    return sqrtf(d2) < cut;
 }
 
 // True when the box [lo, hi] comes within r of point p (open boundaries).
@@ -62,7 +73,7 @@ __device__ __forceinline__ void rebuild_pair(const DevTopo &T, RebuildOwn &o, in
     const int nb = T.nb;
     const float3 dij = min_image(sub3(o.ri, f3(xj)), o.box);
     const float d2 = dij.x * dij.x + dij.y * dij.y + dij.z * dij.z;
-    if (do_lj && sqrtf(d2) < ljcut && o.li != j) {
+    if (do_lj && below_cut(d2, ljcut) && o.li != j) {
         if (o.nlj < T.max_lj) T.lj[o.nlj * nb + o.b] = o.base + j;
         else o.ovf = 1;
         o.nlj++;
