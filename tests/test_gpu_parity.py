@@ -244,6 +244,25 @@ def test_cell_lists_long_open_tube():
     eng.close()
 
 
+def test_lists_tiled_rebuild_for_trajectories_beyond_lds():
+    """A 1950-bead tube is all-pairs (not cells) but too large for the
+    one-workgroup-per-trajectory rebuild: the j-tiled kernel builds its lists,
+    next to a 312-bead tube rebuilt in the same launch."""
+    print("case tiled all-pairs rebuild, 1950 + 312 beads", flush=True)
+    from mt_amd import lattice, topology
+    from mt_amd.engine import LISTS_ALLPAIRS
+    p = base_params()
+    rng = np.random.default_rng(8)
+    atoms = [lattice.microtubule(75), lattice.microtubule(12)]
+    b = topology.make_batch(atoms, [lattice.coords7(a, 0.1, rng) for a in atoms])
+    eng = Engine(p, b.copy(), mode=MODE_STREAMED, list_method=LISTS_ALLPAIRS)
+    orc = Oracle(p, b.copy())
+    eng.rebuild()
+    assert orc.rebuild() == 0
+    _lists_equal(eng, orc)
+    eng.close()
+
+
 def test_periodic_dynamics_and_tea():
     print("case periodic plain + TEA steps vs oracle", flush=True)
     from mt_amd import params as P
