@@ -152,16 +152,27 @@ __global__ void __launch_bounds__(MLP_BLOCK)
         }
     }
     if (feat) {
-        float *f = feat + (size_t)b * MLP_IN;
-        for (int k = 0; k < MLP_NN; k++) {
-            float3 d = make_float3(0.f, 0.f, 0.f);
-            if (k < cnt) {
-                d = min_image(sub3(f3(S.xyz[nj[k]]), ri), box);
-                d = make_float3(d.x / MLP_RC, d.y / MLP_RC, d.z / MLP_RC);
+        // four neighbours (12 floats, three float4 stores) at a time, their
+        // positions gathered together
+        float4 *f = reinterpret_cast<float4 *>(feat + (size_t)b * MLP_IN);
+#pragma unroll
+        for (int k0 = 0; k0 < MLP_NN; k0 += 4) {
+            float v[12];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int k = k0 + u;
+                float3 d = make_float3(0.f, 0.f, 0.f);
+                if (k < cnt) {
+                    d = min_image(sub3(f3(S.xyz[nj[k]]), ri), box);
+                    d = make_float3(d.x / MLP_RC, d.y / MLP_RC, d.z / MLP_RC);
+                }
+                v[3 * u] = d.x;
+                v[3 * u + 1] = d.y;
+                v[3 * u + 2] = d.z;
             }
-            f[3 * k] = d.x;
-            f[3 * k + 1] = d.y;
-            f[3 * k + 2] = d.z;
+            f[3 * k0 / 4] = make_float4(v[0], v[1], v[2], v[3]);
+            f[3 * k0 / 4 + 1] = make_float4(v[4], v[5], v[6], v[7]);
+            f[3 * k0 / 4 + 2] = make_float4(v[8], v[9], v[10], v[11]);
         }
     }
     if (LABELS) {
