@@ -126,6 +126,16 @@ int mte_set_state(mte_engine *e, const mte_coord *r);
 int mte_get_state(mte_engine *e, mte_coord *r);
 /* Any pointer may be NULL (left unchanged). */
 int mte_set_flags(mte_engine *e, const uint8_t *extra, const int32_t *gtp, const int32_t *on_tubule);
+
+/* The updater's tubule classification (mt_length, src/updater.cpp:154-217)
+ * on the engine's current state, without downloading it: a bead is on the
+ * tubule when rad_lo < sqrt(x^2 + y^2) < rad_hi and cos(theta) > cos_min
+ * (the reference: R_MT + R_THRES, 1, cos(ANG_THRES), src/mt.h:23,29-30).
+ * counts[ntraj] receives each trajectory's count and on_tubule[B] (may be
+ * NULL) the per-bead flags; with apply != 0 the flags also replace the
+ * engine's on_tubule, as mte_set_flags(NULL, NULL, on_tubule) would. */
+int mte_mt_length(mte_engine *e, float rad_hi, float rad_lo, float cos_min, int32_t apply, int32_t *counts,
+                  int32_t *on_tubule);
 int mte_set_step(mte_engine *e, int64_t step);
 int mte_get_step(const mte_engine *e, int64_t *step);
 

@@ -50,7 +50,7 @@ def build(verbose: bool = False) -> str:
 
 def build_host(verbose: bool = False) -> str:
     """g++ build of the host library and driver against libmt_engine.so."""
-    link = ["-L" + HERE, "-lmt_engine", "-Wl,-rpath,$ORIGIN"]
+    link = ["-L" + HERE, "-lmt_engine", "-pthread", "-Wl,-rpath,$ORIGIN"]
     tmp = HOST_LIB + ".tmp"
     _run([CXX, *CXXFLAGS, "-fPIC", "-shared", "-o", tmp, *[os.path.join(HOST, s) for s in HOST_SOURCES], *link],
          "libmt_host.so", verbose)

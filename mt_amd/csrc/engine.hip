@@ -288,7 +288,7 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     ALLOC(T.lon_n, nb) ALLOC(T.lon, (size_t)MAXLON * nb) ALLOC(T.lat_n, nb) ALLOC(T.lat, (size_t)MAXLAT * nb)
     ALLOC(T.lj_n, nb) ALLOC(T.lj, (size_t)T.max_lj * nb) ALLOC(p_fixed, nb) ALLOC(p_extra, nb) ALLOC(p_mon, nb)
     ALLOC(p_gtp, nb) ALLOC(p_ont, nb) ALLOC(p_zs, nt) ALLOC(p_box, nt) ALLOC(T.overflow, 1) ALLOC(e->tiles, tiles.size())
-    ALLOC(e->rb_order, nb)
+    ALLOC(e->rb_order, nb) ALLOC(e->mtl_counts, nt) ALLOC(e->mtl_flags, nb)
     for (int k = 0; k < 2; k++) {
         ALLOC(e->S[k].xyz, nb) ALLOC(e->S[k].ang, nb) ALLOC(e->S[k].fn, nb) ALLOC(e->S[k].fp1, nb)
         ALLOC(e->S[k].fp2, nb)
@@ -490,6 +490,18 @@ int mte_set_flags(mte_engine *e, const uint8_t *extra, const int32_t *gtp, const
     }
     if (gtp) HIPCHK(hipMemcpyAsync(e->T.gtp, gtp, sizeof(int) * e->nb, hipMemcpyHostToDevice, e->stream));
     if (on_tubule) HIPCHK(hipMemcpyAsync(e->T.on_tub, on_tubule, sizeof(int) * e->nb, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return MTE_OK;
+}
+
+int mte_mt_length(mte_engine *e, float rad_hi, float rad_lo, float cos_min, int32_t apply, int32_t *counts,
+                  int32_t *on_tubule) {
+    if (!e || !counts) return MTE_ERR_ARG;
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(launch_mt_length(e, rad_hi, rad_lo, cos_min, apply ? 1 : 0));
+    HIPCHK(hipMemcpyAsync(counts, e->mtl_counts, sizeof(int) * e->ntraj, hipMemcpyDeviceToHost, e->stream));
+    if (on_tubule)
+        HIPCHK(hipMemcpyAsync(on_tubule, e->mtl_flags, sizeof(int) * e->nb, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     return MTE_OK;
 }

@@ -168,6 +168,35 @@ static inline int grid_for(int n, int bs = STREAM_BLOCK) { return (n + bs - 1) /
             hipLaunchKernelGGL(KERNEL<256>, dim3(grid_for((e)->nb, 256)), dim3(256), 0, (e)->stream, __VA_ARGS__); \
     } while (0)
 
+// The updater's tubule classification (mt_length, src/updater.cpp:164-205)
+// on the device: a bead is on the tubule when rad_lo < sqrt(x^2 + y^2) <
+// rad_hi and cos(theta) > cos_min.  Writes the per-bead flag, optionally
+// into the barrier's on_tubule array too, and counts per trajectory.
+__global__ void __launch_bounds__(STREAM_BLOCK)
+    k_mt_length(DevTopo T, DevState S, float rad_hi, float rad_lo, float cos_min, int apply, int *flags,
+                int *counts) {
+    const int b = blockIdx.x * STREAM_BLOCK + threadIdx.x;
+    if (b >= T.nb) return;
+    const float4 x = S.xyz[b];
+    const float rad = sqrtf(x.x * x.x + x.y * x.y);
+    const int on = (rad < rad_hi && rad > rad_lo && cosf(S.ang[b].y) > cos_min) ? 1 : 0;
+    flags[b] = on;
+    if (apply) T.on_tub[b] = on;
+    if (on) atomicAdd(&counts[T.tid[b]], 1);
+}
+
+__global__ void __launch_bounds__(256) k_zero_ints(int *p, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+hipError_t launch_mt_length(mte_engine *e, float rad_hi, float rad_lo, float cos_min, int apply) {
+    hipLaunchKernelGGL(k_zero_ints, dim3((e->ntraj + 255) / 256), dim3(256), 0, e->stream, e->mtl_counts, e->ntraj);
+    hipLaunchKernelGGL(k_mt_length, dim3(grid_for(e->nb)), dim3(STREAM_BLOCK), 0, e->stream, e->T, e->S[e->cur],
+                       rad_hi, rad_lo, cos_min, apply, e->mtl_flags, e->mtl_counts);
+    return hipGetLastError();
+}
+
 hipError_t launch_frames(mte_engine *e, int buf) {
     hipLaunchKernelGGL(k_frames, dim3(grid_for(e->nb)), dim3(STREAM_BLOCK), 0, e->stream, e->S[buf], e->nb, e->L);
     return hipGetLastError();

@@ -64,6 +64,7 @@ SIGNATURES = {
     "mte_set_state": ([_P, _P], ctypes.c_int),
     "mte_get_state": ([_P, _P], ctypes.c_int),
     "mte_set_flags": ([_P, _P, _P, _P], ctypes.c_int),
+    "mte_mt_length": ([_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int32, _P, _P], ctypes.c_int),
     "mte_set_step": ([_P, ctypes.c_int64], ctypes.c_int),
     "mte_get_step": ([_P, _P], ctypes.c_int),
     "mte_run": ([_P, ctypes.c_int64], ctypes.c_int),
@@ -192,6 +193,14 @@ class Engine:
         g = None if gtp is None else np.ascontiguousarray(gtp, np.int32)
         o = None if on_tubule is None else np.ascontiguousarray(on_tubule, np.int32)
         self._chk(self.lib.mte_set_flags(self.h, _p(e), _p(g), _p(o)))
+
+    def mt_length(self, rad_hi, rad_lo, cos_min, apply=False):
+        """Tubule classification on the device: (per-trajectory counts, per-bead flags)."""
+        counts = np.zeros(self.ntraj, np.int32)
+        flags = np.zeros(self.nb, np.int32)
+        self._chk(self.lib.mte_mt_length(self.h, float(rad_hi), float(rad_lo), float(cos_min), 1 if apply else 0,
+                                         _p(counts), _p(flags)))
+        return counts, flags
 
     @property
     def step(self) -> int:

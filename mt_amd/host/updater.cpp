@@ -6,6 +6,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <ctime>
+#include <exception>
+#include <thread>
 
 namespace mth {
 
@@ -47,6 +49,43 @@ struct Engine {
     }
 };
 
+// One DCD frame at a time on a worker thread: submit() waits for the
+// previous frame, copies the state and starts the write; finish() waits for
+// the last one.  An error of a write is rethrown by the next submit() or
+// finish(); the destructor only joins.
+class FrameWriter {
+public:
+    explicit FrameWriter(const Updater &up) : up_(up) {}
+    ~FrameWriter() {
+        if (th_.joinable()) th_.join();
+    }
+    void submit(const std::vector<mte_coord> &r) {
+        finish();
+        frame_ = r;
+        th_ = std::thread([this]() {
+            try {
+                up_.save_dcd(frame_);
+            } catch (...) {
+                err_ = std::current_exception();
+            }
+        });
+    }
+    void finish() {
+        if (th_.joinable()) th_.join();
+        if (err_) {
+            std::exception_ptr e = err_;
+            err_ = nullptr;
+            std::rethrow_exception(e);
+        }
+    }
+
+private:
+    const Updater &up_;
+    std::vector<mte_coord> frame_;
+    std::thread th_;
+    std::exception_ptr err_;
+};
+
 int fail(mte_engine *e, const char *what, int rc) {
     std::fprintf(stderr, "%s failed (%d): %s\n", what, rc, e ? mte_last_error(e) : "no engine");
     return rc;
@@ -83,24 +122,21 @@ void Updater::output_forces(const std::vector<mte_coord> &f, const std::vector<m
     }
 }
 
-void Updater::mt_length(long long step, std::vector<int> &mt_len, const std::vector<mte_coord> &r) {
+int Updater::mt_length(mte_engine *e, long long step, std::vector<int> &mt_len) {
     if (step == 0) {
         FILE *f = std::fopen(o_.mt_len_file.c_str(), "w");
         if (f) std::fclose(f);
     }
+    // the classification runs on the device state; the flags reach the
+    // barrier there except at step 0, as the reference uploads them only
+    // for step != 0 (src/compute_cuda.cu:1184-1193)
+    std::vector<int32_t> counts(s_.ntr);
+    const int rc = mte_mt_length(e, R_MT + R_THRES, 1.0f, std::cos(ANG_THRES), step != 0 ? 1 : 0, counts.data(),
+                                 s_.on_tubule_cur.data());
+    if (rc) return rc;
     for (int t = 0; t < s_.ntr; t++) {
-        int sum = 0;
-        for (int i = t * s_.ntot; i < (t + 1) * s_.ntot; i++) {
-            const float rad = std::sqrt(r[i].x * r[i].x + r[i].y * r[i].y);
-            if (rad < R_MT + R_THRES && rad > 1.0f && std::cos(r[i].theta) > std::cos(ANG_THRES)) {
-                sum++;
-                s_.on_tubule_cur[i] = 1;
-            } else {
-                s_.on_tubule_cur[i] = 0;
-            }
-        }
-        mt_len[t] = sum;
-        if (!o_.quiet) std::printf("tubule[%d]: %d\n", t, sum);
+        mt_len[t] = counts[t];
+        if (!o_.quiet) std::printf("tubule[%d]: %d\n", t, counts[t]);
     }
     FILE *f = std::fopen(o_.mt_len_file.c_str(), "a");
     if (!f) throw IoError(o_.mt_len_file + ": cannot append");
@@ -108,6 +144,7 @@ void Updater::mt_length(long long step, std::vector<int> &mt_len, const std::vec
     for (int t = 0; t < s_.ntr; t++) std::fprintf(f, "%f\t", 2 * R_MON * (float)mt_len[t] / PF_NUMBER);
     std::fprintf(f, "\n");
     std::fclose(f);
+    return MTE_OK;
 }
 
 int Updater::change_conc(const std::vector<int> &delta, const std::vector<int> &mt_len, std::vector<mte_coord> &r) {
@@ -310,6 +347,10 @@ int run(RunSetup &s, const RunOptions &o) {
 
     Updater up(s, o);
     up.create_dcds();
+    // DCD frames are written by a second host thread from a copy of the
+    // output step's state while the engine runs the next segment; one frame
+    // is in flight at a time, so the files keep the reference's frame order.
+    FrameWriter writer(up);
     std::vector<mte_coord> r = s.coords, f(nb);
     std::vector<double> energies(7 * (size_t)nt);
     std::vector<int> mt_len(nt, 0), mt_len_prev(nt, 0);
@@ -330,7 +371,7 @@ int run(RunSetup &s, const RunOptions &o) {
                     std::printf("Saving coordinates at step %lld\n", step);
                     print_time(step, s.steps);
                 }
-                up.save_dcd(r);
+                writer.submit(r);
                 if (s.hydrolysis && step % (stride * 10) == 0) up.append_hydrolysis_pdb(step, r);
                 if (s.out_force) up.output_forces(f, r);
                 if (s.out_energy) up.output_energies(energies);
@@ -339,9 +380,7 @@ int run(RunSetup &s, const RunOptions &o) {
                 s.on_tubule_prev = s.on_tubule_cur;
                 if (step != 0) {
                     mt_len_prev = mt_len;
-                    up.mt_length(step, mt_len, r);
-                    if (s.barrier && (rc = mte_set_flags(E.e, nullptr, nullptr, s.on_tubule_cur.data())))
-                        return fail(E.e, "mte_set_flags", rc);
+                    if ((rc = up.mt_length(E.e, step, mt_len))) return fail(E.e, "mte_mt_length", rc);
                     if (s.is_const_conc) {
                         for (int t = 0; t < nt; t++) mt_len_prev[t] = mt_len[t] - mt_len_prev[t];
                         if (up.change_conc(mt_len_prev, mt_len, r)) {
@@ -353,7 +392,7 @@ int run(RunSetup &s, const RunOptions &o) {
                     update();
                 } else {
                     update();
-                    up.mt_length(step, mt_len, r);
+                    if ((rc = up.mt_length(E.e, step, mt_len))) return fail(E.e, "mte_mt_length", rc);
                 }
             } else {
                 update();
@@ -365,6 +404,7 @@ int run(RunSetup &s, const RunOptions &o) {
         if ((rc = mte_run(E.e, next - step))) return fail(E.e, "mte_run", rc);
         step = next;
     }
+    writer.finish();
     // as the reference: the result PDBs hold the host copy of the last output step
     up.save_result_pdb(r);
     return MTE_OK;

@@ -2,7 +2,8 @@
 // the engine through the C ABI.
 //
 //   Updater::output_energies   OutputAllEnergies, src/updater.cpp:3-43
-//   Updater::mt_length         mt_length, :154-217 (on_tubule_cur, mt_len.dat)
+//   Updater::mt_length         mt_length, :154-217 (on_tubule_cur, mt_len.dat; the
+//                              classification itself runs on the device, mte_mt_length)
 //   Updater::change_conc       change_conc, :100-152 (glibc rand(), as there)
 //   Updater::hydrolyse         hydrolyse, :219-256 (glibc rand())
 //   Updater::update            update, :74-98 (DCD frames, energies, forces)
@@ -35,7 +36,7 @@ public:
     Updater(RunSetup &s, const RunOptions &o);
     void output_energies(const std::vector<double> &per_traj) const;
     void output_forces(const std::vector<mte_coord> &f, const std::vector<mte_coord> &r) const;
-    void mt_length(long long step, std::vector<int> &mt_len, const std::vector<mte_coord> &r);
+    int mt_length(mte_engine *e, long long step, std::vector<int> &mt_len);
     int change_conc(const std::vector<int> &delta, const std::vector<int> &mt_len, std::vector<mte_coord> &r);
     void hydrolyse();
     void create_dcds() const;

@@ -90,3 +90,74 @@ def test_driver_run_matches_engine(tmp_path, capfd):
         s.close()
     finally:
         os.chdir(old)
+
+
+R_MT, R_THRES, ANG_THRES = np.float32(8.12), np.float32(16.0), np.float32(1.0)
+
+
+def _classify(xyz, theta):
+    """The reference's mt_length test in float32 (src/updater.cpp:164-205)."""
+    rad = np.sqrt(xyz[:, 0] * xyz[:, 0] + xyz[:, 1] * xyz[:, 1])
+    c = np.cos(theta.astype(np.float32))
+    on = (rad < R_MT + R_THRES) & (rad > np.float32(1.0)) & (c > np.cos(ANG_THRES))
+    near = np.abs(c - np.cos(ANG_THRES)) < 1e-6   # a 1-ulp cosine difference may flip these
+    return on.astype(np.int32), near
+
+
+def test_mt_length_on_device_matches_host_formula():
+    """mte_mt_length classifies the device state as the reference's host
+    loop does, counts per trajectory, and with apply=1 sets the barrier's
+    flags exactly as mte_set_flags with the same flags would."""
+    print("case mte_mt_length vs the host formula, 2 x 312 beads", flush=True)
+    from tests.helpers import base_params, c1_batch
+    p = base_params()
+    b = c1_batch(ntraj=2)
+    rng = np.random.default_rng(4)
+    moved = rng.choice(b.nb, 40, replace=False)
+    b.coords[moved[:20], 0] += 30.0            # off the tube radially
+    b.coords[moved[20:], 4] += 1.2             # tilted beyond the angle threshold
+    eng = Engine(p, b.copy(), max_lj=256, mode=MODE_STREAMED)
+    eng.run(50)
+    st = eng.get_state()
+    counts, flags = eng.mt_length(R_MT + R_THRES, 1.0, np.cos(ANG_THRES), apply=True)
+    want, near = _classify(st[:, :3], st[:, 4])
+    assert np.array_equal(flags[~near], want[~near])
+    assert 0 < want.sum() < b.nb
+    np.testing.assert_array_equal(counts, [flags[:312].sum(), flags[312:].sum()])
+    ref = Engine(p, b.copy(), max_lj=256, mode=MODE_STREAMED)
+    ref.run(50)
+    ref.set_flags(on_tubule=flags)
+    np.testing.assert_array_equal(eng.energies()[1], ref.energies()[1])
+    eng.run(100)
+    ref.run(100)
+    np.testing.assert_array_equal(eng.get_state(), ref.get_state())
+    eng.close()
+    ref.close()
+
+
+def test_driver_mt_len_file_matches_frames(tmp_path, capfd):
+    """tubule_length on: mt_len.dat holds, at every output step, 2 r_mon / 13
+    times the count of beads the reference's test puts on the tube in that
+    step's DCD frame."""
+    print("case host driver: tubule_length yes, mt_len.dat vs DCD frames", flush=True)
+    for f in os.listdir(RUN):
+        shutil.copy(os.path.join(RUN, f), tmp_path / f)
+    old = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        host.run("main.conf", overrides=("tubule_length=yes",), mode=MODE_STREAMED, max_lj=256, quiet=True)
+        capfd.readouterr()
+        _, frames = read_dcd("traj_0.dcd")
+        _, angs = read_dcd("traj_0.dcd_ang")
+        _, frames1 = read_dcd("traj_1.dcd")
+        _, angs1 = read_dcd("traj_1.dcd_ang")
+        rows = [l.split("\t") for l in open("mt_len.dat").read().splitlines()]
+        assert [int(r[0]) for r in rows] == [100 * k for k in range(10)]
+        for k, r in enumerate(rows):
+            for t, (fr, an) in enumerate(((frames, angs), (frames1, angs1))):
+                on, near = _classify(fr[k], an[k][:, 2])   # the ang DCD holds fi, psi, theta
+                if near.any():
+                    continue
+                assert float(r[1 + t]) == float("%f" % (np.float32(4.0) * np.float32(on.sum()) / np.float32(13)))
+    finally:
+        os.chdir(old)
