@@ -192,6 +192,11 @@ int mth_run(const char *path, const char *const *overrides, int n_over, int32_t 
         o.mode = mode;
         o.max_lj = max_lj;
         o.quiet = quiet != 0;
+        mth::Config main_cfg;   // the checkpoint keys, as mt_host reads them
+        main_cfg.parse_file(path, to_vec(overrides, n_over));
+        o.checkpoint_every = main_cfg.long_integer("engine_checkpoint", 0);
+        o.checkpoint_file = main_cfg.masked("engine_checkpoint_file", "checkpoint.bin");
+        if (main_cfg.has("engine_resume")) o.resume_file = main_cfg.masked("engine_resume");
         const int rc = mth::run(s, o);
         if (rc) g_err = "run failed with engine status " + std::to_string(rc);
         return rc;

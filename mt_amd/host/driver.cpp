@@ -9,6 +9,13 @@
 //   engine_max_lj  LJ list capacity per bead     (default 256, the reference's)
 //   engine_quiet   yes | no                      (default no)
 //   engine_autotune  steps per trial of mte_autotune before the run (default 0: off)
+//   engine_checkpoint  write a checkpoint every this many steps (default 0: off;
+//                  a multiple of stride, LJPairsUpdateFreq, tea_epsilon_freq and the
+//                  hydrolysis step, so that nothing is pending at a checkpoint)
+//   engine_checkpoint_file  its path (default checkpoint.bin)
+//   engine_resume  continue from this checkpoint: state, HybridTaus streams, step,
+//                  flags and the host rand() sequence as they were, so the steps
+//                  after it are those of the uninterrupted run
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -32,6 +39,9 @@ int main(int argc, char **argv) {
         opt.max_lj = main_cfg.integer("engine_max_lj", 256);
         opt.quiet = main_cfg.yesno("engine_quiet", false);
         opt.autotune_steps = main_cfg.integer("engine_autotune", 0);
+        opt.checkpoint_every = main_cfg.long_integer("engine_checkpoint", 0);
+        opt.checkpoint_file = main_cfg.masked("engine_checkpoint_file", "checkpoint.bin");
+        if (main_cfg.has("engine_resume")) opt.resume_file = main_cfg.masked("engine_resume");
         mth::RunSetup setup = mth::load_run(argv[1], overrides);
         return mth::run(setup, opt) == MTE_OK ? 0 : 1;
     } catch (const std::exception &ex) {

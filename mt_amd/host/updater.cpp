@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <ctime>
 #include <exception>
 #include <thread>
@@ -18,6 +19,89 @@ constexpr float ANG_THRES = 1.0f;      // src/mt.h:29
 constexpr int PF_NUMBER = 13;
 
 clock_t g_t0;
+long long g_rand_calls = 0;   // glibc rand() draws since srand(rseed): replayed on resume
+
+// The reference's host randomness (glibc rand() after srand(rseed)), counted.
+int host_rand() {
+    g_rand_calls++;
+    return std::rand();
+}
+
+// Checkpoint of a run at a step where nothing is pending: the engine state,
+// both HybridTaus streams, the updater's flags and counts, and the number of
+// rand() draws so far.  Host byte order; version in the magic.
+constexpr char CKPT_MAGIC[8] = {'M', 'T', 'E', 'C', 'K', 'P', 'T', '1'};
+
+struct Checkpoint {
+    long long step = 0, rand_calls = 0;
+    std::vector<mte_coord> coords;
+    std::vector<uint32_t> rng;   // 8 words per bead: the xyz stream, then the angular stream
+    std::vector<uint8_t> extra;
+    std::vector<int32_t> gtp, on_cur, on_prev, mt_len;
+};
+
+template <class T> void put(FILE *f, const std::vector<T> &v) {
+    if (std::fwrite(v.data(), sizeof(T), v.size(), f) != v.size()) throw IoError("checkpoint: short write");
+}
+template <class T> void get(FILE *f, std::vector<T> &v, size_t n) {
+    v.resize(n);
+    if (std::fread(v.data(), sizeof(T), n, f) != n) throw IoError("checkpoint: file too short");
+}
+
+void write_checkpoint(const std::string &path, const Checkpoint &c, int ntr, int ntot) {
+    const std::string tmp = path + ".tmp";
+    FILE *f = std::fopen(tmp.c_str(), "wb");
+    if (!f) throw IoError("cannot write checkpoint " + tmp);
+    const int32_t dims[2] = {ntr, ntot};
+    const int64_t counters[2] = {c.step, c.rand_calls};
+    bool ok = std::fwrite(CKPT_MAGIC, 1, 8, f) == 8 && std::fwrite(dims, sizeof dims, 1, f) == 1 &&
+              std::fwrite(counters, sizeof counters, 1, f) == 1;
+    if (ok) {
+        put(f, c.coords);
+        put(f, c.rng);
+        put(f, c.extra);
+        put(f, c.gtp);
+        put(f, c.on_cur);
+        put(f, c.on_prev);
+        put(f, c.mt_len);
+    }
+    ok = std::fclose(f) == 0 && ok;
+    if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0) throw IoError("cannot write checkpoint " + path);
+}
+
+Checkpoint read_checkpoint(const std::string &path, int ntr, int ntot) {
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) throw IoError("cannot read checkpoint " + path);
+    char magic[8];
+    int32_t dims[2];
+    int64_t counters[2];
+    Checkpoint c;
+    try {
+        if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, CKPT_MAGIC, 8) != 0)
+            throw IoError("checkpoint " + path + ": not a checkpoint of this format");
+        if (std::fread(dims, sizeof dims, 1, f) != 1 || std::fread(counters, sizeof counters, 1, f) != 1)
+            throw IoError("checkpoint " + path + ": file too short");
+        if (dims[0] != ntr || dims[1] != ntot)
+            throw IoError("checkpoint " + path + ": written for " + std::to_string(dims[0]) + " x " +
+                          std::to_string(dims[1]) + " beads, this run has " + std::to_string(ntr) + " x " +
+                          std::to_string(ntot));
+        const size_t nb = (size_t)ntr * ntot;
+        c.step = counters[0];
+        c.rand_calls = counters[1];
+        get(f, c.coords, nb);
+        get(f, c.rng, 8 * nb);
+        get(f, c.extra, nb);
+        get(f, c.gtp, nb);
+        get(f, c.on_cur, nb);
+        get(f, c.on_prev, nb);
+        get(f, c.mt_len, ntr);
+    } catch (...) {
+        std::fclose(f);
+        throw;
+    }
+    std::fclose(f);
+    return c;
+}
 
 void print_formatted_time(float t) {
     const int days = (int)(t / (3600.0f * 24.0f));
@@ -164,8 +248,8 @@ int Updater::change_conc(const std::vector<int> &delta, const std::vector<int> &
                     s_.extra[p + 1] = 0;
                     float x, y;
                     for (;;) {
-                        x = s_.rep_r - 2 * (std::rand() % int(s_.rep_r));
-                        y = s_.rep_r - 2 * (std::rand() % int(s_.rep_r));
+                        x = s_.rep_r - 2 * (host_rand() % int(s_.rep_r));
+                        y = s_.rep_r - 2 * (host_rand() % int(s_.rep_r));
                         if (x * x + y * y <= s_.rep_r * s_.rep_r) {
                             if (!o_.quiet)
                                 std::printf("New x,y coordinates for extra particle: %f  %f index: %d\n", x, y, (int)p);
@@ -204,7 +288,7 @@ void Updater::hydrolyse() {
         for (int t = 0; t < s_.ntr; t++) {
             const size_t p = i + (size_t)t * n;
             if (s_.gtp[p] == 1 && !s_.extra[p] && s_.on_tubule_cur[p] * s_.on_tubule_prev[p] == 1) {
-                const double prob = std::rand() / (double)RAND_MAX;
+                const double prob = host_rand() / (double)RAND_MAX;
                 if (prob < 0.02) {
                     s_.gtp[p] = 0;
                     s_.gtp[p + 1] = 0;
@@ -295,7 +379,17 @@ int run(RunSetup &s, const RunOptions &o) {
         return MTE_ERR_ARG;
     }
     std::srand((unsigned)s.rseed);   // main.cpp:67
+    g_rand_calls = 0;
     g_t0 = std::clock();
+    if (o.checkpoint_every < 0 ||
+        (o.checkpoint_every > 0 && (o.checkpoint_every % std::llabs(s.stride) != 0 ||
+                                    o.checkpoint_every % s.ljpairsupdatefreq != 0 ||
+                                    (s.tea_on && o.checkpoint_every % s.tea_epsilon_freq != 0) ||
+                                    (s.hydrolysis && s.hydrostep > 0 && o.checkpoint_every % s.hydrostep != 0)))) {
+        std::fprintf(stderr, "engine_checkpoint %lld: must be a multiple of stride, LJPairsUpdateFreq, "
+                             "tea_epsilon_freq (TEA) and the hydrolysis step\n", o.checkpoint_every);
+        return MTE_ERR_ARG;
+    }
     const int n = s.ntot, nt = s.ntr;
     const size_t nb = (size_t)n * nt;
     wrap_angles(s.coords);           // initIntegration wraps the host copy, then uploads it
@@ -345,6 +439,25 @@ int run(RunSetup &s, const RunOptions &o) {
         if (!o.quiet) std::printf("Autotune:\n%s", report);
     }
 
+    long long step = 0;
+    std::vector<int> mt_len(nt, 0), mt_len_prev(nt, 0);
+    if (!o.resume_file.empty()) {
+        // continue a checkpointed run: the engine and the updater exactly as
+        // they were when the checkpoint's step was about to run
+        const Checkpoint c = read_checkpoint(o.resume_file, nt, n);
+        s.coords = c.coords;
+        s.extra = c.extra;
+        s.gtp = c.gtp;
+        s.on_tubule_cur = c.on_cur;
+        s.on_tubule_prev = c.on_prev;
+        mt_len.assign(c.mt_len.begin(), c.mt_len.end());
+        for (long long k = 0; k < c.rand_calls; k++) host_rand();
+        if ((rc = mte_set_state(E.e, s.coords.data())) || (rc = mte_set_rng(E.e, c.rng.data())) ||
+            (rc = mte_set_flags(E.e, s.extra.data(), s.gtp.data(), s.on_tubule_cur.data())) ||
+            (rc = mte_set_step(E.e, c.step)))
+            return fail(E.e, "resume", rc);
+        step = c.step;
+    }
     Updater up(s, o);
     up.create_dcds();
     // DCD frames are written by a second host thread from a copy of the
@@ -353,51 +466,69 @@ int run(RunSetup &s, const RunOptions &o) {
     FrameWriter writer(up);
     std::vector<mte_coord> r = s.coords, f(nb);
     std::vector<double> energies(7 * (size_t)nt);
-    std::vector<int> mt_len(nt, 0), mt_len_prev(nt, 0);
     const long long stride = std::llabs(s.stride);
-    long long step = 0;
+    bool resumed = !o.resume_file.empty();   // the checkpoint's step had its events before it was written
     while (step < s.steps) {
-        if (s.hydrolysis && s.hydrostep > 0 && step % s.hydrostep == 0 && step != 0) {
-            up.hydrolyse();
-            if ((rc = mte_set_flags(E.e, nullptr, s.gtp.data(), nullptr))) return fail(E.e, "mte_set_flags", rc);
-        }
-        if (step % stride == 0) {
-            if (s.out_energy && (rc = mte_compute_energies(E.e, nullptr, energies.data())))
-                return fail(E.e, "mte_compute_energies", rc);
-            if (s.out_force && (rc = mte_compute_forces(E.e, f.data()))) return fail(E.e, "mte_compute_forces", rc);
-            if ((rc = mte_get_state(E.e, r.data()))) return fail(E.e, "mte_get_state", rc);
-            auto update = [&]() {
-                if (!o.quiet) {
-                    std::printf("Saving coordinates at step %lld\n", step);
-                    print_time(step, s.steps);
-                }
-                writer.submit(r);
-                if (s.hydrolysis && step % (stride * 10) == 0) up.append_hydrolysis_pdb(step, r);
-                if (s.out_force) up.output_forces(f, r);
-                if (s.out_energy) up.output_energies(energies);
-            };
-            if (s.tub_length) {
-                s.on_tubule_prev = s.on_tubule_cur;
-                if (step != 0) {
-                    mt_len_prev = mt_len;
-                    if ((rc = up.mt_length(E.e, step, mt_len))) return fail(E.e, "mte_mt_length", rc);
-                    if (s.is_const_conc) {
-                        for (int t = 0; t < nt; t++) mt_len_prev[t] = mt_len[t] - mt_len_prev[t];
-                        if (up.change_conc(mt_len_prev, mt_len, r)) {
-                            if ((rc = mte_set_flags(E.e, s.extra.data(), nullptr, nullptr)))
-                                return fail(E.e, "mte_set_flags", rc);
-                            if ((rc = mte_set_state(E.e, r.data()))) return fail(E.e, "mte_set_state", rc);
-                        }
+        if (!resumed) {
+            if (s.hydrolysis && s.hydrostep > 0 && step % s.hydrostep == 0 && step != 0) {
+                up.hydrolyse();
+                if ((rc = mte_set_flags(E.e, nullptr, s.gtp.data(), nullptr))) return fail(E.e, "mte_set_flags", rc);
+            }
+            if (step % stride == 0) {
+                if (s.out_energy && (rc = mte_compute_energies(E.e, nullptr, energies.data())))
+                    return fail(E.e, "mte_compute_energies", rc);
+                if (s.out_force && (rc = mte_compute_forces(E.e, f.data()))) return fail(E.e, "mte_compute_forces", rc);
+                if ((rc = mte_get_state(E.e, r.data()))) return fail(E.e, "mte_get_state", rc);
+                auto update = [&]() {
+                    if (!o.quiet) {
+                        std::printf("Saving coordinates at step %lld\n", step);
+                        print_time(step, s.steps);
                     }
-                    update();
+                    writer.submit(r);
+                    if (s.hydrolysis && step % (stride * 10) == 0) up.append_hydrolysis_pdb(step, r);
+                    if (s.out_force) up.output_forces(f, r);
+                    if (s.out_energy) up.output_energies(energies);
+                };
+                if (s.tub_length) {
+                    s.on_tubule_prev = s.on_tubule_cur;
+                    if (step != 0) {
+                        mt_len_prev = mt_len;
+                        if ((rc = up.mt_length(E.e, step, mt_len))) return fail(E.e, "mte_mt_length", rc);
+                        if (s.is_const_conc) {
+                            for (int t = 0; t < nt; t++) mt_len_prev[t] = mt_len[t] - mt_len_prev[t];
+                            if (up.change_conc(mt_len_prev, mt_len, r)) {
+                                if ((rc = mte_set_flags(E.e, s.extra.data(), nullptr, nullptr)))
+                                    return fail(E.e, "mte_set_flags", rc);
+                                if ((rc = mte_set_state(E.e, r.data()))) return fail(E.e, "mte_set_state", rc);
+                            }
+                        }
+                        update();
+                    } else {
+                        update();
+                        if ((rc = up.mt_length(E.e, step, mt_len))) return fail(E.e, "mte_mt_length", rc);
+                    }
                 } else {
                     update();
-                    if ((rc = up.mt_length(E.e, step, mt_len))) return fail(E.e, "mte_mt_length", rc);
                 }
-            } else {
-                update();
+            }
+            if (o.checkpoint_every > 0 && step > 0 && step % o.checkpoint_every == 0) {
+                Checkpoint c;
+                c.step = step;
+                c.rand_calls = g_rand_calls;
+                c.coords.resize(nb);
+                c.rng.resize(8 * nb);
+                if ((rc = mte_get_state(E.e, c.coords.data())) || (rc = mte_get_rng(E.e, c.rng.data())))
+                    return fail(E.e, "checkpoint", rc);
+                c.extra = s.extra;
+                c.gtp = s.gtp;
+                c.on_cur = s.on_tubule_cur;
+                c.on_prev = s.on_tubule_prev;
+                c.mt_len.assign(mt_len.begin(), mt_len.end());
+                write_checkpoint(o.checkpoint_file, c, nt, n);
+                if (!o.quiet) std::printf("Checkpoint at step %lld: %s\n", step, o.checkpoint_file.c_str());
             }
         }
+        resumed = false;
         long long next = (step / stride + 1) * stride;
         if (s.hydrolysis && s.hydrostep > 0) next = std::min(next, (step / s.hydrostep + 1) * (long long)s.hydrostep);
         next = std::min(next, s.steps);

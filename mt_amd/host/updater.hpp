@@ -27,6 +27,9 @@ struct RunOptions {
     int32_t max_lj = 256;           // LJ list capacity (the reference's)
     bool quiet = false;             // suppress the per-event log lines
     int autotune_steps = 0;         // > 0: mte_autotune with this many steps per trial before the run
+    long long checkpoint_every = 0; // > 0: write a checkpoint at every multiple of this many steps
+    std::string checkpoint_file = "checkpoint.bin";
+    std::string resume_file;        // non-empty: continue the run from this checkpoint
     std::string result_xyz = "result_xyz.pdb", result_ang = "result_ang.pdb";
     std::string mt_len_file = "mt_len.dat", hydrolysis_pdb = "dcd/hydrolysis.pdb";
 };

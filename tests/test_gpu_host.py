@@ -161,3 +161,45 @@ def test_driver_mt_len_file_matches_frames(tmp_path, capfd):
                 assert float(r[1 + t]) == float("%f" % (np.float32(4.0) * np.float32(on.sum()) / np.float32(13)))
     finally:
         os.chdir(old)
+
+
+def test_checkpoint_resume_continues_exactly(tmp_path, capfd):
+    """engine_checkpoint writes the run's state at step 500 (engine state,
+    HybridTaus streams, flags, the host rand() count); engine_resume continues
+    from it, and every later DCD frame, mt_len.dat row and printed energy
+    equals the uninterrupted run's, with hydrolysis and the tubule test on."""
+    print("case host driver: checkpoint at 500, resume to 1000", flush=True)
+    over = ("tubule_length=yes", "hydrolysis=yes")
+    runs = {}
+    for name in ("full", "resumed"):
+        d = tmp_path / name
+        d.mkdir()
+        (d / "dcd").mkdir()
+        for f in os.listdir(RUN):
+            shutil.copy(os.path.join(RUN, f), d / f)
+    old = os.getcwd()
+    try:
+        os.chdir(tmp_path / "full")
+        capfd.readouterr()
+        host.run("main.conf", overrides=over + ("engine_checkpoint=500",), mode=MODE_STREAMED, max_lj=256,
+                 quiet=True)
+        out_full = capfd.readouterr().out
+        shutil.copy("checkpoint.bin", tmp_path / "resumed" / "checkpoint.bin")
+        os.chdir(tmp_path / "resumed")
+        host.run("main.conf", overrides=over + ("engine_resume=checkpoint.bin",), mode=MODE_STREAMED, max_lj=256,
+                 quiet=True)
+        out_res = capfd.readouterr().out
+        for name in ("traj_0.dcd", "traj_1.dcd", "traj_0.dcd_ang", "traj_1.dcd_ang"):
+            _, full = read_dcd(str(tmp_path / "full" / name))
+            _, res = read_dcd(str(tmp_path / "resumed" / name))
+            assert len(full) == 10 and len(res) == 4
+            for k in range(4):
+                np.testing.assert_array_equal(res[k], full[6 + k])
+        rows_full = open(tmp_path / "full" / "mt_len.dat").read().splitlines()
+        rows_res = open(tmp_path / "resumed" / "mt_len.dat").read().splitlines()
+        assert rows_res == rows_full[6:]
+        e_full = [l for l in out_full.splitlines() if l.startswith("Energies[")]
+        e_res = [l for l in out_res.splitlines() if l.startswith("Energies[")]
+        assert len(e_full) == 20 and e_res == e_full[12:]
+    finally:
+        os.chdir(old)
