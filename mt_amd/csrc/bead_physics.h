@@ -68,6 +68,49 @@ __device__ __forceinline__ void bending(const mte_params &P, float rm, float th0
     }
 }
 
+// Per-bead flags packed into xyz.w when a trajectory's state is staged in LDS.
+constexpr int FL_MON = 1, FL_GTP1 = 2, FL_OFFTUB = 4;
+
+__device__ __forceinline__ int pack_flags(const DevTopo &T, int b) {
+    return (T.mon_type[b] ? FL_MON : 0) | (T.gtp[b] == 1 ? FL_GTP1 : 0) | (T.on_tub[b] == 0 ? FL_OFFTUB : 0);
+}
+
+// A trajectory's bead data staged in LDS, by local index: position with the
+// packed flags in w, and the rotated offsets with the Euler angles in w
+// (n|fi, p1|theta, p2|psi).
+struct LdsAcc {
+    const float4 *x, *n, *p1, *p2;
+    __device__ __forceinline__ float4 xyz(int j) const { return x[j]; }
+    __device__ __forceinline__ float4 ang(int j) const { return make_float4(n[j].w, p1[j].w, p2[j].w, 0.f); }
+    __device__ __forceinline__ float4 fn(int j) const { return n[j]; }
+    __device__ __forceinline__ float4 fp1(int j) const { return p1[j]; }
+    __device__ __forceinline__ float4 fp2(int j) const { return p2[j]; }
+    __device__ __forceinline__ int flags(int j) const { return __float_as_int(x[j].w); }
+    __device__ __forceinline__ int mon(int j) const { return flags(j) & FL_MON; }
+    __device__ __forceinline__ bool gtp1(int j) const { return (flags(j) & FL_GTP1) != 0; }
+    __device__ __forceinline__ bool offtub(int j) const { return (flags(j) & FL_OFFTUB) != 0; }
+};
+
+// Stages beads [0, n) of the trajectory starting at global bead `base` into
+// the four LDS arrays (all threads of the workgroup; no barrier).
+__device__ __forceinline__ void stage_state(const DevTopo &T, const DevState &S, int base, int n, float4 *lx,
+                                            float4 *ln, float4 *l1, float4 *l2) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int b = base + i;
+        float4 x = S.xyz[b];
+        x.w = __int_as_float(pack_flags(T, b));
+        const float4 a = S.ang[b];
+        float4 fn = S.fn[b], f1 = S.fp1[b], f2 = S.fp2[b];
+        fn.w = a.x;
+        f1.w = a.y;
+        f2.w = a.z;
+        lx[i] = x;
+        ln[i] = fn;
+        l1[i] = f1;
+        l2[i] = f2;
+    }
+}
+
 // Neighbour lists held in the k-major global arrays of bead b, with entry
 // indices taken relative to `base` (0: global indices, the streamed kernels;
 // the trajectory's first bead: local indices, the resident kernel).

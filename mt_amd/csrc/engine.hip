@@ -420,6 +420,8 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
         if (hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device) != hipSuccess ||
             !rebuild_traj_plan(nmax, lds_limit, &e->rb_traj_threads, &e->rb_traj_lds))
             e->rb_traj_threads = 0;
+        // forces with the trajectory staged in LDS when it fits
+        e->forces_lds = forces_lds_bytes(nmax) <= (size_t)lds_limit ? forces_lds_bytes(nmax) : 0;
     }
     ResidentPlan plan;
     const bool fits = resident_fits(e, &plan);

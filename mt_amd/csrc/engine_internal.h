@@ -104,6 +104,7 @@ struct mte_engine {
     int ntiles = 0;
     int *rb_order = nullptr;   // per trajectory: local bead handled at each rebuild position (spatial order)
     int *mtl_counts = nullptr, *mtl_flags = nullptr;   // mte_mt_length: per trajectory, per bead
+    size_t forces_lds = 0;     // > 0: forces by k_forces_lds with this much dynamic LDS
     int rb_traj_threads = 0;   // > 0: list rebuilds by k_rebuild_traj with this workgroup size
     size_t rb_traj_lds = 0;    // and this much dynamic LDS
     int *d_overflow = nullptr;
@@ -145,6 +146,7 @@ hipError_t launch_forces_tea(mte_engine *e, const float *noise_step);
 hipError_t launch_energies(mte_engine *e);
 hipError_t launch_rebuild(mte_engine *e);
 hipError_t launch_rebuild_order(mte_engine *e);
+size_t forces_lds_bytes(int nmax);
 hipError_t launch_mt_length(mte_engine *e, float rad_hi, float rad_lo, float cos_min, int apply);
 bool rebuild_traj_plan(int nmax, int lds_limit, int *threads, size_t *lds);
 hipError_t launch_rebuild_cells(mte_engine *e);

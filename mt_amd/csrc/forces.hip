@@ -10,6 +10,8 @@ namespace mte {
 // engine's tuning (64, 128 or 256).
 constexpr int STREAM_BLOCK = 256;
 
+constexpr int FORCE_TILE = 256;   // k_forces_lds: beads per workgroup (the 256-bead tiles of e->tiles)
+
 struct GlobalAcc {
     DevState S;
     const DevTopo &T;
@@ -120,6 +122,48 @@ __global__ void __launch_bounds__(BS)
     }
 }
 
+// The same forces with the bead's whole trajectory staged in LDS (positions,
+// rotated offsets, angles and flags of every bead), one workgroup per
+// 256-bead tile of a trajectory: neighbour data are read from LDS instead of
+// gathered from L2 per pair.  For trajectories that fit (forces_lds_bytes).
+__global__ void __launch_bounds__(FORCE_TILE)
+    k_forces_lds(mte_params P, DevTopo T, DevState S, const int2 *tiles, float4 *fxyz, float4 *fang, TeaDraw D) {
+    extern __shared__ float4 lds[];
+    const int2 tl = tiles[blockIdx.x];
+    const int t = tl.x, base = T.tbase[t], n = T.tsize[t];
+    float4 *lx = lds, *ln = lds + n, *l1 = ln + n, *l2 = l1 + n;
+    stage_state(T, S, base, n, lx, ln, l1, l2);
+    __syncthreads();
+    const int i = tl.y + threadIdx.x;
+    if (i >= n) return;
+    const int b = base + i;
+    float F[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (!T.extra[b]) {
+        const float4 fn = ln[i], f1 = l1[i], f2 = l2[i];
+        Own I = make_own(lx[i], make_float4(fn.w, f1.w, f2.w, 0.f), fn, f1, f2);
+        I.box = T.tbox[t];
+        bead_force(P, LdsAcc{lx, ln, l1, l2}, GlobalLists{T, b, base}, i, P.is_wall ? T.zs[t] : 0.f, I, F);
+    }
+    fxyz[b] = make_float4(F[0], F[1], F[2], 0.f);
+    fang[b] = make_float4(F[3], F[4], F[5], 0.f);   // fi, theta, psi
+    if (D.R) {
+        const float var = sqrtf(2.0f * KB * P.temp * P.gammaR / P.dt);
+        float4 r;
+        if (D.noise) {
+            const float *nz = D.noise + 6 * (size_t)b;
+            r = make_float4(nz[0], nz[1], nz[2], 0.f);
+        } else {
+            uint4 s = D.seeds[b];
+            r = normals4(s);
+            D.seeds[b] = s;
+        }
+        D.R[b] = make_float4(r.x * var, r.y * var, r.z * var, 0.f);
+    }
+}
+
+// Dynamic LDS of k_forces_lds for trajectories of up to nmax beads.
+size_t forces_lds_bytes(int nmax) { return (size_t)4 * sizeof(float4) * nmax; }
+
 template <int BS>
 __global__ void __launch_bounds__(BS)
     k_energies(mte_params P, DevTopo T, DevState S, double *en) {
@@ -208,14 +252,23 @@ hipError_t launch_step_plain(mte_engine *e, const float *noise_step) {
     return hipGetLastError();
 }
 
-hipError_t launch_forces(mte_engine *e) {
-    MTE_LAUNCH_BS(k_forces, e, e->P, e->T, e->S[e->cur], e->fxyz, e->fang, TeaDraw{nullptr, nullptr, nullptr});
+static hipError_t launch_forces_draw(mte_engine *e, TeaDraw D) {
+    if (e->forces_lds > 0) {
+        const hipError_t s = hipFuncSetAttribute((const void *)k_forces_lds,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->forces_lds);
+        if (s != hipSuccess) return s;
+        hipLaunchKernelGGL(k_forces_lds, dim3(e->ntiles), dim3(FORCE_TILE), e->forces_lds, e->stream, e->P, e->T,
+                           e->S[e->cur], e->tiles, e->fxyz, e->fang, D);
+    } else {
+        MTE_LAUNCH_BS(k_forces, e, e->P, e->T, e->S[e->cur], e->fxyz, e->fang, D);
+    }
     return hipGetLastError();
 }
 
+hipError_t launch_forces(mte_engine *e) { return launch_forces_draw(e, TeaDraw{nullptr, nullptr, nullptr}); }
+
 hipError_t launch_forces_tea(mte_engine *e, const float *noise_step) {
-    MTE_LAUNCH_BS(k_forces, e, e->P, e->T, e->S[e->cur], e->fxyz, e->fang, TeaDraw{e->seeds, noise_step, e->tea.R});
-    return hipGetLastError();
+    return launch_forces_draw(e, TeaDraw{e->seeds, noise_step, e->tea.R});
 }
 
 hipError_t launch_energies(mte_engine *e) {

@@ -35,28 +35,11 @@ constexpr float FILTER_R = LJ_CUT + 2.f * SKIN_HALF + 0.01f;
 // Minimum room for list entries per bead when sizing the plan.
 constexpr int MIN_ENTRIES_PER_BEAD = 24;
 
-// Per-bead flags packed into xyz.w in LDS.
-constexpr int FL_MON = 1, FL_GTP1 = 2, FL_OFFTUB = 4;
-
 // Packed counts: bit 0 harmonic present, bits 1-4 longitudinal, 5-9 lateral,
 // 10-19 LJ.
 __device__ __forceinline__ int pack_counts(int h, int nlon, int nlat, int nlj) {
     return (h ? 1 : 0) | (nlon << 1) | (nlat << 5) | (nlj << 10);
 }
-
-// Bead data in LDS by local index.
-struct LdsAcc {
-    const float4 *x, *n, *p1, *p2;
-    __device__ __forceinline__ float4 xyz(int j) const { return x[j]; }
-    __device__ __forceinline__ float4 ang(int j) const { return make_float4(n[j].w, p1[j].w, p2[j].w, 0.f); }
-    __device__ __forceinline__ float4 fn(int j) const { return n[j]; }
-    __device__ __forceinline__ float4 fp1(int j) const { return p1[j]; }
-    __device__ __forceinline__ float4 fp2(int j) const { return p2[j]; }
-    __device__ __forceinline__ int flags(int j) const { return __float_as_int(x[j].w); }
-    __device__ __forceinline__ int mon(int j) const { return flags(j) & FL_MON; }
-    __device__ __forceinline__ bool gtp1(int j) const { return (flags(j) & FL_GTP1) != 0; }
-    __device__ __forceinline__ bool offtub(int j) const { return (flags(j) & FL_OFFTUB) != 0; }
-};
 
 // Lists compacted in LDS: packed entries with local indices, the encoding
 // bead_force reads through LdsAcc.
@@ -175,21 +158,7 @@ __global__ void __launch_bounds__(MAXT)
     const int t = blockIdx.x;
     const int base = T.tbase[t], n = T.tsize[t];
     const int nb = T.nb;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const int b = base + i;
-        float4 x = S.xyz[b];
-        const int fl = (T.mon_type[b] ? FL_MON : 0) | (T.gtp[b] == 1 ? FL_GTP1 : 0) | (T.on_tub[b] == 0 ? FL_OFFTUB : 0);
-        x.w = __int_as_float(fl);
-        const float4 a = S.ang[b];
-        float4 fn = S.fn[b], f1 = S.fp1[b], f2 = S.fp2[b];
-        fn.w = a.x;
-        f1.w = a.y;
-        f2.w = a.z;
-        lx[i] = x;
-        ln[i] = fn;
-        l1[i] = f1;
-        l2[i] = f2;
-    }
+    stage_state(T, S, base, n, lx, ln, l1, l2);
     bool mob[PT];
     uint4 s0[PT], s1[PT];
     float4 trig[PT];
