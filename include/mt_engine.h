@@ -182,10 +182,15 @@ int mte_get_mode(const mte_engine *e, int32_t *mode);
  * Launch shapes of the engine's kernels (no counterpart in the reference,
  * whose launches are fixed at BLOCK_SIZE threads, src/Cuda.h:122-135).
  * A field set to 0 keeps the current choice. */
+/* tea_chunk value: 64-bead chunks with each (i, j) RPY block evaluated once
+ * for both beads (the other chunk values evaluate it for i and for j). */
+#define MTE_TEA_SYMMETRIC 1
+
 typedef struct {
     int32_t resident_per_thread;  /* beads per thread of the resident kernel: 1, 2 or 3 */
     int32_t stream_block;         /* workgroup size of the streamed bead kernels: 64, 128 or 256 */
-    int32_t tea_chunk;            /* j-chunk of the TEA pair kernel: 64, 128 or 256 beads (0 without TEA) */
+    int32_t tea_chunk;            /* j-chunk of the TEA pair kernel: 64, 128 or 256 beads, or
+                                     MTE_TEA_SYMMETRIC (0 without TEA) */
     int32_t cell_stencil;         /* cell-list rebuilds: 1 = 27 cells of side r, 2 = 125 cells of side r/2 */
     int32_t mode;                 /* MTE_MODE_RESIDENT or MTE_MODE_STREAMED (plain dynamics) */
     int32_t reserved[3];

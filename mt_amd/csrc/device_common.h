@@ -77,6 +77,15 @@ __device__ __forceinline__ float3 min_image(float3 d, float L) {
     }
     return d;
 }
+// min_image with 1/L taken outside a loop (iL = 0 on open boundaries, where
+// d comes back unchanged): no branch in the loop body
+__device__ __forceinline__ float3 min_image_inv(float3 d, float L, float iL) {
+    d.x -= L * rintf(d.x * iL);
+    d.y -= L * rintf(d.y * iL);
+    d.z -= L * rintf(d.z * iL);
+    return d;
+}
+__device__ __forceinline__ float inv_side(float L) { return L > 0.f ? 1.f / L : 0.f; }
 __device__ __forceinline__ float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
 
 // Periodic cell index: i in [-dims, 2 dims) wrapped into [0, dims) (a

@@ -46,7 +46,8 @@ static void release(mte_engine *e, void *p) {
 }
 
 // Work units (trajectory, 256-bead i group, j chunk) of the TEA pair kernel
-// and its per-chunk partial buffer, for j-chunks of `chunk` beads.
+// and its per-chunk partial buffer, for j-chunks of `chunk` beads; for
+// MTE_TEA_SYMMETRIC, units (trajectory, 64-bead chunk I, chunk J >= I).
 hipError_t mte::tea_build_units(mte_engine *e, int chunk) {
     release(e, e->tea.units);
     release(e, e->tea.nchunks);
@@ -60,10 +61,25 @@ hipError_t mte::tea_build_units(mte_engine *e, int chunk) {
     for (int t = 0; t < e->ntraj; t++) {
         const int n = (int)(e->off[t + 1] - e->off[t]);
         int c = 0;
-        for (int j0 = 0; j0 < n; j0 += chunk, c++)
-            for (int i0 = 0; i0 < n; i0 += 256) units.push_back(make_int4(t, i0, j0, c));
+        if (chunk == MTE_TEA_SYMMETRIC) {
+            for (int i0 = 0; i0 < n; i0 += 64, c++)
+                for (int j0 = i0; j0 < n; j0 += 64) units.push_back(make_int4(t, i0, j0, c));
+        } else {
+            for (int j0 = 0; j0 < n; j0 += chunk, c++)
+                for (int i0 = 0; i0 < n; i0 += 256) units.push_back(make_int4(t, i0, j0, c));
+        }
         nchunks[t] = c;
         maxc = std::max(maxc, c);
+    }
+    if (chunk == MTE_TEA_SYMMETRIC) {
+        // longest first: off-diagonal blocks (64 steps of both sides), then
+        // diagonal blocks (one-sided, one step per bead of the chunk)
+        auto cost = [&](const int4 &u) {
+            const int n = (int)(e->off[u.x + 1] - e->off[u.x]);
+            return u.y == u.z ? std::min(64, n - u.z) : 128;
+        };
+        std::stable_sort(units.begin(), units.end(),
+                         [&](const int4 &a, const int4 &b) { return cost(a) > cost(b); });
     }
     hipError_t s;
     if ((s = dalloc(e, &e->tea.units, units.size())) != hipSuccess) return s;
@@ -320,8 +336,10 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     if (topo->wall_z) std::copy(topo->wall_z, topo->wall_z + nt, zs.begin());
     std::vector<float> box(nt, 0.f);
     if (topo->box) std::copy(topo->box, topo->box + nt, box.begin());
-    for (int t = 0; t < nt; t++)
+    for (int t = 0; t < nt; t++) {
         if (!(box[t] >= 0.f)) { e->err = "mte_create: periodic box side must be >= 0"; return bail(MTE_ERR_ARG); }
+        if (box[t] > 0.f) e->periodic = true;
+    }
     std::vector<uint32_t> seeds((size_t)8 * nb);
     generate_seeds(opt->rseed, seeds.size(), seeds.data());
 
@@ -1122,8 +1140,8 @@ int mte_set_tuning(mte_engine *e, const mte_tuning *t) {
         e->tune.stream_block = t->stream_block;
     }
     if (t->tea_chunk && t->tea_chunk != e->tune.tea_chunk) {
-        if (t->tea_chunk != 64 && t->tea_chunk != 128 && t->tea_chunk != 256)
-            return fail(e, MTE_ERR_ARG, "mte_set_tuning: tea_chunk must be 64, 128 or 256");
+        if (t->tea_chunk != 64 && t->tea_chunk != 128 && t->tea_chunk != 256 && t->tea_chunk != MTE_TEA_SYMMETRIC)
+            return fail(e, MTE_ERR_ARG, "mte_set_tuning: tea_chunk must be 64, 128, 256 or MTE_TEA_SYMMETRIC");
         if (!e->P.tea_on) return fail(e, MTE_ERR_ARG, "mte_set_tuning: tea_chunk needs TEA");
         if (t->tea_chunk != e->tea.unit_chunk) HIPCHK(tea_build_units(e, t->tea_chunk));
     }
@@ -1236,7 +1254,7 @@ int mte_autotune(mte_engine *e, int32_t trial_steps, mte_tuning *chosen, char *r
         if ((rc = restore()) || (rc = mte_set_tuning(e, &best))) return rc;
     } else {
         if ((rc = family("stream_block", {64, 128, 256}, &e->tune.stream_block))) return rc;
-        if ((rc = family("tea_chunk", {64, 128, 256}, &e->tune.tea_chunk))) return rc;
+        if ((rc = family("tea_chunk", {64, 128, 256, MTE_TEA_SYMMETRIC}, &e->tune.tea_chunk))) return rc;
     }
     if (e->mode == MTE_MODE_STREAMED && e->use_cells && need_lists &&
         (rc = family("cell_stencil", {1, 2}, &e->tune.cell_stencil)))

@@ -84,6 +84,7 @@ struct mte_engine {
     mte_params P{};
     mte_options O{};
     int nb = 0, ntraj = 0;
+    bool periodic = false;   // some trajectory has a periodic box
     std::vector<int64_t> off;
     int device = 0;
     int num_cus = 1;

@@ -42,8 +42,10 @@ __device__ __forceinline__ RpyCoef rpy_select(float r2, float rinv, float a, flo
 // block sums in closed form (|e| = 1):
 //   sum of the 9 entries of D_hat      = 3 cii + crr (ex + ey + ez)^2
 //   row a sum of squares               = cii^2 + crr (2 cii + crr) e_a^2
-// Extra beads and the j == i slot enter with weight 0 (their distance kept
-// finite), so no branch sits in the pair loop.
+// Extra beads enter with weight 0; the j == i slot (d = 0, its distance
+// clamped finite) is taken out after the loop, so no branch sits in the pair
+// loop.  PERIODIC: some trajectory has a periodic box.
+template <bool PERIODIC>
 __global__ void __launch_bounds__(TEA_TILE)
     k_tea_eps(mte_params P, DevTopo T, DevState S, const int2 *tiles, float4 *ci, float *eps) {
     __shared__ float4 sx[TEA_TILE];
@@ -58,7 +60,7 @@ __global__ void __launch_bounds__(TEA_TILE)
         ri = f3(S.xyz[b]);
         skip = T.extra[b] != 0;
     }
-    const float box = T.tbox[tl.x];
+    const float box = T.tbox[tl.x], ibox = inv_side(box);
     const float a = P.tea_a, ia = rcp_fast(P.tea_a);
     float scc = 0.f, sx2 = 0.f, sy2 = 0.f, sz2 = 0.f, sall = 0.f;
     for (int j0 = 0; j0 < n; j0 += TEA_TILE) {
@@ -71,14 +73,14 @@ __global__ void __launch_bounds__(TEA_TILE)
         __syncthreads();
         const int m = min(TEA_TILE, n - j0);
         if (active && !skip) {
-            const int self = li - j0;
-#pragma unroll 4
+            // not unrolled: measured faster than unrolled by 2 or 4
+#pragma unroll 1
             for (int jj = 0; jj < m; jj++) {
                 const float4 xj = sx[jj];
-                const float3 d = min_image(sub3(f3(xj), ri), box);
-                const bool is_self = jj == self;
-                const float wj = is_self ? 0.f : xj.w;
-                const float r2 = is_self ? 1.f : fmaxf(d.x * d.x + d.y * d.y + d.z * d.z, 1e-12f);
+                float3 d = sub3(f3(xj), ri);
+                if (PERIODIC) d = min_image_inv(d, box, ibox);
+                const float wj = xj.w;
+                const float r2 = fmaxf(d.x * d.x + d.y * d.y + d.z * d.z, 1e-12f);
                 const float rinv = rsq_fast(r2);
                 const RpyCoef c = rpy_select(r2, rinv, a, ia);
                 const float ex = d.x * rinv, ey = d.y * rinv, ez = d.z * rinv;
@@ -93,6 +95,14 @@ __global__ void __launch_bounds__(TEA_TILE)
             }
         }
         __syncthreads();
+    }
+    if (active && !skip) {
+        // the j == i slot entered as a coincident bead of weight 1 (d = 0):
+        // cii^2 into scc and 3 cii into sall, recomputed and taken out
+        const float r2 = 1e-12f, rinv = rsq_fast(r2);
+        const float cii = rpy_select(r2, rinv, a, ia).cii;
+        scc = fmaf(-cii, cii, scc);
+        sall = fmaf(-3.f, cii, sall);
     }
     if (active) {
         ci[b] = make_float4(scc + sx2, scc + sy2, scc + sz2, 0.f);
@@ -153,7 +163,7 @@ __global__ void __launch_bounds__(256)
 // noise, so they contribute nothing and need no branch.
 // Splitting j into chunks gives ~10 work units per SIMD at 64 x 1040 beads,
 // against ~1 per SIMD for one thread per bead over the whole trajectory.
-template <int CHUNK>
+template <int CHUNK, bool PERIODIC>
 __global__ void __launch_bounds__(TEA_TILE)
     k_tea_pairs(mte_params P, DevTopo T, DevState S, const float4 *fxyz, const float4 *R, const float4 *ci,
                 const float *beta, const int4 *units, int chunk_stride, float4 *partial) {
@@ -183,10 +193,11 @@ __global__ void __launch_bounds__(TEA_TILE)
         const float cx = bt * rsq_fast(1.f + w * cc.x), cy = bt * rsq_fast(1.f + w * cc.y),
                     cz = bt * rsq_fast(1.f + w * cc.z);
         const float a = P.tea_a, ia = rcp_fast(P.tea_a);
-        const float box = T.tbox[u.x];
+        const float box = T.tbox[u.x], ibox = inv_side(box);
         for (int q = 0; q < jn; q++) {
             const float4 xj = sx[q], Fj = sf[q], Rj = sr[q];
-            const float3 d = min_image(sub3(f3(xj), ri), box);
+            float3 d = sub3(f3(xj), ri);
+            if (PERIODIC) d = min_image_inv(d, box, ibox);
             const float dx = d.x, dy = d.y, dz = d.z;
             // the j == i slot (d = 0) and coincident beads stay finite through the clamp
             // (extra beads carry zero F and R)
@@ -214,6 +225,140 @@ __global__ void __launch_bounds__(TEA_TILE)
         }
     }
     partial[(size_t)u.w * chunk_stride + b] = make_float4(acc.x, acc.y, acc.z, 0.f);
+}
+
+// Pair sums with each (i, j) RPY block evaluated once for both beads.  Work
+// unit: (trajectory, 64-bead chunk I, 64-bead chunk J >= I), one
+// single-wave workgroup each, so the dispatcher balances units wave by wave.  D_hat_ji = D_hat_ij, so the block serves
+//   i-side  D_hat_ij (F_j + beta C_i . R_j)  into partial slot J of bead i,
+//   j-side  D_hat_ij (F_i + beta C_j . R_i)  into partial slot I of bead j,
+// and slot c of a bead holds the sum over chunk c, as k_tea_pairs<64> writes
+// it.  Off the diagonal, lane l takes j = (l + s) mod 64 at step s: every
+// lane meets every j of the chunk once and no two lanes the same j in a
+// step.  The j-side sum of j travels with j in registers: after each step
+// the wave rotates it one lane down (DPP wave_rol:1, lane l takes lane l+1's
+// value), so at step s + 1 it sits in the lane that meets j next, and after
+// 64 steps in lane j.  A diagonal unit (I == J) is one-sided, as
+// k_tea_pairs.  j slots past the trajectory's end and extra beads carry zero
+// F and R, so their terms vanish; their sums are not stored (extra beads
+// store zero).
+// Lane l of the wave receives lane (l + 1) mod 64's value (all lanes active).
+__device__ __forceinline__ float wave_rol1(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
+}
+
+constexpr int SYM_CHUNK = 64;
+template <bool PERIODIC>
+__global__ void __launch_bounds__(SYM_CHUNK)
+    k_tea_pairs_sym(mte_params P, DevTopo T, DevState S, const float4 *fxyz, const float4 *R, const float4 *ci,
+                    const float *beta, const int4 *units, int nunits, int chunk_stride, float4 *partial) {
+    // j data of each wave's block, packed as (x, F.x) (F.y, F.z, R.x, R.y)
+    // (R.z, beta C) and written twice, at q and q + 64, so that lane l reads
+    // slot l + s at step s with no wrap
+    __shared__ float4 jd[3][2 * SYM_CHUNK];
+    const int l = threadIdx.x;
+    const int ui = blockIdx.x;
+    const bool has = ui < nunits;
+    const int4 u = has ? units[ui] : make_int4(0, 0, 0, 0);
+    const int base = T.tbase[u.x], n = T.tsize[u.x];
+    const float bt = beta[u.x], w2 = bt * bt;
+    const float box = T.tbox[u.x], ibox = inv_side(box);
+    // bead of this lane in chunk I ...
+    const int li = u.y + l, bi = base + li;
+    const bool vi = has && li < n;
+    const bool ei = vi && T.extra[bi];
+    float3 xi = make_float3(0.f, 0.f, 0.f), Fi = xi, Ri = xi, Ci = xi;
+    if (vi) {
+        xi = f3(S.xyz[bi]);
+        if (!ei) {
+            Fi = f3(fxyz[bi]);
+            Ri = f3(R[bi]);
+            const float4 cc = ci[bi];
+            Ci = make_float3(bt * rsq_fast(1.f + w2 * cc.x), bt * rsq_fast(1.f + w2 * cc.y),
+                             bt * rsq_fast(1.f + w2 * cc.z));
+        }
+    }
+    // ... and in chunk J, staged for the wave
+    const int lj = u.z + l, bj = base + lj;
+    const bool vj = has && lj < n;
+    const bool ej = vj && T.extra[bj];
+    {
+        float3 x = make_float3(0.f, 0.f, 0.f), F = x, Rj = x, C = x;
+        if (vj) {
+            x = f3(S.xyz[bj]);
+            if (!ej) {
+                F = f3(fxyz[bj]);
+                Rj = f3(R[bj]);
+                const float4 cc = ci[bj];
+                C = make_float3(bt * rsq_fast(1.f + w2 * cc.x), bt * rsq_fast(1.f + w2 * cc.y),
+                                bt * rsq_fast(1.f + w2 * cc.z));
+            }
+        }
+        const float4 d0 = make_float4(x.x, x.y, x.z, F.x), d1 = make_float4(F.y, F.z, Rj.x, Rj.y),
+                     d2 = make_float4(Rj.z, C.x, C.y, C.z);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            jd[0][l + h * SYM_CHUNK] = d0;
+            jd[1][l + h * SYM_CHUNK] = d1;
+            jd[2][l + h * SYM_CHUNK] = d2;
+        }
+    }
+    __syncthreads();
+    const float a = P.tea_a, ia = rcp_fast(P.tea_a);
+    float3 acc = make_float3(0.f, 0.f, 0.f), g = acc;
+    const bool diag = u.y == u.z;
+    if (has && diag) {
+        if (vi && !ei) {
+            const int jn = min(SYM_CHUNK, n - u.z);
+            for (int q = 0; q < jn; q++) {
+                const float4 A = jd[0][q], B = jd[1][q], Cq = jd[2][q];
+                float3 d = make_float3(A.x - xi.x, A.y - xi.y, A.z - xi.z);
+                if (PERIODIC) d = min_image_inv(d, box, ibox);
+                const float r2 = fmaxf(d.x * d.x + d.y * d.y + d.z * d.z, 1e-12f);
+                const float rinv = rsq_fast(r2);
+                const RpyCoef c = rpy_select(r2, rinv, a, ia);
+                const float vx = fmaf(Ci.x, B.z, A.w), vy = fmaf(Ci.y, B.w, B.x), vz = fmaf(Ci.z, Cq.x, B.y);
+                const float sv = (c.crr * rinv * rinv) * (d.x * vx + d.y * vy + d.z * vz);
+                acc.x = fmaf(sv, d.x, fmaf(c.cii, vx, acc.x));
+                acc.y = fmaf(sv, d.y, fmaf(c.cii, vy, acc.y));
+                acc.z = fmaf(sv, d.z, fmaf(c.cii, vz, acc.z));
+            }
+            // the j == i slot (d = 0) left its cii v_i term: taken out again
+            const float r2 = 1e-12f, rinv = rsq_fast(r2);
+            const float cii = rpy_select(r2, rinv, a, ia).cii;
+            acc.x = fmaf(-cii, fmaf(Ci.x, Ri.x, Fi.x), acc.x);
+            acc.y = fmaf(-cii, fmaf(Ci.y, Ri.y, Fi.y), acc.y);
+            acc.z = fmaf(-cii, fmaf(Ci.z, Ri.z, Fi.z), acc.z);
+        }
+    } else if (has) {
+        const float4 *j0 = &jd[0][l], *j1 = &jd[1][l], *j2 = &jd[2][l];
+        for (int s = 0; s < SYM_CHUNK; s++) {
+            const float4 A = j0[s], B = j1[s], Cq = j2[s];
+            float3 d = make_float3(A.x - xi.x, A.y - xi.y, A.z - xi.z);
+            if (PERIODIC) d = min_image_inv(d, box, ibox);
+            const float r2 = fmaxf(d.x * d.x + d.y * d.y + d.z * d.z, 1e-12f);
+            const float rinv = rsq_fast(r2);
+            const RpyCoef c = rpy_select(r2, rinv, a, ia);
+            const float cr = c.crr * rinv * rinv;
+            // i-side: F_j + beta C_i . R_j
+            const float vx = fmaf(Ci.x, B.z, A.w), vy = fmaf(Ci.y, B.w, B.x), vz = fmaf(Ci.z, Cq.x, B.y);
+            const float sv = cr * (d.x * vx + d.y * vy + d.z * vz);
+            acc.x = fmaf(sv, d.x, fmaf(c.cii, vx, acc.x));
+            acc.y = fmaf(sv, d.y, fmaf(c.cii, vy, acc.y));
+            acc.z = fmaf(sv, d.z, fmaf(c.cii, vz, acc.z));
+            // j-side: F_i + beta C_j . R_i (e e^T is even in e)
+            const float wx = fmaf(Cq.y, Ri.x, Fi.x), wy = fmaf(Cq.z, Ri.y, Fi.y), wz = fmaf(Cq.w, Ri.z, Fi.z);
+            const float sw = cr * (d.x * wx + d.y * wy + d.z * wz);
+            g.x = wave_rol1(fmaf(sw, d.x, fmaf(c.cii, wx, g.x)));
+            g.y = wave_rol1(fmaf(sw, d.y, fmaf(c.cii, wy, g.y)));
+            g.z = wave_rol1(fmaf(sw, d.z, fmaf(c.cii, wz, g.z)));
+        }
+    }
+    if (vi) partial[(size_t)(u.z / SYM_CHUNK) * chunk_stride + bi] = ei ? make_float4(0.f, 0.f, 0.f, 0.f)
+                                                                         : make_float4(acc.x, acc.y, acc.z, 0.f);
+    if (vj && !diag)
+        partial[(size_t)(u.y / SYM_CHUNK) * chunk_stride + bj] = ej ? make_float4(0.f, 0.f, 0.f, 0.f)
+                                                                     : make_float4(g.x, g.y, g.z, 0.f);
 }
 
 // Self term, chunk partials in fixed order, and the update of one bead
@@ -270,33 +415,48 @@ __global__ void __launch_bounds__(TEA_TILE)
 }
 
 hipError_t launch_tea_eps(mte_engine *e) {
-    hipLaunchKernelGGL(k_tea_eps, dim3(e->ntiles), dim3(TEA_TILE), 0, e->stream, e->P, e->T, e->S[e->cur],
-                       e->tiles, e->tea.ci, e->tea.eps);
+    if (e->periodic)
+        hipLaunchKernelGGL(k_tea_eps<true>, dim3(e->ntiles), dim3(TEA_TILE), 0, e->stream, e->P, e->T,
+                           e->S[e->cur], e->tiles, e->tea.ci, e->tea.eps);
+    else
+        hipLaunchKernelGGL(k_tea_eps<false>, dim3(e->ntiles), dim3(TEA_TILE), 0, e->stream, e->P, e->T,
+                           e->S[e->cur], e->tiles, e->tea.ci, e->tea.eps);
     hipLaunchKernelGGL(k_tea_beta, dim3(e->ntraj), dim3(256), 0, e->stream, e->P, e->T, e->tea.eps, e->tea.beta,
                        e->tea.stats, e->tea.status);
     return hipGetLastError();
 }
 
+template <int CHUNK>
+static void launch_pairs_chunk(mte_engine *e) {
+    if (e->periodic)
+        hipLaunchKernelGGL((k_tea_pairs<CHUNK, true>), dim3(e->tea.nunits), dim3(TEA_TILE), 0, e->stream, e->P,
+                           e->T, e->S[e->cur], e->fxyz, e->tea.R, e->tea.ci, e->tea.beta, e->tea.units, e->nb,
+                           e->tea.partial);
+    else
+        hipLaunchKernelGGL((k_tea_pairs<CHUNK, false>), dim3(e->tea.nunits), dim3(TEA_TILE), 0, e->stream, e->P,
+                           e->T, e->S[e->cur], e->fxyz, e->tea.R, e->tea.ci, e->tea.beta, e->tea.units, e->nb,
+                           e->tea.partial);
+}
+
 hipError_t launch_tea_pairs(mte_engine *e) {
     // the kernel's chunk must be the one the work units were built for
+    if (e->tea.unit_chunk == MTE_TEA_SYMMETRIC) {
+        const int nwg = e->tea.nunits;
+        if (e->periodic)
+            hipLaunchKernelGGL(k_tea_pairs_sym<true>, dim3(nwg), dim3(SYM_CHUNK), 0, e->stream, e->P, e->T,
+                               e->S[e->cur], e->fxyz, e->tea.R, e->tea.ci, e->tea.beta, e->tea.units,
+                               e->tea.nunits, e->nb, e->tea.partial);
+        else
+            hipLaunchKernelGGL(k_tea_pairs_sym<false>, dim3(nwg), dim3(SYM_CHUNK), 0, e->stream, e->P, e->T,
+                               e->S[e->cur], e->fxyz, e->tea.R, e->tea.ci, e->tea.beta, e->tea.units,
+                               e->tea.nunits, e->nb, e->tea.partial);
+        return hipGetLastError();
+    }
     switch (e->tea.unit_chunk) {
-    case 64:
-        hipLaunchKernelGGL(k_tea_pairs<64>, dim3(e->tea.nunits), dim3(TEA_TILE), 0, e->stream, e->P, e->T,
-                           e->S[e->cur], e->fxyz, e->tea.R, e->tea.ci, e->tea.beta, e->tea.units, e->nb,
-                           e->tea.partial);
-        break;
-    case 256:
-        hipLaunchKernelGGL(k_tea_pairs<256>, dim3(e->tea.nunits), dim3(TEA_TILE), 0, e->stream, e->P, e->T,
-                           e->S[e->cur], e->fxyz, e->tea.R, e->tea.ci, e->tea.beta, e->tea.units, e->nb,
-                           e->tea.partial);
-        break;
-    case 128:
-        hipLaunchKernelGGL(k_tea_pairs<128>, dim3(e->tea.nunits), dim3(TEA_TILE), 0, e->stream, e->P, e->T,
-                           e->S[e->cur], e->fxyz, e->tea.R, e->tea.ci, e->tea.beta, e->tea.units, e->nb,
-                           e->tea.partial);
-        break;
-    default:
-        return hipErrorInvalidValue;
+    case 64: launch_pairs_chunk<64>(e); break;
+    case 128: launch_pairs_chunk<128>(e); break;
+    case 256: launch_pairs_chunk<256>(e); break;
+    default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "libmt_engine.so")
 MTE_OK = 0
 MODE_AUTO, MODE_STREAMED, MODE_RESIDENT = 0, 1, 2
 LISTS_AUTO, LISTS_ALLPAIRS, LISTS_CELLS = 0, 1, 2
+TEA_SYMMETRIC = 1   # tea_chunk: 64-bead chunks, each pair block evaluated once for both beads
 ERRORS = {1: "MTE_ERR_ARG", 2: "MTE_ERR_HIP", 3: "MTE_ERR_OVERFLOW", 4: "MTE_ERR_TEA", 5: "MTE_ERR_STATE",
           6: "MTE_ERR_NOISE"}
 

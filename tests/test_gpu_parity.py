@@ -7,7 +7,7 @@ injected identical noise compared against the FP32 oracle.
 import numpy as np
 import pytest
 
-from mt_amd.engine import Engine, MODE_STREAMED
+from mt_amd.engine import Engine, MODE_STREAMED, TEA_SYMMETRIC
 from oracle.oracle import Oracle
 from tests.helpers import ANG, XYZ, base_params, c1_batch, noise, rel_err, uneven_batch
 
@@ -96,12 +96,15 @@ def test_rng_streams_bit_exact():
     eng.close()
 
 
-def test_tea_step_parity():
-    print("case TEA eps/beta + steps", flush=True)
+@pytest.mark.parametrize("chunk", [0, TEA_SYMMETRIC])
+def test_tea_step_parity(chunk):
+    print(f"case TEA eps/beta + steps, tea_chunk={chunk}", flush=True)
     from mt_amd import params as P
     p = P.derive(dict(P.TEA, tea_epsilon_freq=5))
     b = c1_batch()
     eng, orc = _pair(p, b, precision="f64")
+    if chunk:
+        eng.set_tuning(tea_chunk=chunk)
     rng = np.random.default_rng(5)
     nsteps = 10
     nz = noise(rng, nsteps, b.nb)
@@ -159,14 +162,17 @@ def test_resident_rng_and_uneven_batch():
     eng.close()
 
 
-def test_tea_drift_against_fp64_exact_em():
-    print("case TEA drift vs FP64 exact EM (312 beads)", flush=True)
+@pytest.mark.parametrize("chunk", [0, TEA_SYMMETRIC])
+def test_tea_drift_against_fp64_exact_em(chunk):
+    print(f"case TEA drift vs FP64 exact EM (312 beads), tea_chunk={chunk}", flush=True)
     from mt_amd import params as P
     from oracle import em_exact
     # dt = 1000 ps: the zero-noise step is ~10 nm, resolvable in float32 positions
     p = P.derive(dict(P.TEA, tea_epsilon_freq=1, dt=1000.0))
     b = c1_batch()
     eng = Engine(p, b.copy(), mode=MODE_STREAMED)
+    if chunk:
+        eng.set_tuning(tea_chunk=chunk)
     orc = Oracle(p, b.copy())
     orc.rebuild()
     F = orc.forces()
@@ -183,11 +189,12 @@ def test_tea_drift_against_fp64_exact_em():
     eng.close()
 
 
+@pytest.mark.parametrize("chunk", [0, TEA_SYMMETRIC])
 @pytest.mark.parametrize("periodic", [False, True])
-def test_tea_batch_with_extra_beads_matches_oracle(periodic):
+def test_tea_batch_with_extra_beads_matches_oracle(periodic, chunk):
     """Extra (reservoir) beads take no part in TEA: excluded from epsilon and
     from every pair sum, also under the minimum image of a periodic box."""
-    print(f"case TEA uneven batch with extra beads, periodic={periodic}", flush=True)
+    print(f"case TEA uneven batch with extra beads, periodic={periodic}, tea_chunk={chunk}", flush=True)
     from mt_amd import params as P
     p = P.derive(dict(P.TEA, tea_epsilon_freq=3))
     b = uneven_batch()
@@ -198,6 +205,8 @@ def test_tea_batch_with_extra_beads_matches_oracle(periodic):
     if periodic:
         b.box = np.array([130.0, 100.0, 180.0], np.float32)   # each larger than its system
     eng, orc = _pair(p, b, precision="f64")
+    if chunk:
+        eng.set_tuning(tea_chunk=chunk)
     rng = np.random.default_rng(8)
     nz = noise(rng, 6, b.nb)
     eng.set_noise(nz)

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from mt_amd import params as P, workloads
-from mt_amd.engine import Engine, EngineError, MODE_RESIDENT, MODE_STREAMED
+from mt_amd.engine import Engine, EngineError, MODE_RESIDENT, MODE_STREAMED, TEA_SYMMETRIC
 from tests.helpers import c1_batch
 
 pytestmark = pytest.mark.gpu
@@ -59,8 +59,8 @@ def test_autotune_tea_and_cells():
     t = Engine(p, b.copy(), max_lj=48, mode=MODE_STREAMED, rseed=9)
     chosen, report = t.autotune(20)
     print(chosen, report, flush=True)
-    assert chosen["tea_chunk"] in (64, 128, 256) and chosen["cell_stencil"] in (1, 2)
-    assert sum(l.startswith("tea_chunk") for l in report) == 3
+    assert chosen["tea_chunk"] in (64, 128, 256, TEA_SYMMETRIC) and chosen["cell_stencil"] in (1, 2)
+    assert sum(l.startswith("tea_chunk") for l in report) == 4
     assert sum(l.startswith("cell_stencil") for l in report) == 2
     t.run(120)
     got = t.get_state()
@@ -78,6 +78,7 @@ def test_autotune_tea_and_cells():
     x.close()
     # each shape on its own (and after other shapes), against the default run
     for shapes in ([{"tea_chunk": 64}], [{"tea_chunk": 256}], [{"tea_chunk": 256}, {"tea_chunk": 64}],
+                   [{"tea_chunk": TEA_SYMMETRIC}], [{"tea_chunk": TEA_SYMMETRIC}, {"tea_chunk": 128}],
                    [{"cell_stencil": 2}]):
         e = Engine(p, b.copy(), max_lj=48, mode=MODE_STREAMED, rseed=9)
         for shape in shapes:
