@@ -17,6 +17,7 @@
 // which adds the self term and integrates.  Each (i, j) block costs one rsqrt
 // and ~35 FP32 operations: the product is VALU-bound.
 #include "engine_internal.h"
+#include <type_traits>
 
 namespace mte {
 
@@ -34,6 +35,15 @@ __device__ __forceinline__ RpyCoef rpy_select(float r2, float rinv, float a, flo
     RpyCoef c;
     c.cii = near ? fmaf(-0.28125f, ra, 1.f) : 0.75f * u1 * fmaf(0.66666666666666667f, u2, 1.f);
     c.crr = near ? 0.09375f * ra : 0.75f * u1 * fmaf(-2.f, u2, 1.f);
+    return c;
+}
+
+// The far branch of rpy_select alone (r > 2a).
+__device__ __forceinline__ RpyCoef rpy_far(float rinv, float a) {
+    const float u1 = a * rinv, u2 = u1 * u1;
+    RpyCoef c;
+    c.cii = 0.75f * u1 * fmaf(0.66666666666666667f, u2, 1.f);
+    c.crr = 0.75f * u1 * fmaf(-2.f, u2, 1.f);
     return c;
 }
 
@@ -244,7 +254,8 @@ __global__ void __launch_bounds__(TEA_TILE)
 // store zero).
 // Lane l of the wave receives lane (l + 1) mod 64's value (all lanes active).
 __device__ __forceinline__ float wave_rol1(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
+    const int x = __float_as_int(v);
+    return __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x134, 0xF, 0xF, false));
 }
 
 constexpr int SYM_CHUNK = 64;
@@ -332,27 +343,53 @@ __global__ void __launch_bounds__(SYM_CHUNK)
         }
     } else if (has) {
         const float4 *j0 = &jd[0][l], *j1 = &jd[1][l], *j2 = &jd[2][l];
-        for (int s = 0; s < SYM_CHUNK; s++) {
-            const float4 A = j0[s], B = j1[s], Cq = j2[s];
-            float3 d = make_float3(A.x - xi.x, A.y - xi.y, A.z - xi.z);
-            if (PERIODIC) d = min_image_inv(d, box, ibox);
-            const float r2 = fmaxf(d.x * d.x + d.y * d.y + d.z * d.z, 1e-12f);
-            const float rinv = rsq_fast(r2);
-            const RpyCoef c = rpy_select(r2, rinv, a, ia);
-            const float cr = c.crr * rinv * rinv;
-            // i-side: F_j + beta C_i . R_j
-            const float vx = fmaf(Ci.x, B.z, A.w), vy = fmaf(Ci.y, B.w, B.x), vz = fmaf(Ci.z, Cq.x, B.y);
-            const float sv = cr * (d.x * vx + d.y * vy + d.z * vz);
-            acc.x = fmaf(sv, d.x, fmaf(c.cii, vx, acc.x));
-            acc.y = fmaf(sv, d.y, fmaf(c.cii, vy, acc.y));
-            acc.z = fmaf(sv, d.z, fmaf(c.cii, vz, acc.z));
-            // j-side: F_i + beta C_j . R_i (e e^T is even in e)
-            const float wx = fmaf(Cq.y, Ri.x, Fi.x), wy = fmaf(Cq.z, Ri.y, Fi.y), wz = fmaf(Cq.w, Ri.z, Fi.z);
-            const float sw = cr * (d.x * wx + d.y * wy + d.z * wz);
-            g.x = wave_rol1(fmaf(sw, d.x, fmaf(c.cii, wx, g.x)));
-            g.y = wave_rol1(fmaf(sw, d.y, fmaf(c.cii, wy, g.y)));
-            g.z = wave_rol1(fmaf(sw, d.z, fmaf(c.cii, wz, g.z)));
+        auto sweep = [&](auto far) {
+            for (int s = 0; s < SYM_CHUNK; s++) {
+                const float4 A = j0[s], B = j1[s], Cq = j2[s];
+                float3 d = make_float3(A.x - xi.x, A.y - xi.y, A.z - xi.z);
+                if (PERIODIC) d = min_image_inv(d, box, ibox);
+                const float r2 = fmaxf(d.x * d.x + d.y * d.y + d.z * d.z, 1e-12f);
+                const float rinv = rsq_fast(r2);
+                const RpyCoef c = decltype(far)::value ? rpy_far(rinv, a) : rpy_select(r2, rinv, a, ia);
+                const float cr = c.crr * rinv * rinv;
+                // i-side: F_j + beta C_i . R_j
+                const float vx = fmaf(Ci.x, B.z, A.w), vy = fmaf(Ci.y, B.w, B.x), vz = fmaf(Ci.z, Cq.x, B.y);
+                const float sv = cr * (d.x * vx + d.y * vy + d.z * vz);
+                acc.x = fmaf(sv, d.x, fmaf(c.cii, vx, acc.x));
+                acc.y = fmaf(sv, d.y, fmaf(c.cii, vy, acc.y));
+                acc.z = fmaf(sv, d.z, fmaf(c.cii, vz, acc.z));
+                // j-side: F_i + beta C_j . R_i (e e^T is even in e)
+                const float wx = fmaf(Cq.y, Ri.x, Fi.x), wy = fmaf(Cq.z, Ri.y, Fi.y), wz = fmaf(Cq.w, Ri.z, Fi.z);
+                const float sw = cr * (d.x * wx + d.y * wy + d.z * wz);
+                g.x = wave_rol1(fmaf(sw, d.x, fmaf(c.cii, wx, g.x)));
+                g.y = wave_rol1(fmaf(sw, d.y, fmaf(c.cii, wy, g.y)));
+                g.z = wave_rol1(fmaf(sw, d.z, fmaf(c.cii, wz, g.z)));
+            }
+        };
+        // Open boundaries: when the bounding boxes of the two chunks lie more
+        // than 2a apart (with a margin over the rounding of r / a), every
+        // pair takes the far branch and the selection is skipped (same
+        // values).  Slots past the trajectory's end stand in as the chunk's
+        // first bead.
+        bool far = false;
+        if (!PERIODIC) {
+            const float4 xj = jd[0][vj ? l : 0];
+            float lo[6] = {xi.x, xi.y, xi.z, xj.x, xj.y, xj.z}, hi[6] = {xi.x, xi.y, xi.z, xj.x, xj.y, xj.z};
+#pragma unroll
+            for (int m = 1; m < SYM_CHUNK; m <<= 1)
+#pragma unroll
+                for (int k = 0; k < 6; k++) {
+                    lo[k] = fminf(lo[k], __shfl_xor(lo[k], m));
+                    hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], m));
+                }
+            const float gx = fmaxf(fmaxf(lo[3] - hi[0], lo[0] - hi[3]), 0.f);
+            const float gy = fmaxf(fmaxf(lo[4] - hi[1], lo[1] - hi[4]), 0.f);
+            const float gz = fmaxf(fmaxf(lo[5] - hi[2], lo[2] - hi[5]), 0.f);
+            const float lim = 2.002f * a;
+            far = __builtin_amdgcn_readfirstlane((int)(gx * gx + gy * gy + gz * gz > lim * lim)) != 0;
         }
+        if (far) sweep(std::true_type{});
+        else sweep(std::false_type{});
     }
     if (vi) partial[(size_t)(u.z / SYM_CHUNK) * chunk_stride + bi] = ei ? make_float4(0.f, 0.f, 0.f, 0.f)
                                                                          : make_float4(acc.x, acc.y, acc.z, 0.f);
