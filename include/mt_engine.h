@@ -186,6 +186,13 @@ int mte_get_mode(const mte_engine *e, int32_t *mode);
  * for both beads (the other chunk values evaluate it for i and for j). */
 #define MTE_TEA_SYMMETRIC 1
 
+/* tea_eps values: each pair's epsilon terms evaluated once for both beads,
+ * through a partial-sum buffer of 40 x ceil(n_max / 1024) bytes per bead
+ * (the default when it stays within 6 GiB), or evaluated for each bead of
+ * the pair (no buffer). */
+#define MTE_TEA_EPS_ONEPASS 1
+#define MTE_TEA_EPS_TWOSIDED 2
+
 typedef struct {
     int32_t resident_per_thread;  /* beads per thread of the resident kernel: 1, 2 or 3 */
     int32_t stream_block;         /* workgroup size of the streamed bead kernels: 64, 128 or 256 */
@@ -193,7 +200,8 @@ typedef struct {
                                      MTE_TEA_SYMMETRIC (0 without TEA) */
     int32_t cell_stencil;         /* cell-list rebuilds: 1 = 27 cells of side r, 2 = 125 cells of side r/2 */
     int32_t mode;                 /* MTE_MODE_RESIDENT or MTE_MODE_STREAMED (plain dynamics) */
-    int32_t reserved[3];
+    int32_t tea_eps;              /* TEA epsilon: MTE_TEA_EPS_ONEPASS or MTE_TEA_EPS_TWOSIDED (0 without TEA) */
+    int32_t reserved[2];
 } mte_tuning;
 
 int mte_get_tuning(const mte_engine *e, mte_tuning *t);

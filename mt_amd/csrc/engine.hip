@@ -98,6 +98,35 @@ hipError_t mte::tea_build_units(mte_engine *e, int chunk) {
     return hipStreamSynchronize(e->stream);
 }
 
+// Work units of the one-pass epsilon kernel: every pair (I group, J group >=
+// I group) of 1024-bead groups of a trajectory, and its partial buffer.  Left
+// out (the two-sided kernel computes epsilon) when the buffer would exceed
+// EPS_PART_MAX_BYTES.
+hipError_t mte::tea_build_eps_units(mte_engine *e) {
+    std::vector<int4> units;
+    int ngrp = 0;
+    for (int t = 0; t < e->ntraj; t++) {
+        const int n = (int)(e->off[t + 1] - e->off[t]);
+        const int g = (n + EPS_GROUP - 1) / EPS_GROUP;
+        ngrp = std::max(ngrp, g);
+        for (int i = 0; i < g; i++)
+            for (int j = i; j < g; j++) units.push_back(make_int4(t, i, j, 0));
+    }
+    const size_t part = (size_t)2 * ngrp * 5 * e->nb;
+    e->tune.tea_eps = MTE_TEA_EPS_TWOSIDED;
+    if (units.empty() || part * sizeof(float) > EPS_PART_MAX_BYTES) return hipSuccess;
+    hipError_t s;
+    if ((s = dalloc(e, &e->tea.eps_units, units.size())) != hipSuccess) return s;
+    if ((s = dalloc(e, &e->tea.eps_part, part)) != hipSuccess) return s;
+    if ((s = hipMemcpyAsync(e->tea.eps_units, units.data(), sizeof(int4) * units.size(), hipMemcpyHostToDevice,
+                            e->stream)) != hipSuccess)
+        return s;
+    e->tea.eps_nunits = (int)units.size();
+    e->tea.eps_ngrp = ngrp;
+    e->tune.tea_eps = MTE_TEA_EPS_ONEPASS;
+    return hipStreamSynchronize(e->stream);
+}
+
 // src/ran2.h:18-56 with state in a struct; positive seed skips the table
 // fill (shuffle table all zero, iy = 0, idum2 = 123456789).
 namespace {
@@ -314,7 +343,7 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     if (e->P.tea_on) {
         ALLOC(e->tea.ci, nb) ALLOC(e->tea.eps, nb) ALLOC(e->tea.beta, nt) ALLOC(e->tea.stats, 2 * nt)
         ALLOC(e->tea.status, 1) ALLOC(e->tea.R, nb)
-        if (tea_build_units(e, TEA_CHUNK_BEADS) != hipSuccess) {
+        if (tea_build_units(e, TEA_CHUNK_BEADS) != hipSuccess || tea_build_eps_units(e) != hipSuccess) {
             e->err = "mte_create: TEA work units";
             return bail(MTE_ERR_HIP);
         }
@@ -1144,6 +1173,14 @@ int mte_set_tuning(mte_engine *e, const mte_tuning *t) {
             return fail(e, MTE_ERR_ARG, "mte_set_tuning: tea_chunk must be 64, 128, 256 or MTE_TEA_SYMMETRIC");
         if (!e->P.tea_on) return fail(e, MTE_ERR_ARG, "mte_set_tuning: tea_chunk needs TEA");
         if (t->tea_chunk != e->tea.unit_chunk) HIPCHK(tea_build_units(e, t->tea_chunk));
+    }
+    if (t->tea_eps) {
+        if (t->tea_eps != MTE_TEA_EPS_ONEPASS && t->tea_eps != MTE_TEA_EPS_TWOSIDED)
+            return fail(e, MTE_ERR_ARG, "mte_set_tuning: tea_eps must be MTE_TEA_EPS_ONEPASS or MTE_TEA_EPS_TWOSIDED");
+        if (!e->P.tea_on) return fail(e, MTE_ERR_ARG, "mte_set_tuning: tea_eps needs TEA");
+        if (t->tea_eps == MTE_TEA_EPS_ONEPASS && !e->tea.eps_units)
+            return fail(e, MTE_ERR_ARG, "mte_set_tuning: the one-pass epsilon buffer exceeds its 6 GiB cap here");
+        e->tune.tea_eps = t->tea_eps;
     }
     if (t->cell_stencil) {
         if (t->cell_stencil != 1 && t->cell_stencil != 2)

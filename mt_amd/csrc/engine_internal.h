@@ -31,7 +31,19 @@ struct TeaBuffers {
     int *nchunks = nullptr;    // per trajectory
     int max_chunks = 0;
     float4 *partial = nullptr; // max_chunks x nb pair partial sums
+    // epsilon with each pair evaluated once (k_tea_eps_sym): work units
+    // (traj, I group, J group >= I group) of 1024-bead groups, and the
+    // partial sums [side][group][5][nb]; null when they would not fit
+    int4 *eps_units = nullptr;
+    int eps_nunits = 0;
+    int eps_ngrp = 0;          // most groups of one trajectory
+    float *eps_part = nullptr;
 };
+
+// Beads per group of the one-pass epsilon kernel (16 chunks of 64) and the
+// largest partial-sum buffer it may take.
+constexpr int EPS_GROUP = 1024;
+constexpr size_t EPS_PART_MAX_BYTES = size_t(6) << 30;
 
 // j-chunk length of the TEA pair kernel.
 constexpr int TEA_CHUNK_BEADS = 128;
@@ -163,6 +175,7 @@ size_t mlp_image_bytes();
 hipError_t build_cells(mte_engine *e, float side, int K);
 float candidate_radius(const mte_params &P);
 hipError_t launch_tea_eps(mte_engine *e);
+hipError_t tea_build_eps_units(mte_engine *e);
 hipError_t launch_tea_pairs(mte_engine *e);
 hipError_t launch_tea_finish(mte_engine *e, const float *noise_step);
 hipError_t launch_resident(mte_engine *e, int nsteps, const float *noise);

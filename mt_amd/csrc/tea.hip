@@ -38,6 +38,12 @@ __device__ __forceinline__ RpyCoef rpy_select(float r2, float rinv, float a, flo
     return c;
 }
 
+// Lane l of the wave receives lane (l + 1) mod 64's value (all lanes active).
+__device__ __forceinline__ float wave_rol1(float v) {
+    const int x = __float_as_int(v);
+    return __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x134, 0xF, 0xF, false));
+}
+
 // The far branch of rpy_select alone (r > 2a).
 __device__ __forceinline__ RpyCoef rpy_far(float rinv, float a) {
     const float u1 = a * rinv, u2 = u1 * u1;
@@ -118,6 +124,141 @@ __global__ void __launch_bounds__(TEA_TILE)
         ci[b] = make_float4(scc + sx2, scc + sy2, scc + sz2, 0.f);
         eps[b] = sall;
     }
+}
+
+// Epsilon partials and ci with each pair evaluated once.  The five sums of
+// k_tea_eps are even in e (cii^2, crr (2 cii + crr) e_a^2, 3 cii + crr
+// (ex + ey + ez)^2), so a pair adds the same terms to both beads.  Work unit:
+// (trajectory, 1024-bead group I, group J >= I), one workgroup of 16 waves;
+// wave w takes chunk c_I = 16 I + w (64 beads) against each 64-bead chunk c_J
+// of group J in turn, staged in LDS:
+//   c_J > c_I  both sides, lane l meeting j = (l + s) mod 64 at step s, the
+//              j-side sums carried along by a one-lane rotation (wave_rol1),
+//   c_J = c_I  one-sided over the chunk, the j == i term taken out after,
+//   c_J < c_I  nothing (that pair of chunks is the wave c_J's).
+// The waves' j-side sums of a chunk are added in wave order in LDS.  Output
+// (deterministic, no atomics): i-side sums of a bead of group H into
+// part[0][J][.] for J >= H, j-side sums into part[1][I][.] for I <= H; the
+// pair weight is 1 for two non-extra beads and 0 otherwise.
+constexpr int EPS_CHUNKS = EPS_GROUP / 64;
+template <bool PERIODIC>
+__global__ void __launch_bounds__(EPS_GROUP)
+    k_tea_eps_sym(mte_params P, DevTopo T, DevState S, const int4 *units, int ngrp, float *part) {
+    __shared__ float4 jx[2 * 64];               // chunk c_J: x, weight; twice, no wrap
+    __shared__ float jred[EPS_CHUNKS][5][64];   // j-side sums of each wave
+    const int4 u = units[blockIdx.x];
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    const int base = T.tbase[u.x], n = T.tsize[u.x], nb = T.nb;
+    const int nch = (n + 63) / 64;
+    const float box = T.tbox[u.x], ibox = inv_side(box);
+    const float a = P.tea_a, ia = rcp_fast(P.tea_a);
+    const int cI = u.y * EPS_CHUNKS + w;
+    const int li = cI * 64 + l;
+    const bool vi = li < n;
+    float3 xi = make_float3(0.f, 0.f, 0.f);
+    float wi = 0.f;
+    if (vi) {
+        xi = f3(S.xyz[base + li]);
+        wi = T.extra[base + li] ? 0.f : 1.f;
+    }
+    float ai[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    const int c0 = u.z * EPS_CHUNKS, c1 = min(c0 + EPS_CHUNKS, nch);
+    for (int cJ = c0; cJ < c1; cJ++) {
+        if (threadIdx.x < 128) {
+            const int lj = cJ * 64 + (threadIdx.x % 64);
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (lj < n) {
+                v = S.xyz[base + lj];
+                v.w = T.extra[base + lj] ? 0.f : 1.f;
+            }
+            jx[threadIdx.x] = v;
+        }
+        __syncthreads();
+        float gj[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+        // one pair's five terms into the sums t (and u when both sides)
+        auto pair = [&](float4 xj, float *t, float *t2) {
+            float3 d = make_float3(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z);
+            if (PERIODIC) d = min_image_inv(d, box, ibox);
+            const float r2 = fmaxf(d.x * d.x + d.y * d.y + d.z * d.z, 1e-12f);
+            const float rinv = rsq_fast(r2);
+            const RpyCoef c = rpy_select(r2, rinv, a, ia);
+            const float ww = wi * xj.w;
+            const float ex = d.x * rinv, ey = d.y * rinv, ez = d.z * rinv;
+            const float cw = ww * c.cii;
+            const float k = ww * c.crr * fmaf(2.f, c.cii, c.crr);
+            const float se = ex + ey + ez;
+            const float tall = fmaf(ww * c.crr, se * se, 3.f * cw);
+            const float qx = ex * ex, qy = ey * ey, qz = ez * ez;
+            t[0] = fmaf(cw, c.cii, t[0]);
+            t[1] = fmaf(k, qx, t[1]);
+            t[2] = fmaf(k, qy, t[2]);
+            t[3] = fmaf(k, qz, t[3]);
+            t[4] += tall;
+            if (t2) {
+                t2[0] = fmaf(cw, c.cii, t2[0]);
+                t2[1] = fmaf(k, qx, t2[1]);
+                t2[2] = fmaf(k, qy, t2[2]);
+                t2[3] = fmaf(k, qz, t2[3]);
+                t2[4] += tall;
+            }
+        };
+        if (cI < nch && cJ > cI) {
+            for (int s = 0; s < 64; s++) {
+                pair(jx[l + s], ai, gj);
+#pragma unroll
+                for (int k = 0; k < 5; k++) gj[k] = wave_rol1(gj[k]);
+            }
+        } else if (cI < nch && cJ == cI) {
+            const int jn = min(64, n - cJ * 64);
+            for (int q = 0; q < jn; q++) pair(jx[q], ai, nullptr);
+            if (wi != 0.f) {
+                // the j == i slot entered as a coincident bead (d = 0):
+                // cii^2 into the first sum and 3 cii into the last, taken out
+                const float r2 = 1e-12f, rinv = rsq_fast(r2);
+                const float cii = rpy_select(r2, rinv, a, ia).cii;
+                ai[0] = fmaf(-cii, cii, ai[0]);
+                ai[4] = fmaf(-3.f, cii, ai[4]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 5; k++) jred[w][k][l] = gj[k];
+        __syncthreads();
+        if (threadIdx.x < 5 * 64) {
+            const int k = threadIdx.x / 64, q = threadIdx.x % 64;
+            const int lj = cJ * 64 + q;
+            if (lj < n) {
+                float sum = 0.f;
+                for (int v = 0; v < EPS_CHUNKS; v++) sum += jred[v][k][q];
+                part[(((size_t)1 * ngrp + u.y) * 5 + k) * nb + base + lj] = sum;
+            }
+        }
+        __syncthreads();
+    }
+    if (vi) {
+#pragma unroll
+        for (int k = 0; k < 5; k++) part[(((size_t)0 * ngrp + u.z) * 5 + k) * nb + base + li] = ai[k];
+    }
+}
+
+// Per bead: the group sums of k_tea_eps_sym added in fixed order into ci and
+// eps, as k_tea_eps writes them.
+__global__ void __launch_bounds__(256)
+    k_tea_eps_reduce(DevTopo T, int ngrp, const float *part, float4 *ci, float *eps) {
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= T.nb) return;
+    const int t = T.tid[b], n = T.tsize[t];
+    const int h = (b - T.tbase[t]) / EPS_GROUP, g = (n + EPS_GROUP - 1) / EPS_GROUP;
+    const int nb = T.nb;
+    float s[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        float v = 0.f;
+        for (int j = h; j < g; j++) v += part[(((size_t)0 * ngrp + j) * 5 + k) * nb + b];
+        for (int i = 0; i <= h; i++) v += part[(((size_t)1 * ngrp + i) * 5 + k) * nb + b];
+        s[k] = v;
+    }
+    ci[b] = make_float4(s[0] + s[1], s[0] + s[2], s[0] + s[3], 0.f);
+    eps[b] = s[4];
 }
 
 // FP64 epsilon / beta per trajectory, one workgroup each.
@@ -252,12 +393,6 @@ __global__ void __launch_bounds__(TEA_TILE)
 // k_tea_pairs.  j slots past the trajectory's end and extra beads carry zero
 // F and R, so their terms vanish; their sums are not stored (extra beads
 // store zero).
-// Lane l of the wave receives lane (l + 1) mod 64's value (all lanes active).
-__device__ __forceinline__ float wave_rol1(float v) {
-    const int x = __float_as_int(v);
-    return __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x134, 0xF, 0xF, false));
-}
-
 constexpr int SYM_CHUNK = 64;
 template <bool PERIODIC>
 __global__ void __launch_bounds__(SYM_CHUNK)
@@ -452,7 +587,16 @@ __global__ void __launch_bounds__(TEA_TILE)
 }
 
 hipError_t launch_tea_eps(mte_engine *e) {
-    if (e->periodic)
+    if (e->tune.tea_eps == MTE_TEA_EPS_ONEPASS) {
+        if (e->periodic)
+            hipLaunchKernelGGL(k_tea_eps_sym<true>, dim3(e->tea.eps_nunits), dim3(EPS_GROUP), 0, e->stream, e->P,
+                               e->T, e->S[e->cur], e->tea.eps_units, e->tea.eps_ngrp, e->tea.eps_part);
+        else
+            hipLaunchKernelGGL(k_tea_eps_sym<false>, dim3(e->tea.eps_nunits), dim3(EPS_GROUP), 0, e->stream, e->P,
+                               e->T, e->S[e->cur], e->tea.eps_units, e->tea.eps_ngrp, e->tea.eps_part);
+        hipLaunchKernelGGL(k_tea_eps_reduce, dim3((e->nb + 255) / 256), dim3(256), 0, e->stream, e->T,
+                           e->tea.eps_ngrp, e->tea.eps_part, e->tea.ci, e->tea.eps);
+    } else if (e->periodic)
         hipLaunchKernelGGL(k_tea_eps<true>, dim3(e->ntiles), dim3(TEA_TILE), 0, e->stream, e->P, e->T,
                            e->S[e->cur], e->tiles, e->tea.ci, e->tea.eps);
     else

@@ -7,7 +7,7 @@ injected identical noise compared against the FP32 oracle.
 import numpy as np
 import pytest
 
-from mt_amd.engine import Engine, MODE_STREAMED, TEA_SYMMETRIC
+from mt_amd.engine import Engine, MODE_STREAMED, TEA_EPS_ONEPASS, TEA_EPS_TWOSIDED, TEA_SYMMETRIC
 from oracle.oracle import Oracle
 from tests.helpers import ANG, XYZ, base_params, c1_batch, noise, rel_err, uneven_batch
 
@@ -205,8 +205,12 @@ def test_tea_batch_with_extra_beads_matches_oracle(periodic, chunk):
     if periodic:
         b.box = np.array([130.0, 100.0, 180.0], np.float32)   # each larger than its system
     eng, orc = _pair(p, b, precision="f64")
+    # the one-sided chunks run beside the two-sided epsilon, the symmetric pair
+    # blocks beside the one-pass epsilon (the default)
     if chunk:
         eng.set_tuning(tea_chunk=chunk)
+    else:
+        eng.set_tuning(tea_eps=TEA_EPS_TWOSIDED)
     rng = np.random.default_rng(8)
     nz = noise(rng, 6, b.nb)
     eng.set_noise(nz)
@@ -217,6 +221,37 @@ def test_tea_batch_with_extra_beads_matches_oracle(periodic, chunk):
     np.testing.assert_allclose(beta, orc.beta_t, rtol=1e-5)
     g = eng.get_state()
     assert np.abs(g[:, :3] - orc.b.coords[:, :3]).max() < 1e-4
+    eng.close()
+
+
+@pytest.mark.parametrize("method", [TEA_EPS_ONEPASS, TEA_EPS_TWOSIDED])
+@pytest.mark.parametrize("periodic", [False, True])
+def test_tea_eps_over_several_bead_groups(periodic, method):
+    """Epsilon and ci of trajectories spanning several 1024-bead groups of the
+    one-pass epsilon kernel (1040 beads: a group and 16 beads; 2600 beads:
+    three groups), against the oracle, and the steps they drive; the same
+    with the two-sided epsilon kernel."""
+    print(f"case TEA eps over 1040 + 2600 beads, periodic={periodic}, tea_eps={method}", flush=True)
+    from mt_amd import lattice, params as P, topology
+    p = P.derive(dict(P.TEA, tea_epsilon_freq=2))
+    rng = np.random.default_rng(12)
+    atoms = [lattice.microtubule(40), lattice.microtubule(100)]
+    b = topology.make_batch(atoms, [lattice.coords7(a, 0.05, rng) for a in atoms])
+    if periodic:
+        b.box = np.array([400.0, 900.0], np.float32)   # each longer than its tube (320, 800 nm)
+    eng, orc = _pair(p, b, precision="f64")
+    assert eng.tuning()["tea_eps"] == TEA_EPS_ONEPASS
+    eng.set_tuning(tea_eps=method)
+    nz = noise(rng, 3, b.nb)
+    eng.set_noise(nz)
+    eng.run(3)
+    assert orc.run(0, 3, noise=nz) == 0
+    eps, beta = eng.tea_stats()
+    np.testing.assert_allclose(eps, orc.eps_t, rtol=1e-5)
+    np.testing.assert_allclose(beta, orc.beta_t, rtol=1e-5)
+    dx = np.abs(eng.get_state()[:, :3] - orc.b.coords[:, :3]).max()
+    print("max |dx|", dx, flush=True)
+    assert dx < 1e-4
     eng.close()
 
 

@@ -104,6 +104,8 @@ def test_set_tuning_rejects_unusable_shapes():
         e.set_tuning(resident_per_thread=1)     # 1040 beads do not fit one per thread
     with pytest.raises(EngineError):
         e.set_tuning(tea_chunk=128)              # no TEA in this run
+    with pytest.raises(EngineError):
+        e.set_tuning(tea_eps=1)                  # no TEA in this run
     e.set_tuning(resident_per_thread=3)
     assert e.tuning()["resident_per_thread"] == 3
     e.close()
