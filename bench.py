@@ -165,8 +165,10 @@ def tea_bench(steps, warmup, cpu=True):
                         "frac": flops / avg / 1e12 / 157.3, "traffic": read_tea_traffic(tuned["tea_chunk"]),
                         "kernel": "k_tea_pairs", "avg_launch_ms": avg * 1e3, "alg_flops_per_launch": flops,
                         "note": "algorithmic work 36 B(B-1) flops per trajectory-step; the kernel evaluates it "
-                                "on VALU (FP32, ~41 instructions per block, at the non-packed issue rate); the "
-                                "peak quoted is the FP32 figure, which counts packed FMAs"},
+                                "on VALU (FP32, at the non-packed issue rate); with tea_chunk 1 each pair's RPY "
+                                "block is evaluated once for both beads (~24 instructions per block against ~41 "
+                                "one-sided), so the achieved figure counts each block's algorithmic flops on "
+                                "both sides; the peak quoted is the FP32 figure, which counts packed FMAs"},
            "force_error_vs_fp64": tea_error_vs_fp64()}
     if cpu:
         from oracle.oracle import Oracle, generate_seeds, max_threads
