@@ -57,15 +57,13 @@ __device__ __forceinline__ Own make_own(float4 xyz, float4 ang, float4 fn, float
 __device__ __forceinline__ void bending(const mte_params &P, float rm, float th0, const Own &I, float4 aj,
                                         float F[6]) {
     const float psiji = aj.z - I.psi, thetaji = aj.y - I.theta, fiji = aj.x - I.fi;
-    if (rm > 0.f) {
-        F[5] += P.B_psi * sin_fast_near0(psiji - P.psi_0);
-        F[3] += P.B_fi * sin_fast_near0(fiji - P.fi_0);
-        F[4] += P.B_theta * sin_fast_near0(thetaji - th0);
-    } else {
-        F[5] -= P.B_psi * sin_fast_near0(-psiji - P.psi_0);
-        F[3] -= P.B_fi * sin_fast_near0(-fiji - P.fi_0);
-        F[4] -= P.B_theta * sin_fast_near0(-thetaji - th0);
-    }
+    // rm > 0: F += B sin(d - d0); rm < 0: F -= B sin(-d - d0).  One branch-free
+    // form with s = +-1 (exact sign flips: the same values), since the two
+    // monomers of a dimer sit in neighbouring lanes and take opposite signs.
+    const float sg = rm > 0.f ? 1.f : -1.f;
+    F[5] += sg * P.B_psi * sin_fast_near0(sg * psiji - P.psi_0);
+    F[3] += sg * P.B_fi * sin_fast_near0(sg * fiji - P.fi_0);
+    F[4] += sg * P.B_theta * sin_fast_near0(sg * thetaji - th0);
 }
 
 // Per-bead flags packed into xyz.w when a trajectory's state is staged in LDS.
