@@ -135,9 +135,8 @@ __device__ __forceinline__ void lj_forces(const mte_params &P, const Acc &A, con
     const int nlj = Ls.nlj();
     const float s0 = P.ljscale * P.ljsigma6;
     float fx = F[0], fy = F[1], fz = F[2];   // same accumulation order, scalar registers
-    for (int k = 0; k < nlj; k++) {
-        const int j = Ls.lj(k);
-        float3 d = sub3(I.r, f3(A.xyz(j)));
+    auto pair = [&](float4 xj) {
+        float3 d = sub3(I.r, f3(xj));
         if (PERIODIC) d = min_image(d, I.box);
         const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
         const float inv = rcp_fast(r2);
@@ -145,7 +144,20 @@ __device__ __forceinline__ void lj_forces(const mte_params &P, const Acc &A, con
         fx += s * d.x;
         fy += s * d.y;
         fz += s * d.z;
+    };
+    // four entries and their positions requested before the first is used
+    // (one wait for the four gathers rather than one each), then summed in
+    // list order
+    int k = 0;
+    for (; k + 4 <= nlj; k += 4) {
+        const int j0 = Ls.lj(k), j1 = Ls.lj(k + 1), j2 = Ls.lj(k + 2), j3 = Ls.lj(k + 3);
+        const float4 x0 = A.xyz(j0), x1 = A.xyz(j1), x2 = A.xyz(j2), x3 = A.xyz(j3);
+        pair(x0);
+        pair(x1);
+        pair(x2);
+        pair(x3);
     }
+    for (; k < nlj; k++) pair(A.xyz(Ls.lj(k)));
     F[0] = fx;
     F[1] = fy;
     F[2] = fz;
