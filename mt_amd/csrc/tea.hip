@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(EPS_GROUP)
         }
         __syncthreads();
         float gj[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-        // one pair's five terms into the sums t (and u when both sides)
+        // one pair's five terms into the sums t (and t2 when both sides)
         auto pair = [&](float4 xj, float *t, float *t2) {
             float3 d = make_float3(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z);
             if (PERIODIC) d = min_image_inv(d, box, ibox);
@@ -380,7 +380,8 @@ __global__ void __launch_bounds__(TEA_TILE)
 
 // Pair sums with each (i, j) RPY block evaluated once for both beads.  Work
 // unit: (trajectory, 64-bead chunk I, 64-bead chunk J >= I), one
-// single-wave workgroup each, so the dispatcher balances units wave by wave.  D_hat_ji = D_hat_ij, so the block serves
+// single-wave workgroup each, so the dispatcher balances units wave by
+// wave.  D_hat_ji = D_hat_ij, so the block serves
 //   i-side  D_hat_ij (F_j + beta C_i . R_j)  into partial slot J of bead i,
 //   j-side  D_hat_ij (F_i + beta C_j . R_i)  into partial slot I of bead j,
 // and slot c of a bead holds the sum over chunk c, as k_tea_pairs<64> writes
