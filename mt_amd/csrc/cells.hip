@@ -86,11 +86,11 @@ constexpr int SCAN_T = 256, SCAN_PER = CELL_SCAN_SPAN / SCAN_T;
 
 __global__ void __launch_bounds__(SCAN_T) k_scan_partial(const int *count, int n, int *bsum) {
     __shared__ int red[SCAN_T];
-    const int base = blockIdx.x * CELL_SCAN_SPAN + threadIdx.x * SCAN_PER;
+    const int g0 = blockIdx.x * CELL_SCAN_SPAN + threadIdx.x;   // lanes read consecutive buckets
     int s = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_PER; k++)
-        if (base + k < n) s += count[base + k];
+        if (g0 + k * SCAN_T < n) s += count[g0 + k * SCAN_T];
     red[threadIdx.x] = s;
     __syncthreads();
     for (int d = SCAN_T / 2; d > 0; d >>= 1) {
@@ -123,16 +123,28 @@ __global__ void __launch_bounds__(1024) k_scan_top(int *bsum, int nblk) {
 }
 
 // Bucket starts (for the scatter), packed (start, count) ranges (for the
-// searches); clears the scatter cursors.
+// searches); clears the scatter cursors.  The span's counts pass through LDS
+// so that global loads and stores are coalesced (lane-consecutive buckets)
+// while each thread scans a contiguous run of SCAN_PER; one pad word per 16
+// keeps a run's reads off shared banks.
+__device__ __forceinline__ int scan_pad(int i) { return i + (i >> 4); }
 __global__ void __launch_bounds__(SCAN_T) k_scan_apply(CellTables C) {
     __shared__ int part[SCAN_T];
+    __shared__ int cnt[CELL_SCAN_SPAN + CELL_SCAN_SPAN / 16], st[CELL_SCAN_SPAN + CELL_SCAN_SPAN / 16];
     const int n = C.total;
-    const int base = blockIdx.x * CELL_SCAN_SPAN + threadIdx.x * SCAN_PER;
+    const int g0 = blockIdx.x * CELL_SCAN_SPAN;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; k++) {
+        const int idx = k * SCAN_T + threadIdx.x;
+        cnt[scan_pad(idx)] = g0 + idx < n ? C.count[g0 + idx] : 0;
+    }
+    __syncthreads();
+    const int run = threadIdx.x * SCAN_PER;
     int v[SCAN_PER];
     int s = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_PER; k++) {
-        v[k] = base + k < n ? C.count[base + k] : 0;
+        v[k] = cnt[scan_pad(run + k)];
         s += v[k];
     }
     part[threadIdx.x] = s;
@@ -146,12 +158,19 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_apply(CellTables C) {
     int acc = C.bsum[blockIdx.x] + part[threadIdx.x] - s;
 #pragma unroll
     for (int k = 0; k < SCAN_PER; k++) {
-        if (base + k < n) {
-            C.start[base + k] = acc;
-            C.fill[base + k] = 0;
-            C.range[base + k] = make_int2(acc, v[k]);
-        }
+        st[scan_pad(run + k)] = acc;
         acc += v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; k++) {
+        const int idx = k * SCAN_T + threadIdx.x;
+        if (g0 + idx < n) {
+            const int a = st[scan_pad(idx)];
+            C.start[g0 + idx] = a;
+            C.fill[g0 + idx] = 0;
+            C.range[g0 + idx] = make_int2(a, cnt[scan_pad(idx)]);
+        }
     }
 }
 

@@ -484,7 +484,9 @@ __global__ void __launch_bounds__(SYM_CHUNK)
                 const float4 A = j0[s], B = j1[s], Cq = j2[s];
                 float3 d = make_float3(A.x - xi.x, A.y - xi.y, A.z - xi.z);
                 if (PERIODIC) d = min_image_inv(d, box, ibox);
-                const float r2 = fmaxf(d.x * d.x + d.y * d.y + d.z * d.z, 1e-12f);
+                // (the clamp is the identity beyond 2a)
+                const float r2s = d.x * d.x + d.y * d.y + d.z * d.z;
+                const float r2 = decltype(far)::value ? r2s : fmaxf(r2s, 1e-12f);
                 const float rinv = rsq_fast(r2);
                 const RpyCoef c = decltype(far)::value ? rpy_far(rinv, a) : rpy_select(r2, rinv, a, ia);
                 const float cr = c.crr * rinv * rinv;
