@@ -175,13 +175,18 @@ __global__ void __launch_bounds__(EPS_GROUP)
         }
         __syncthreads();
         float gj[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-        // one pair's five terms into the sums t (and t2 when both sides)
+        // one pair's five terms into the sums t (and t2 when both sides); a
+        // wave with no pair inside 2a (margin over the rounding of r / a)
+        // takes the far branch alone, which is the branch rpy_select picks
+        const float near2 = (2.002f * a) * (2.002f * a);
         auto pair = [&](float4 xj, float *t, float *t2) {
             float3 d = make_float3(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z);
             if (PERIODIC) d = min_image_inv(d, box, ibox);
             const float r2 = fmaxf(d.x * d.x + d.y * d.y + d.z * d.z, 1e-12f);
             const float rinv = rsq_fast(r2);
-            const RpyCoef c = rpy_select(r2, rinv, a, ia);
+            RpyCoef c;
+            if (__any(r2 <= near2)) c = rpy_select(r2, rinv, a, ia);
+            else c = rpy_far(rinv, a);
             const float ww = wi * xj.w;
             const float ex = d.x * rinv, ey = d.y * rinv, ez = d.z * rinv;
             const float cw = ww * c.cii;
