@@ -556,14 +556,28 @@ __global__ void __launch_bounds__(TEA_TILE)
     // C_i = 1/sqrt(1 + beta^2 ci) per row (eq. 19)
     const float cx = rsq_fast(1.f + w * cc.x), cy = rsq_fast(1.f + w * cc.y), cz = rsq_fast(1.f + w * cc.z);
     float fx = fmaf(Ri.x, cx, Fi.x), fy = fmaf(Ri.y, cy, Fi.y), fz = fmaf(Ri.z, cz, Fi.z);
+    // the bead's state is requested before the partial sums are added
+    float4 x = Sin.xyz[b], an = Sin.ang[b];
     const int nc = nchunks[t];
-    for (int c = 0; c < nc; c++) {
+    // partials four at a time: four loads in flight, then added in order
+    int c = 0;
+    for (; c + 4 <= nc; c += 4) {
+        float4 pp[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) pp[u] = partial[(size_t)(c + u) * chunk_stride + b];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            fx += pp[u].x;
+            fy += pp[u].y;
+            fz += pp[u].z;
+        }
+    }
+    for (; c < nc; c++) {
         const float4 pp = partial[(size_t)c * chunk_stride + b];
         fx += pp.x;
         fy += pp.y;
         fz += pp.z;
     }
-    float4 x = Sin.xyz[b], an = Sin.ang[b];
     float4 ra;
     if (noise) {
         const float *q = noise + 6 * (size_t)b;
