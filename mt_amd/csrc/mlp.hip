@@ -464,12 +464,40 @@ struct MlpStepArgs {
 // so cluster centre-of-mass diffusion follows TEA's.  A prediction whose
 // covariance is not positive definite falls back to C^2 for the bead (counted).
 // Noise draws as the plain integrator (mobile beads only).
-__device__ __forceinline__ void mlp_integrate(const MlpStepArgs &A, int b, const float y[MLP_OUT]) {
+// What the integrator reads of bead b, requested before the tile's network
+// runs so that the loads complete behind the matrix work.
+struct MlpIn {
+    float4 x, an, n4, p1, p2, F, fa, cc;
+    uint4 s0, s1;
+    float bt;
+    bool mobile;
+};
+__device__ __forceinline__ MlpIn mlp_load(const MlpStepArgs &A, int b) {
+    MlpIn v;
+    const DevTopo &T = A.T;
+    v.x = A.Sin.xyz[b];
+    v.an = A.Sin.ang[b];
+    v.n4 = A.Sin.fn[b];
+    v.p1 = A.Sin.fp1[b];
+    v.p2 = A.Sin.fp2[b];
+    v.F = A.fxyz[b];
+    v.fa = A.fang[b];
+    v.cc = A.ci[b];
+    v.bt = A.beta[T.tid[b]];
+    v.mobile = !T.fixed[b] && !T.extra[b];
+    if (!A.noise) {
+        v.s0 = A.seeds[b];
+        v.s1 = A.seeds[b + T.nb];
+    }
+    return v;
+}
+
+__device__ __forceinline__ void mlp_integrate(const MlpStepArgs &A, int b, const float y[MLP_OUT], const MlpIn &in) {
     const mte_params &P = A.P;
     const DevTopo &T = A.T;
-    float4 x = A.Sin.xyz[b], an = A.Sin.ang[b];
-    float4 n4 = A.Sin.fn[b], p1 = A.Sin.fp1[b], p2 = A.Sin.fp2[b];
-    const bool mobile = !T.fixed[b] && !T.extra[b];
+    float4 x = in.x, an = in.an;
+    float4 n4 = in.n4, p1 = in.p1, p2 = in.p2;
+    const bool mobile = in.mobile;
     if (mobile) {
         if (A.yout)
             for (int q = 0; q < MLP_OUT; q++) A.yout[(size_t)b * MLP_OUT + q] = y[q];
@@ -478,8 +506,8 @@ __device__ __forceinline__ void mlp_integrate(const MlpStepArgs &A, int b, const
         for (int r = 0; r < 3; r++)
 #pragma unroll
             for (int c = 0; c < 3; c++) G[3 * r + c] = 0.5f * (y[3 * r + c] + y[3 * c + r]);
-        const float bt = A.beta[T.tid[b]];
-        const float4 cc = A.ci[b];
+        const float bt = in.bt;
+        const float4 cc = in.cc;
         const float Cd[3] = {rsq_fast(1.f + bt * bt * cc.x), rsq_fast(1.f + bt * bt * cc.y),
                              rsq_fast(1.f + bt * bt * cc.z)};
         float Sg[9], Lc[6];
@@ -497,14 +525,14 @@ __device__ __forceinline__ void mlp_integrate(const MlpStepArgs &A, int b, const
             rx = make_float4(nz[0], nz[1], nz[2], 0.f);
             ra = make_float4(nz[3], nz[4], nz[5], 0.f);
         } else {
-            uint4 s = A.seeds[b];
+            uint4 s = in.s0;
             rx = normals4(s);
             A.seeds[b] = s;
-            s = A.seeds[b + T.nb];
+            s = in.s1;
             ra = normals4(s);
             A.seeds[b + T.nb] = s;
         }
-        const float4 F = A.fxyz[b];
+        const float4 F = in.F;
         const float dtg = P.dt / P.gammaR;
         const float d0 = F.x + G[0] * F.x + G[1] * F.y + G[2] * F.z;
         const float d1 = F.y + G[3] * F.x + G[4] * F.y + G[5] * F.z;
@@ -513,7 +541,7 @@ __device__ __forceinline__ void mlp_integrate(const MlpStepArgs &A, int b, const
         x.x += dtg * d0 + P.varR * n0;
         x.y += dtg * d1 + P.varR * n1;
         x.z += dtg * d2 + P.varR * n2;
-        const float4 fa = A.fang[b];
+        const float4 fa = in.fa;
         const float dta = P.dt / (P.gammaTheta * P.alpha), dtt = P.dt / P.gammaTheta;
         const float vta = P.varTheta * sqrtf(P.freeze_temp / P.alpha);
         an.x += dta * fa.x + vta * ra.x;
@@ -541,10 +569,13 @@ __global__ void __launch_bounds__(NET_THREADS)
     const int n = A.T.nb;
     const int ntiles = (n + NET_TILE - 1) / NET_TILE;
     for (int tile = blockIdx.x * NET_WAVES + wave; tile < ntiles; tile += gridDim.x * NET_WAVES) {
+        const int b = tile * NET_TILE + lane;
+        const bool integ = lane < NET_TILE && b < n;
+        MlpIn in{};
+        if (integ) in = mlp_load(A, b);
         float y[MLP_OUT];
         net_tile(net_lds, bias, feat, n, tile * NET_TILE, lane, y);
-        const int b = tile * NET_TILE + lane;
-        if (lane < NET_TILE && b < n) mlp_integrate(A, b, y);
+        if (integ) mlp_integrate(A, b, y, in);
     }
 }
 
