@@ -140,7 +140,7 @@ def tea_bench(steps, warmup, cpu=True):
     b = topology.replicate(lattice.microtubule(40), 64, 0.05, 77)
     b.coords = lattice.wrap_angles(b.coords)
     eng = Engine(p, b, max_lj=48, mode=MODE_STREAMED)
-    tuned, report = eng.autotune(20)
+    tuned, report = eng.autotune(100)   # 100-step trials (~8 ms each): the shapes differ by ~10%
     print("[bench] tea autotune: " + "; ".join(report), file=sys.stderr, flush=True)
     eng.run(warmup)
     # timed region: graph-replayed step segments
