@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(256)
     const int3 c = cell_of(f3(S.xyz[b]), box, g);
     const int bucket = C.toff[t] + (int)(cell_hash(c.x, c.y, c.z) & (uint32_t)(C.tsize[t] - 1));
     C.cell[b] = make_int4(c.x, c.y, c.z, bucket);
-    atomicAdd(&C.count[bucket], 1);
+    C.rank[b] = atomicAdd(&C.count[bucket], 1);   // the bead's slot within its bucket
 }
 
 // Exclusive scan of the bucket counts over ALL trajectories at once.  Each
@@ -122,8 +122,8 @@ __global__ void __launch_bounds__(1024) k_scan_top(int *bsum, int nblk) {
     }
 }
 
-// Bucket starts (for the scatter), packed (start, count) ranges (for the
-// searches); clears the scatter cursors.  The span's counts pass through LDS
+// Packed (first slot, count) ranges of the buckets (for the scatter and the
+// searches).  The span's counts pass through LDS
 // so that global loads and stores are coalesced (lane-consecutive buckets)
 // while each thread scans a contiguous run of SCAN_PER; one pad word per 16
 // keeps a run's reads off shared banks.
@@ -165,12 +165,7 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_apply(CellTables C) {
 #pragma unroll
     for (int k = 0; k < SCAN_PER; k++) {
         const int idx = k * SCAN_T + threadIdx.x;
-        if (g0 + idx < n) {
-            const int a = st[scan_pad(idx)];
-            C.start[g0 + idx] = a;
-            C.fill[g0 + idx] = 0;
-            C.range[g0 + idx] = make_int2(a, cnt[scan_pad(idx)]);
-        }
+        if (g0 + idx < n) C.range[g0 + idx] = make_int2(st[scan_pad(idx)], cnt[scan_pad(idx)]);
     }
 }
 
@@ -179,8 +174,7 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_apply(CellTables C) {
 __global__ void __launch_bounds__(256) k_cell_scatter(DevTopo T, DevState S, CellTables C) {
     const int b = blockIdx.x * 256 + threadIdx.x;
     if (b >= T.nb) return;
-    const int bucket = C.cell[b].w;
-    const int slot = C.start[bucket] + atomicAdd(&C.fill[bucket], 1);
+    const int slot = C.range[C.cell[b].w].x + C.rank[b];
     C.sorted[slot] = b;
     const float4 x = S.xyz[b];
     C.spos[slot] = make_float4(x.x, x.y, x.z, __int_as_float(b | (T.extra[b] ? (int)0x80000000 : 0)));

@@ -426,8 +426,8 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
             CellTables &C = e->cells;
             int *p_toff, *p_tsz;
             if (dalloc(e, &p_toff, nt) != hipSuccess || dalloc(e, &p_tsz, nt) != hipSuccess ||
-                dalloc(e, &C.count, total) != hipSuccess || dalloc(e, &C.start, total) != hipSuccess ||
-                dalloc(e, &C.fill, total) != hipSuccess || dalloc(e, &C.cell, nb) != hipSuccess ||
+                dalloc(e, &C.count, total) != hipSuccess || dalloc(e, &C.rank, nb) != hipSuccess ||
+                dalloc(e, &C.cell, nb) != hipSuccess ||
                 dalloc(e, &C.sorted, nb) != hipSuccess || dalloc(e, &C.range, total) != hipSuccess ||
                 dalloc(e, &C.spos, nb) != hipSuccess ||
                 dalloc(e, &C.bsum, (total + CELL_SCAN_SPAN - 1) / CELL_SCAN_SPAN) != hipSuccess) {

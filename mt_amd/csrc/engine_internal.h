@@ -52,9 +52,10 @@ constexpr int TEA_CHUNK_BEADS = 128;
 struct CellTables {
     const int *toff = nullptr;   // trajectory -> first bucket
     const int *tsize = nullptr;  // trajectory -> bucket count (power of two)
-    int *count = nullptr, *start = nullptr, *fill = nullptr;   // per bucket
+    int *count = nullptr;        // per bucket
     int2 *range = nullptr;       // per bucket: first slot, bead count
     int4 *cell = nullptr;        // per bead: cell coordinates, bucket
+    int *rank = nullptr;         // per bead: its place among the beads of its bucket
     int *sorted = nullptr;       // beads in bucket order
     float4 *spos = nullptr;      // per slot: position of the bead there, w = its index (bit 31: extra)
     int *bsum = nullptr;         // scan partials, one per CELL_SCAN_SPAN buckets
