@@ -27,6 +27,21 @@ constexpr int STATE_ARRAYS = 4;         // xyz|flags, n|fi, p1|theta, p2|psi
 constexpr int HDR_INTS = 2;             // per bead: list offset, packed counts
 constexpr int SCAN_INTS = RES_MAX_THREADS;
 constexpr int FLAG_INTS = 2;            // per-step "a bead left its skin" flags, alternating by step parity
+// Noise offload (two beads per thread): a second-pass bead owned by a wave
+// of the first SIMD (wave % 4 == 0) draws its normals through the thread
+// RNG_HELP places further on, which has no second-pass bead of its own; the
+// helper draws them one step ahead into LDS (two buffers by step parity).
+// The first SIMD carries one more second-pass wave than the others when a
+// trajectory is not a multiple of 256 beads; this moves the HybridTaus and
+// Box-Muller work of its second-pass beads to an idle SIMD.
+constexpr int RNG_HELP = 6 * 64;
+constexpr int RNG_SLOTS = 128;          // offloaded beads at most (two waves)
+constexpr int RNG_F4 = 2 * RNG_SLOTS * 2;   // float4: [parity][slot][rx, ra]
+__device__ __forceinline__ int rng_slot(int t) { return (t / 256) * 64 + (t % 64); }
+// Whether thread t's second-pass bead draws through a helper (t + RNG_HELP).
+__device__ __forceinline__ bool rng_offloaded(int t, int m2, int nthreads) {
+    return (t / 64) % 4 == 0 && t < m2 && t + RNG_HELP < nthreads && t + RNG_HELP >= m2 && rng_slot(t) < RNG_SLOTS;
+}
 // LJ entries copied to LDS are those within LJ_CUT + 2 SKIN_HALF (+ margin)
 // of the bead at the launch's first step; while every bead stays within
 // SKIN_HALF of where it was then, no dropped pair can come below LJ_CUT.
@@ -88,7 +103,7 @@ __device__ __forceinline__ void run_steps(const mte_params &P, const DevTopo &T,
                                           float4 *ln, float4 *l1, float4 *l2, const bool mob[PT], uint4 s0[PT],
                                           uint4 s1[PT], float4 trig[PT], const float *noise, int nsteps, const LatOffsets &L,
                                           const MakeLists &make_lists, bool filtered, int *flags,
-                                          const Refill &refill) {
+                                          const Refill &refill, float4 *rbuf, int own_slot, int help_slot) {
     const int base = T.tbase[t], nb = T.nb;
     const float box = T.tbox[t];
     const float wall_z = P.is_wall ? T.zs[t] : 0.f;
@@ -115,6 +130,9 @@ __device__ __forceinline__ void run_steps(const mte_params &P, const DevTopo &T,
                 const float *nz = noise + (size_t)s * 6 * nb + 6 * (size_t)b;
                 rx = make_float4(nz[0], nz[1], nz[2], 0.f);
                 ra = make_float4(nz[3], nz[4], nz[5], 0.f);
+            } else if (k == 1 && own_slot >= 0) {   // drawn by the helper (PT == 2)
+                rx = rbuf[((s & 1) * RNG_SLOTS + own_slot) * 2];
+                ra = rbuf[((s & 1) * RNG_SLOTS + own_slot) * 2 + 1];
             } else {
                 rx = normals4(s0[k]);
                 ra = normals4(s1[k]);
@@ -126,6 +144,10 @@ __device__ __forceinline__ void run_steps(const mte_params &P, const DevTopo &T,
                 const float dx = x.x - xr[k].x, dy = x.y - xr[k].y, dz = x.z - xr[k].z;
                 moved |= dx * dx + dy * dy + dz * dz >= SKIN_HALF * SKIN_HALF;
             }
+        }
+        if (PT == 2 && help_slot >= 0 && s + 1 < nsteps) {   // the owner's draws of the next step
+            rbuf[(((s + 1) & 1) * RNG_SLOTS + help_slot) * 2] = normals4(s0[PT - 1]);
+            rbuf[(((s + 1) & 1) * RNG_SLOTS + help_slot) * 2 + 1] = normals4(s1[PT - 1]);
         }
         if (filtered && moved) flags[s & 1] = 1;
         __syncthreads();
@@ -150,7 +172,8 @@ __global__ void __launch_bounds__(MAXT)
                int stride, int entry_cap) {
     extern __shared__ float4 lds[];
     float4 *lx = lds, *ln = lds + stride, *l1 = lds + 2 * stride, *l2 = lds + 3 * stride;
-    int *hoff = reinterpret_cast<int *>(lds + STATE_ARRAYS * stride);
+    float4 *rbuf = lds + STATE_ARRAYS * stride;
+    int *hoff = reinterpret_cast<int *>(rbuf + RNG_F4);
     int *hcnt = hoff + stride;
     int *scan = hcnt + stride;
     int *flags = scan + SCAN_INTS;
@@ -176,6 +199,22 @@ __global__ void __launch_bounds__(MAXT)
         if (mob[k] && !noise) {
             s0[k] = seeds[b];
             s1[k] = seeds[b + nb];
+        }
+    }
+    // noise offload roles (two beads per thread, drawn noise)
+    const int m2 = n - (int)blockDim.x;
+    const int own_slot = (PT == 2 && !noise && rng_offloaded(threadIdx.x, m2, blockDim.x)) ? rng_slot(threadIdx.x) : -1;
+    const int owner = (int)threadIdx.x - RNG_HELP;
+    int help_slot = -1, help_b = -1;
+    if (PT == 2 && !noise && owner >= 0 && rng_offloaded(owner, m2, blockDim.x)) {
+        const int b = base + owner + blockDim.x;   // the owner's second-pass bead
+        if (!T.fixed[b] && !T.extra[b]) {
+            help_slot = rng_slot(owner);
+            help_b = b;
+            s0[PT - 1] = seeds[b];
+            s1[PT - 1] = seeds[b + nb];
+            rbuf[help_slot * 2] = normals4(s0[PT - 1]);   // step 0's draws (parity 0)
+            rbuf[help_slot * 2 + 1] = normals4(s1[PT - 1]);
         }
     }
     __syncthreads();
@@ -280,12 +319,12 @@ __global__ void __launch_bounds__(MAXT)
         auto mk = [&](int i, int) { return LdsLists{ent, hoff[i], hcnt[i]}; };
         auto refill = [&]() { compact(0.f, true); };
         run_steps<PT, LdsLists>(P, T, A, t, lx, ln, l1, l2, mob, s0, s1, trig, noise, nsteps, L, mk, filter, flags,
-                                refill);
+                                refill, rbuf, own_slot, help_slot);
     } else {
         auto mk = [&](int, int b) { return GlobalLists{T, b, base}; };
         auto refill = [&]() {};
         run_steps<PT, GlobalLists>(P, T, A, t, lx, ln, l1, l2, mob, s0, s1, trig, noise, nsteps, L, mk, false, flags,
-                                   refill);
+                                   refill, rbuf, own_slot, help_slot);
     }
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int b = base + i;
@@ -300,17 +339,21 @@ __global__ void __launch_bounds__(MAXT)
     }
 #pragma unroll
     for (int k = 0; k < PT; k++) {
-        if (mob[k] && !noise) {
+        if (mob[k] && !noise && !(k == 1 && own_slot >= 0)) {   // an offloaded bead's state is the helper's
             const int b = base + threadIdx.x + k * blockDim.x;
             seeds[b] = s0[k];
             seeds[b + nb] = s1[k];
         }
     }
+    if (help_b >= 0) {
+        seeds[help_b] = s0[PT - 1];
+        seeds[help_b + nb] = s1[PT - 1];
+    }
 }
 
 static size_t fixed_lds_bytes(int nmax) {
-    return (size_t)STATE_ARRAYS * sizeof(float4) * nmax + (size_t)HDR_INTS * sizeof(int) * nmax +
-           (size_t)(SCAN_INTS + FLAG_INTS) * sizeof(int);
+    return (size_t)STATE_ARRAYS * sizeof(float4) * nmax + (size_t)RNG_F4 * sizeof(float4) +
+           (size_t)HDR_INTS * sizeof(int) * nmax + (size_t)(SCAN_INTS + FLAG_INTS) * sizeof(int);
 }
 
 // Launch plan with pt beads per thread (pt = 1 allows up to 1024 threads,
@@ -328,6 +371,10 @@ bool resident_plan_for(const mte_engine *e, int pt, ResidentPlan *plan) {
     if (fixed + (size_t)MIN_ENTRIES_PER_BEAD * 2 * nmax > (size_t)lds_limit) return false;
     int threads = (nmax + pt - 1) / pt;
     threads = ((threads + 63) / 64) * 64;
+    // Two beads per thread run on 12 waves when the batch allows: the
+    // second-pass beads then leave idle waves on the third SIMD for the
+    // noise offload.
+    if (pt == 2 && nmax > RES_MULTI_THREADS) threads = RES_MULTI_THREADS;
     if (threads > (pt == 1 ? RES_MAX_THREADS : RES_MULTI_THREADS)) return false;
     plan->threads = threads;
     plan->per_thread = pt;

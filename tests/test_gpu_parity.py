@@ -162,6 +162,27 @@ def test_resident_rng_and_uneven_batch():
     eng.close()
 
 
+def test_resident_two_beads_per_thread_rng_matches_oracle():
+    """1040-bead trajectories run two beads per thread on 12 waves; the
+    second-pass beads of the first SIMD's waves draw their noise through
+    helper threads one step ahead: the HybridTaus states after the run and
+    the positions equal the oracle's from the same seeds."""
+    print("case resident 2 x 1040 beads, two per thread, drawn noise, 150 steps", flush=True)
+    from mt_amd.engine import MODE_RESIDENT
+    p = base_params()
+    b = c1_batch(ntraj=2, n_dimers=40)
+    eng, orc = _pair(p, b, mode=MODE_RESIDENT, precision="f32")
+    assert eng.mode == MODE_RESIDENT and eng.tuning()["resident_per_thread"] == 2
+    seeds = eng.get_rng().copy()
+    eng.run(150)
+    assert orc.run(0, 150, seeds=seeds) == 0
+    np.testing.assert_array_equal(eng.get_rng(), seeds)
+    dx = np.abs(eng.get_state()[:, :3] - orc.b.coords[:, :3]).max()
+    print("resident 2 per thread max |dx|", dx, flush=True)
+    assert dx < 1e-3
+    eng.close()
+
+
 @pytest.mark.parametrize("chunk", [0, TEA_SYMMETRIC])
 def test_tea_drift_against_fp64_exact_em(chunk):
     print(f"case TEA drift vs FP64 exact EM (312 beads), tea_chunk={chunk}", flush=True)
