@@ -98,6 +98,27 @@ hipError_t mte::tea_build_units(mte_engine *e, int chunk) {
     return hipStreamSynchronize(e->stream);
 }
 
+// Lane order of the resident kernel: per trajectory the mobile beads in index
+// order, then the fixed and extra ones, so that the last (partly filled) wave
+// of a pass holds beads with nothing to compute where the trajectory has
+// enough of them.  Any order gives the same results (each bead's sums do not
+// depend on the lane that computes them).
+hipError_t mte::upload_resident_order(mte_engine *e, const uint8_t *extra) {
+    std::vector<int> order(e->nb);
+    for (int t = 0; t < e->ntraj; t++) {
+        const int64_t b0 = e->off[t], n = e->off[t + 1] - e->off[t];
+        int k = 0;
+        for (int pass = 0; pass < 2; pass++)
+            for (int i = 0; i < n; i++) {
+                const bool still = e->fixed_h[b0 + i] || extra[b0 + i];
+                if (still == (pass == 1)) order[b0 + k++] = i;
+            }
+    }
+    hipError_t s = hipMemcpyAsync(e->res_order, order.data(), sizeof(int) * e->nb, hipMemcpyHostToDevice, e->stream);
+    if (s != hipSuccess) return s;
+    return hipStreamSynchronize(e->stream);
+}
+
 // Work units of the one-pass epsilon kernel: every pair (I group, J group >=
 // I group) of 1024-bead groups of a trajectory, and its partial buffer.  Left
 // out (the two-sided kernel computes epsilon) when the buffer would exceed
@@ -333,7 +354,7 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     ALLOC(T.lon_n, nb) ALLOC(T.lon, (size_t)MAXLON * nb) ALLOC(T.lat_n, nb) ALLOC(T.lat, (size_t)MAXLAT * nb)
     ALLOC(T.lj_n, nb) ALLOC(T.lj, (size_t)T.max_lj * nb) ALLOC(p_fixed, nb) ALLOC(p_extra, nb) ALLOC(p_mon, nb)
     ALLOC(p_gtp, nb) ALLOC(p_ont, nb) ALLOC(p_zs, nt) ALLOC(p_box, nt) ALLOC(T.overflow, 1) ALLOC(e->tiles, tiles.size())
-    ALLOC(e->rb_order, nb) ALLOC(e->mtl_counts, nt) ALLOC(e->mtl_flags, nb)
+    ALLOC(e->rb_order, nb) ALLOC(e->res_order, nb) ALLOC(e->mtl_counts, nt) ALLOC(e->mtl_flags, nb)
     for (int k = 0; k < 2; k++) {
         ALLOC(e->S[k].xyz, nb) ALLOC(e->S[k].ang, nb) ALLOC(e->S[k].fn, nb) ALLOC(e->S[k].fp1, nb)
         ALLOC(e->S[k].fp2, nb)
@@ -390,6 +411,11 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
         for (int i = 0; i < tsize[t]; i++) order[tbase[t] + i] = i;
     UP(e->rb_order, order.data(), nb)
 #undef UP
+    e->fixed_h = fixed;
+    if (upload_resident_order(e, extra.data()) != hipSuccess) {
+        e->err = "mte_create: upload failed for the resident lane order";
+        return bail(MTE_ERR_HIP);
+    }
     if (hipMemsetAsync(T.lj_n, 0, sizeof(int) * nb, e->stream) != hipSuccess ||
         hipMemsetAsync(T.overflow, 0, sizeof(int), e->stream) != hipSuccess) {
         e->err = "mte_create: memset failed";
@@ -535,6 +561,7 @@ int mte_set_flags(mte_engine *e, const uint8_t *extra, const int32_t *gtp, const
         std::vector<uint8_t> x(extra, extra + e->nb);
         for (auto &v : x) v = v ? 1 : 0;
         HIPCHK(hipMemcpyAsync(e->T.extra, x.data(), e->nb, hipMemcpyHostToDevice, e->stream));
+        HIPCHK(upload_resident_order(e, x.data()));
         HIPCHK(hipStreamSynchronize(e->stream));
     }
     if (gtp) HIPCHK(hipMemcpyAsync(e->T.gtp, gtp, sizeof(int) * e->nb, hipMemcpyHostToDevice, e->stream));

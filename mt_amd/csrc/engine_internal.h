@@ -117,6 +117,8 @@ struct mte_engine {
     int2 *tiles = nullptr;
     int ntiles = 0;
     int *rb_order = nullptr;   // per trajectory: local bead handled at each rebuild position (spatial order)
+    int *res_order = nullptr;  // per trajectory: local bead at each resident lane slot (immobile beads last)
+    std::vector<uint8_t> fixed_h;   // host copy of the fixed flags (the resident lane order)
     int *mtl_counts = nullptr, *mtl_flags = nullptr;   // mte_mt_length: per trajectory, per bead
     size_t forces_lds = 0;     // > 0: forces by k_forces_lds with this much dynamic LDS
     int rb_traj_threads = 0;   // > 0: list rebuilds by k_rebuild_traj with this workgroup size
@@ -177,6 +179,7 @@ hipError_t build_cells(mte_engine *e, float side, int K);
 float candidate_radius(const mte_params &P);
 hipError_t launch_tea_eps(mte_engine *e);
 hipError_t tea_build_eps_units(mte_engine *e);
+hipError_t upload_resident_order(mte_engine *e, const uint8_t *extra);
 hipError_t launch_tea_pairs(mte_engine *e);
 hipError_t launch_tea_finish(mte_engine *e, const float *noise_step);
 hipError_t launch_resident(mte_engine *e, int nsteps, const float *noise);

@@ -103,20 +103,21 @@ __device__ __forceinline__ void run_steps(const mte_params &P, const DevTopo &T,
                                           float4 *ln, float4 *l1, float4 *l2, const bool mob[PT], uint4 s0[PT],
                                           uint4 s1[PT], float4 trig[PT], const float *noise, int nsteps, const LatOffsets &L,
                                           const MakeLists &make_lists, bool filtered, int *flags,
-                                          const Refill &refill, float4 *rbuf, int own_slot, int help_slot) {
+                                          const Refill &refill, float4 *rbuf, int own_slot, int help_slot,
+                                          const int ib[PT]) {
     const int base = T.tbase[t], nb = T.nb;
     const float box = T.tbox[t];
     const float wall_z = P.is_wall ? T.zs[t] : 0.f;
     float3 xr[PT];
 #pragma unroll
-    for (int k = 0; k < PT; k++) xr[k] = mob[k] ? f3(lx[threadIdx.x + k * blockDim.x]) : make_float3(0.f, 0.f, 0.f);
+    for (int k = 0; k < PT; k++) xr[k] = mob[k] ? f3(lx[ib[k]]) : make_float3(0.f, 0.f, 0.f);
     for (int s = 0; s < nsteps; s++) {
         float4 nx[PT], na[PT];
         bool moved = false;
 #pragma unroll
         for (int k = 0; k < PT; k++) {
             if (!mob[k]) continue;
-            const int i = threadIdx.x + k * blockDim.x;
+            const int i = ib[k];
             const int b = base + i;
             float4 x = lx[i];
             float4 a = make_float4(ln[i].w, l1[i].w, l2[i].w, 0.f);
@@ -156,7 +157,7 @@ __device__ __forceinline__ void run_steps(const mte_params &P, const DevTopo &T,
 #pragma unroll
         for (int k = 0; k < PT; k++) {
             if (!mob[k]) continue;
-            trig[k] = store_state(lx, ln, l1, l2, threadIdx.x + k * blockDim.x, nx[k], na[k], L);
+            trig[k] = store_state(lx, ln, l1, l2, ib[k], nx[k], na[k], L);
         }
         __syncthreads();
         if (left) {   // uniform: every thread read the same flag
@@ -169,7 +170,7 @@ __device__ __forceinline__ void run_steps(const mte_params &P, const DevTopo &T,
 template <int PT, int MAXT>
 __global__ void __launch_bounds__(MAXT)
     k_resident(mte_params P, DevTopo T, DevState S, uint4 *seeds, const float *noise, int nsteps, LatOffsets L,
-               int stride, int entry_cap) {
+               int stride, int entry_cap, const int *order) {
     extern __shared__ float4 lds[];
     float4 *lx = lds, *ln = lds + stride, *l1 = lds + 2 * stride, *l2 = lds + 3 * stride;
     float4 *rbuf = lds + STATE_ARRAYS * stride;
@@ -181,13 +182,17 @@ __global__ void __launch_bounds__(MAXT)
     const int t = blockIdx.x;
     const int base = T.tbase[t], n = T.tsize[t];
     const int nb = T.nb;
+    if (threadIdx.x == 0) flags[0] = 0;
     stage_state(T, S, base, n, lx, ln, l1, l2);
     bool mob[PT];
     uint4 s0[PT], s1[PT];
     float4 trig[PT];
+    int ib[PT];   // the bead of each of this thread's lane slots (order: immobile beads last)
 #pragma unroll
     for (int k = 0; k < PT; k++) {
-        const int i = threadIdx.x + k * blockDim.x;
+        const int slot = threadIdx.x + k * blockDim.x;
+        const int i = slot < n ? order[base + slot] : slot;
+        ib[k] = i;
         const int b = base + i;
         mob[k] = i < n && !T.fixed[b] && !T.extra[b];
         trig[k] = make_float4(0.f, 1.f, 0.f, 1.f);
@@ -201,13 +206,24 @@ __global__ void __launch_bounds__(MAXT)
             s1[k] = seeds[b + nb];
         }
     }
-    // noise offload roles (two beads per thread, drawn noise)
+    // noise offload roles (two beads per thread, drawn noise); the first
+    // wave offloads too only when a later second-pass wave of the first SIMD
+    // has work (its lanes past the mobile beads hold the immobile ones)
     const int m2 = n - (int)blockDim.x;
-    const int own_slot = (PT == 2 && !noise && rng_offloaded(threadIdx.x, m2, blockDim.x)) ? rng_slot(threadIdx.x) : -1;
+    // (flags[0] was cleared before the first barrier; the skin test clears it
+    // again in compact before its first use)
+    __syncthreads();
+    if (PT == 2 && threadIdx.x >= 256 && mob[PT - 1]) flags[0] = 1;
+    __syncthreads();
+    const bool tail_busy = flags[0] != 0;
+    auto offloaded = [&](int tt) {
+        return PT == 2 && !noise && rng_offloaded(tt, m2, blockDim.x) && (tt >= 256 || tail_busy);
+    };
+    const int own_slot = offloaded(threadIdx.x) ? rng_slot(threadIdx.x) : -1;
     const int owner = (int)threadIdx.x - RNG_HELP;
     int help_slot = -1, help_b = -1;
-    if (PT == 2 && !noise && owner >= 0 && rng_offloaded(owner, m2, blockDim.x)) {
-        const int b = base + owner + blockDim.x;   // the owner's second-pass bead
+    if (owner >= 0 && offloaded(owner)) {
+        const int b = base + order[base + owner + blockDim.x];   // the owner's second-pass bead
         if (!T.fixed[b] && !T.extra[b]) {
             help_slot = rng_slot(owner);
             help_b = b;
@@ -319,12 +335,12 @@ __global__ void __launch_bounds__(MAXT)
         auto mk = [&](int i, int) { return LdsLists{ent, hoff[i], hcnt[i]}; };
         auto refill = [&]() { compact(0.f, true); };
         run_steps<PT, LdsLists>(P, T, A, t, lx, ln, l1, l2, mob, s0, s1, trig, noise, nsteps, L, mk, filter, flags,
-                                refill, rbuf, own_slot, help_slot);
+                                refill, rbuf, own_slot, help_slot, ib);
     } else {
         auto mk = [&](int, int b) { return GlobalLists{T, b, base}; };
         auto refill = [&]() {};
         run_steps<PT, GlobalLists>(P, T, A, t, lx, ln, l1, l2, mob, s0, s1, trig, noise, nsteps, L, mk, false, flags,
-                                   refill, rbuf, own_slot, help_slot);
+                                   refill, rbuf, own_slot, help_slot, ib);
     }
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int b = base + i;
@@ -340,7 +356,7 @@ __global__ void __launch_bounds__(MAXT)
 #pragma unroll
     for (int k = 0; k < PT; k++) {
         if (mob[k] && !noise && !(k == 1 && own_slot >= 0)) {   // an offloaded bead's state is the helper's
-            const int b = base + threadIdx.x + k * blockDim.x;
+            const int b = base + ib[k];
             seeds[b] = s0[k];
             seeds[b + nb] = s1[k];
         }
@@ -398,7 +414,7 @@ static hipError_t launch_pt(mte_engine *e, int nsteps, const float *noise) {
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds_bytes);
     if (s != hipSuccess) return s;
     hipLaunchKernelGGL((k_resident<PT, MAXT>), dim3(e->ntraj), dim3(pl.threads), pl.lds_bytes, e->stream, e->P,
-                       e->T, e->S[e->cur], e->seeds, noise, nsteps, e->L, pl.max_beads, pl.entry_cap);
+                       e->T, e->S[e->cur], e->seeds, noise, nsteps, e->L, pl.max_beads, pl.entry_cap, e->res_order);
     return hipGetLastError();
 }
 
