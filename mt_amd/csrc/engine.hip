@@ -105,14 +105,17 @@ hipError_t mte::tea_build_units(mte_engine *e, int chunk) {
 // depend on the lane that computes them).
 hipError_t mte::upload_resident_order(mte_engine *e, const uint8_t *extra) {
     std::vector<int> order(e->nb);
+    e->res_mobile_max = 0;
     for (int t = 0; t < e->ntraj; t++) {
         const int64_t b0 = e->off[t], n = e->off[t + 1] - e->off[t];
         int k = 0;
-        for (int pass = 0; pass < 2; pass++)
+        for (int pass = 0; pass < 2; pass++) {
             for (int i = 0; i < n; i++) {
                 const bool still = e->fixed_h[b0 + i] || extra[b0 + i];
                 if (still == (pass == 1)) order[b0 + k++] = i;
             }
+            if (pass == 0) e->res_mobile_max = std::max(e->res_mobile_max, k);
+        }
     }
     hipError_t s = hipMemcpyAsync(e->res_order, order.data(), sizeof(int) * e->nb, hipMemcpyHostToDevice, e->stream);
     if (s != hipSuccess) return s;
@@ -563,6 +566,15 @@ int mte_set_flags(mte_engine *e, const uint8_t *extra, const int32_t *gtp, const
         HIPCHK(hipMemcpyAsync(e->T.extra, x.data(), e->nb, hipMemcpyHostToDevice, e->stream));
         HIPCHK(upload_resident_order(e, x.data()));
         HIPCHK(hipStreamSynchronize(e->stream));
+        // a one-bead-per-thread plan over a trajectory longer than the
+        // workgroup needs its beads past the workgroup to stay immobile
+        ResidentPlan plan;
+        if (e->rplan.per_thread == 1 && !resident_plan_for(e, 1, &plan)) {
+            if (!resident_fits(e, &plan))
+                return fail(e, MTE_ERR_STATE, "mte_set_flags: no resident plan fits the new flags");
+            e->rplan = plan;
+            if (e->tune.resident_per_thread) e->tune.resident_per_thread = plan.per_thread;
+        }
     }
     if (gtp) HIPCHK(hipMemcpyAsync(e->T.gtp, gtp, sizeof(int) * e->nb, hipMemcpyHostToDevice, e->stream));
     if (on_tubule) HIPCHK(hipMemcpyAsync(e->T.on_tub, on_tubule, sizeof(int) * e->nb, hipMemcpyHostToDevice, e->stream));

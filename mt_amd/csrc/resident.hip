@@ -247,12 +247,7 @@ __global__ void __launch_bounds__(MAXT)
     // per-bead totals into offsets -> copy (when `copy` and they fit).
     // Returns the number of entries.
     auto compact = [&](float rf2, bool copy) {
-        int mytot[PT];
-#pragma unroll
-        for (int k = 0; k < PT; k++) {
-            const int i = threadIdx.x + k * blockDim.x;
-            mytot[k] = 0;
-            if (i >= n) continue;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
             const int b = base + i;
             const int h = T.harm[b] >= 0 ? 1 : 0;
             const int nlon = T.lon_n[b], nlat = T.lat_n[b];
@@ -264,16 +259,11 @@ __global__ void __launch_bounds__(MAXT)
                     for (int q = 0; q < m; q++) nlj += lj_keep(i, T.lj[q * nb + b] - base, rf2) ? 1 : 0;
             }
             hcnt[i] = pack_counts(h, nlon, nlat, nlj);
-            mytot[k] = h + nlon + nlat + nlj;
+            hoff[i] = h + nlon + nlat + nlj;
         }
         // Chunked exclusive scan: thread t owns beads [t*c, (t+1)*c); the
         // totals go through hoff, indexed by bead.
         const int c = (n + blockDim.x - 1) / blockDim.x;
-#pragma unroll
-        for (int k = 0; k < PT; k++) {
-            const int i = threadIdx.x + k * blockDim.x;
-            if (i < n) hoff[i] = mytot[k];
-        }
         __syncthreads();
         int run = 0;
         const int lo = threadIdx.x * c, hi = min(n, lo + c);
@@ -295,10 +285,7 @@ __global__ void __launch_bounds__(MAXT)
         }
         __syncthreads();
         if (copy && total <= entry_cap) {
-#pragma unroll
-            for (int k = 0; k < PT; k++) {
-                const int i = threadIdx.x + k * blockDim.x;
-                if (i >= n) continue;
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
                 const int b = base + i;
                 int o = hoff[i];
                 const int hc = hcnt[i];
@@ -387,6 +374,10 @@ bool resident_plan_for(const mte_engine *e, int pt, ResidentPlan *plan) {
     if (fixed + (size_t)MIN_ENTRIES_PER_BEAD * 2 * nmax > (size_t)lds_limit) return false;
     int threads = (nmax + pt - 1) / pt;
     threads = ((threads + 63) / 64) * 64;
+    // One bead per thread also covers a trajectory longer than the largest
+    // workgroup when every bead past it is immobile: the lane order puts the
+    // mobile beads first, and an immobile bead needs its place in LDS only.
+    if (pt == 1 && threads > RES_MAX_THREADS && e->res_mobile_max <= RES_MAX_THREADS) threads = RES_MAX_THREADS;
     // Two beads per thread run on 12 waves when the batch allows: the
     // second-pass beads then leave idle waves on the third SIMD for the
     // noise offload.

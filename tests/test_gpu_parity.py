@@ -172,6 +172,7 @@ def test_resident_two_beads_per_thread_rng_matches_oracle():
     p = base_params()
     b = c1_batch(ntraj=2, n_dimers=40)
     eng, orc = _pair(p, b, mode=MODE_RESIDENT, precision="f32")
+    eng.set_tuning(resident_per_thread=2)
     assert eng.mode == MODE_RESIDENT and eng.tuning()["resident_per_thread"] == 2
     seeds = eng.get_rng().copy()
     eng.run(150)
@@ -181,6 +182,67 @@ def test_resident_two_beads_per_thread_rng_matches_oracle():
     print("resident 2 per thread max |dx|", dx, flush=True)
     assert dx < 1e-3
     eng.close()
+
+
+def test_resident_one_bead_per_thread_immobile_tail_matches_oracle():
+    """1040-bead trajectories whose 26 fixed beads take the lanes past the
+    1024-thread workgroup run one bead per thread by default; the fixed tail
+    is only staged in LDS.  The HybridTaus states and positions equal the
+    oracle's, and the two-beads-per-thread plan agrees."""
+    print("case resident 2 x 1040 beads, one per thread with a fixed tail, 150 steps", flush=True)
+    from mt_amd.engine import MODE_RESIDENT
+    p = base_params()
+    b = c1_batch(ntraj=2, n_dimers=40)
+    eng, orc = _pair(p, b, mode=MODE_RESIDENT, precision="f32")
+    assert eng.mode == MODE_RESIDENT and eng.tuning()["resident_per_thread"] == 1
+    two = Engine(p, b.copy(), max_lj=64, mode=MODE_RESIDENT)
+    two.set_tuning(resident_per_thread=2)
+    seeds = eng.get_rng().copy()
+    np.testing.assert_array_equal(two.get_rng(), seeds)
+    eng.run(150)
+    two.run(150)
+    assert orc.run(0, 150, seeds=seeds) == 0
+    np.testing.assert_array_equal(eng.get_rng(), seeds)
+    np.testing.assert_array_equal(two.get_rng(), seeds)
+    dx = np.abs(eng.get_state()[:, :3] - orc.b.coords[:, :3]).max()
+    d12 = np.abs(eng.get_state() - two.get_state()).max()
+    print("resident 1 per thread max |dx|", dx, "vs two per thread", d12, flush=True)
+    assert dx < 1e-3 and d12 < 1e-4
+    eng.close()
+    two.close()
+
+
+def test_resident_plan_follows_the_mobile_count():
+    """Unfixed 1040-bead trajectories with 30 extra beads each fit one bead
+    per thread; clearing the extra flags leaves 1040 mobile beads, and the
+    engine moves to two beads per thread: the run then equals that of an
+    engine created with the cleared flags, bit for bit, which follows the
+    oracle."""
+    print("case resident plan after set_flags, 2 x 1040 unfixed beads, 60 steps", flush=True)
+    from mt_amd.engine import MODE_RESIDENT
+    p = base_params()
+    b = c1_batch(ntraj=2, n_dimers=40)
+    b.fixed[:] = 0
+    be = b.copy()
+    for t in range(2):
+        be.extra[int(b.off[t]) + 1000:int(b.off[t]) + 1030] = 1
+    eng = Engine(p, be, max_lj=64, mode=MODE_RESIDENT)
+    assert eng.tuning()["resident_per_thread"] == 1
+    eng.set_flags(extra=np.zeros(b.nb, np.uint8))
+    assert eng.mode == MODE_RESIDENT and eng.tuning()["resident_per_thread"] == 2
+    ref, orc = _pair(p, b, mode=MODE_RESIDENT, precision="f32")
+    assert ref.tuning()["resident_per_thread"] == 2
+    seeds = ref.get_rng().copy()
+    eng.run(60)
+    ref.run(60)
+    np.testing.assert_array_equal(eng.get_state(), ref.get_state())
+    np.testing.assert_array_equal(eng.get_rng(), ref.get_rng())
+    assert orc.run(0, 60, seeds=seeds) == 0
+    dx = np.abs(ref.get_state()[:, :3] - orc.b.coords[:, :3]).max()
+    print("resident after set_flags max |dx|", dx, flush=True)
+    assert dx < 1e-3
+    eng.close()
+    ref.close()
 
 
 @pytest.mark.parametrize("chunk", [0, TEA_SYMMETRIC])

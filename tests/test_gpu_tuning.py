@@ -22,7 +22,7 @@ def test_autotune_plain_keeps_results(mode):
     paths sum the LJ terms in different orders)."""
     print(f"case autotune plain mode={mode}", flush=True)
     p = P.derive(P.BASE)
-    b = c1_batch(ntraj=8, n_dimers=40)     # 1040 beads: resident plans with 2 or 3 beads per thread
+    b = c1_batch(ntraj=8, n_dimers=40)     # 1040 beads, 26 fixed: resident plans with 1, 2 or 3 beads per thread
     a = Engine(p, b.copy(), max_lj=48, mode=mode, rseed=77)
     start_mode = a.mode
     a.run(300)
@@ -32,7 +32,7 @@ def test_autotune_plain_keeps_results(mode):
     print(chosen, report, flush=True)
     assert t.tuning() == chosen and chosen["mode"] in (MODE_RESIDENT, MODE_STREAMED)
     assert sum(l.startswith("stream_block") for l in report) == 3
-    assert sum(l.startswith("resident_per_thread") for l in report) == 2   # 2 and 3 beads per thread
+    assert sum(l.startswith("resident_per_thread") for l in report) == 3   # 1, 2 and 3 beads per thread
     t.run(300)
     got = t.get_state()
     if chosen["mode"] == start_mode:
@@ -97,11 +97,13 @@ def test_autotune_tea_and_cells():
 def test_set_tuning_rejects_unusable_shapes():
     print("case set_tuning argument checks", flush=True)
     p = P.derive(P.BASE)
-    e = Engine(p, c1_batch(ntraj=2, n_dimers=40), max_lj=48, mode=MODE_RESIDENT)
+    b = c1_batch(ntraj=2, n_dimers=40)
+    b.fixed[:] = 0
+    e = Engine(p, b, max_lj=48, mode=MODE_RESIDENT)
     with pytest.raises(EngineError):
         e.set_tuning(stream_block=100)
     with pytest.raises(EngineError):
-        e.set_tuning(resident_per_thread=1)     # 1040 beads do not fit one per thread
+        e.set_tuning(resident_per_thread=1)     # 1040 mobile beads do not fit one per thread
     with pytest.raises(EngineError):
         e.set_tuning(tea_chunk=128)              # no TEA in this run
     with pytest.raises(EngineError):
