@@ -124,7 +124,10 @@ const char *mte_last_error(const mte_engine *e);
 int mte_num_beads(const mte_engine *e, int64_t *nbeads, int32_t *ntraj);
 int mte_set_state(mte_engine *e, const mte_coord *r);
 int mte_get_state(mte_engine *e, mte_coord *r);
-/* Any pointer may be NULL (left unchanged). */
+/* Any pointer may be NULL (left unchanged).  New extra flags reorder the
+ * resident kernel's lanes (immobile beads last); a one-bead-per-thread plan
+ * that no longer covers a trajectory's mobile beads becomes the default plan
+ * (resident_per_thread in mte_get_tuning shows it). */
 int mte_set_flags(mte_engine *e, const uint8_t *extra, const int32_t *gtp, const int32_t *on_tubule);
 
 /* The updater's tubule classification (mt_length, src/updater.cpp:154-217)
