@@ -25,6 +25,7 @@
  *   mte_get_lists              the device lists pairs_kernel / LJ_kernel
  *                              fill, :527-674, :913-940
  *   mte_tea_stats              tea.h_epsilon / h_beta_ij, src/bdhitea.cu:57-118
+ *   mte_tea_ci                 tea.d_ci, src/bdhitea_kernel.cu:61-101
  *   mte_get_rng / mte_set_rng  ht.d_seeds, src/HybridTaus.cu:18-30
  *
  * Conventions: every call returns MTE_OK (0) or an MTE_ERR_* code and never
@@ -170,6 +171,11 @@ int mte_set_rng(mte_engine *e, const uint32_t *seeds);
 
 /* Per-trajectory epsilon and beta of the last TEA refresh (FP64). */
 int mte_tea_stats(mte_engine *e, double *eps, double *beta);
+/* Per-bead row sums of squares of the normalised RPY tensor over j != i
+ * (x, y, z rows; 3 floats per bead) from the last TEA refresh: the
+ * reference's tea.d_ci (src/bdhitea_kernel.cu:61-101), from which
+ * C_i = 1 / sqrt(1 + beta^2 ci) (eq. 19 of Geyer & Winter). */
+int mte_tea_ci(mte_engine *e, float *ci);
 
 /* Kernel timing with HIP events on the engine's stream. */
 int mte_set_timing(mte_engine *e, int32_t on);

@@ -52,6 +52,7 @@ struct DevTopo {
     const float *zs;
     const float *tbox;    // trajectory -> periodic box side (0: open boundaries)
     int *overflow;        // set to 1 by a rebuild that ran out of list capacity
+    int *frozen;          // steps the resident kernel left undone because `overflow` was set
 };
 
 // One buffer of bead state: positions, angles and the rotated offsets.

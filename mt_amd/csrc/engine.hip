@@ -103,20 +103,28 @@ hipError_t mte::tea_build_units(mte_engine *e, int chunk) {
 // of a pass holds beads with nothing to compute where the trajectory has
 // enough of them.  Any order gives the same results (each bead's sums do not
 // depend on the lane that computes them).
+int mte::resident_mobile_max(const mte_engine *e, const uint8_t *extra) {
+    int most = 0;
+    for (int t = 0; t < e->ntraj; t++) {
+        int k = 0;
+        for (int64_t b = e->off[t]; b < e->off[t + 1]; b++) k += (e->fixed_h[b] || extra[b]) ? 0 : 1;
+        most = std::max(most, k);
+    }
+    return most;
+}
+
 hipError_t mte::upload_resident_order(mte_engine *e, const uint8_t *extra) {
     std::vector<int> order(e->nb);
-    e->res_mobile_max = 0;
     for (int t = 0; t < e->ntraj; t++) {
         const int64_t b0 = e->off[t], n = e->off[t + 1] - e->off[t];
         int k = 0;
-        for (int pass = 0; pass < 2; pass++) {
+        for (int pass = 0; pass < 2; pass++)
             for (int i = 0; i < n; i++) {
                 const bool still = e->fixed_h[b0 + i] || extra[b0 + i];
                 if (still == (pass == 1)) order[b0 + k++] = i;
             }
-            if (pass == 0) e->res_mobile_max = std::max(e->res_mobile_max, k);
-        }
     }
+    e->res_mobile_max = resident_mobile_max(e, extra);
     hipError_t s = hipMemcpyAsync(e->res_order, order.data(), sizeof(int) * e->nb, hipMemcpyHostToDevice, e->stream);
     if (s != hipSuccess) return s;
     return hipStreamSynchronize(e->stream);
@@ -356,7 +364,7 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     ALLOC(p_tid, nb) ALLOC(p_tbase, nt) ALLOC(p_tsize, nt) ALLOC(p_harm, nb) ALLOC(p_harm_raw, nb)
     ALLOC(T.lon_n, nb) ALLOC(T.lon, (size_t)MAXLON * nb) ALLOC(T.lat_n, nb) ALLOC(T.lat, (size_t)MAXLAT * nb)
     ALLOC(T.lj_n, nb) ALLOC(T.lj, (size_t)T.max_lj * nb) ALLOC(p_fixed, nb) ALLOC(p_extra, nb) ALLOC(p_mon, nb)
-    ALLOC(p_gtp, nb) ALLOC(p_ont, nb) ALLOC(p_zs, nt) ALLOC(p_box, nt) ALLOC(T.overflow, 1) ALLOC(e->tiles, tiles.size())
+    ALLOC(p_gtp, nb) ALLOC(p_ont, nb) ALLOC(p_zs, nt) ALLOC(p_box, nt) ALLOC(T.overflow, 1) ALLOC(T.frozen, 1) ALLOC(e->tiles, tiles.size())
     ALLOC(e->rb_order, nb) ALLOC(e->res_order, nb) ALLOC(e->mtl_counts, nt) ALLOC(e->mtl_flags, nb)
     for (int k = 0; k < 2; k++) {
         ALLOC(e->S[k].xyz, nb) ALLOC(e->S[k].ang, nb) ALLOC(e->S[k].fn, nb) ALLOC(e->S[k].fp1, nb)
@@ -420,7 +428,8 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
         return bail(MTE_ERR_HIP);
     }
     if (hipMemsetAsync(T.lj_n, 0, sizeof(int) * nb, e->stream) != hipSuccess ||
-        hipMemsetAsync(T.overflow, 0, sizeof(int), e->stream) != hipSuccess) {
+        hipMemsetAsync(T.overflow, 0, sizeof(int), e->stream) != hipSuccess ||
+        hipMemsetAsync(T.frozen, 0, sizeof(int), e->stream) != hipSuccess) {
         e->err = "mte_create: memset failed";
         return bail(MTE_ERR_HIP);
     }
@@ -563,17 +572,27 @@ int mte_set_flags(mte_engine *e, const uint8_t *extra, const int32_t *gtp, const
     if (extra) {
         std::vector<uint8_t> x(extra, extra + e->nb);
         for (auto &v : x) v = v ? 1 : 0;
+        // The resident plan for the new flags is decided before anything
+        // changes: a one-bead-per-thread plan over a trajectory longer than
+        // the workgroup needs its beads past the workgroup to stay immobile.
+        const int mobile_old = e->res_mobile_max;
+        e->res_mobile_max = resident_mobile_max(e, x.data());
+        ResidentPlan plan = e->rplan;
+        bool plan_ok = true;
+        if (e->rplan.per_thread == 1 && !resident_plan_for(e, 1, &plan)) plan_ok = resident_fits(e, &plan);
+        if (!plan_ok && e->mode == MTE_MODE_RESIDENT) {
+            e->res_mobile_max = mobile_old;   // engine unchanged
+            return fail(e, MTE_ERR_STATE, "mte_set_flags: no resident plan fits the new flags");
+        }
         HIPCHK(hipMemcpyAsync(e->T.extra, x.data(), e->nb, hipMemcpyHostToDevice, e->stream));
         HIPCHK(upload_resident_order(e, x.data()));
         HIPCHK(hipStreamSynchronize(e->stream));
-        // a one-bead-per-thread plan over a trajectory longer than the
-        // workgroup needs its beads past the workgroup to stay immobile
-        ResidentPlan plan;
-        if (e->rplan.per_thread == 1 && !resident_plan_for(e, 1, &plan)) {
-            if (!resident_fits(e, &plan))
-                return fail(e, MTE_ERR_STATE, "mte_set_flags: no resident plan fits the new flags");
+        if (plan_ok) {
             e->rplan = plan;
             if (e->tune.resident_per_thread) e->tune.resident_per_thread = plan.per_thread;
+        } else {   // streamed engine: no resident plan fits these flags
+            e->rplan = ResidentPlan{};
+            e->tune.resident_per_thread = 0;
         }
     }
     if (gtp) HIPCHK(hipMemcpyAsync(e->T.gtp, gtp, sizeof(int) * e->nb, hipMemcpyHostToDevice, e->stream));
@@ -606,15 +625,31 @@ int mte_get_step(const mte_engine *e, int64_t *step) {
     return MTE_OK;
 }
 
+// The list-capacity flag of the rebuilds since the last check.  After an
+// overflow the resident kernel leaves the state and RNG streams at the
+// overflowing rebuild's step (T.frozen counts the steps it did not take), so
+// the step counter and the injected-noise cursor go back by that many; the
+// next run rebuilds the lists again (and overflows again unless the state
+// changed).
 static int check_overflow(mte_engine *e) {
-    int ov = 0;
+    int ov = 0, frozen = 0;
     HIPCHK(hipMemcpyAsync(&ov, e->T.overflow, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(&frozen, e->T.frozen, sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
-    if (ov) {
-        (void)hipMemsetAsync(e->T.overflow, 0, sizeof(int), e->stream);
-        return fail(e, MTE_ERR_OVERFLOW, "neighbour list capacity exceeded (longitudinal 8, lateral 16 or LJ max_lj)");
-    }
-    return MTE_OK;
+    if (!ov) return MTE_OK;
+    HIPCHK(hipMemsetAsync(e->T.overflow, 0, sizeof(int), e->stream));
+    HIPCHK(hipMemsetAsync(e->T.frozen, 0, sizeof(int), e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->step -= frozen;
+    if (e->noise) e->noise_cursor -= frozen;
+    e->lists_step = -1;
+    e->lists_valid = false;
+    char buf[200];
+    std::snprintf(buf, sizeof buf,
+                  "neighbour list capacity exceeded (longitudinal 8, lateral 16 or LJ max_lj) by the rebuild at "
+                  "step %lld; the state is that step's",
+                  (long long)e->step);
+    return fail(e, MTE_ERR_OVERFLOW, buf);
 }
 
 static int rebuild_now(mte_engine *e) {
@@ -655,21 +690,14 @@ static int check_tea(mte_engine *e) {
 
 static bool mlp_active(const mte_engine *e) { return e->hi_mode == MTE_HI_MLP && !e->mlp_fallback; }
 
-// Enqueue `len` streamed steps starting at global step s0 (list rebuilds and
-// TEA refreshes at their periods), flipping e->cur per step.  The same
-// sequence is launched directly or recorded into a graph.
+// Enqueue `len` streamed steps starting at global step s0 (TEA refreshes and
+// MLP checks at their periods; a list rebuild due at s0 has run already),
+// flipping e->cur per step.  The same sequence is launched directly or
+// recorded into a graph.
 static hipError_t enqueue_streamed(mte_engine *e, int64_t s0, int64_t len, int64_t noise_cursor) {
-    const bool need_lists = e->P.lj_on || e->P.is_assembly;
     hipError_t st = hipSuccess;
     for (int64_t k = 0; k < len && st == hipSuccess; k++) {
         const int64_t step = s0 + k;
-        if (need_lists && step % e->P.ljpairsupdatefreq == 0 && e->lists_step != step) {
-            hipEvent_t ev{};
-            time_begin(e, "rebuild", &ev);
-            st = launch_rebuild(e);
-            time_end(e, "rebuild", ev);
-            if (st != hipSuccess) break;
-        }
         const float *nz = e->noise ? e->noise + (size_t)6 * e->nb * (noise_cursor + k) : nullptr;
         hipEvent_t ev{};
         if (mlp_active(e)) {
@@ -776,14 +804,18 @@ int mte_run(mte_engine *e, int64_t nsteps) {
             if (rc) return rc;
             e->tea_valid = true;
         }
-        const bool rebuilds = need_lists && rebuild_due(e);
+        if (need_lists && rebuild_due(e)) {
+            // Lists are rebuilt outside the captured steps, and their
+            // capacity flag read, so that an overflow is reported before any
+            // step runs on truncated lists.
+            int rc = rebuild_now(e);
+            if (!rc) rc = check_overflow(e);
+            if (rc) return rc;
+        }
         if (e->use_graphs && !e->noise && !e->timing) {
-            // The rebuild-at-first-step decision is part of the key: a segment
-            // starting on a rebuild step whose lists already exist skips it.
-            const int64_t lj_phase = rebuilds ? e->step % freq : (e->step % freq == 0 ? -1 : e->step % freq);
             const int64_t hi_phase = (e->P.tea_on ? e->step % e->P.tea_epsilon_freq : 0) +
                                      (mlp_now ? 1000003 * (1 + e->step % e->mlp_check_every) : 0);
-            const auto key = std::make_tuple(lj_phase, hi_phase, seg, e->cur);
+            const auto key = std::make_tuple(hi_phase, seg, e->cur);
             auto it = e->graphs.find(key);
             const int cur0 = e->cur;
             if (it == e->graphs.end()) {
@@ -805,7 +837,6 @@ int mte_run(mte_engine *e, int64_t nsteps) {
         } else {
             HIPCHK(enqueue_streamed(e, e->step, seg, e->noise_cursor));
         }
-        if (rebuilds) { e->lists_step = e->step; e->lists_valid = true; }
         if (tea_refresh) {
             int rc = check_tea(e);
             if (rc) return rc;
@@ -945,6 +976,21 @@ int mte_tea_stats(mte_engine *e, double *eps, double *beta) {
     for (int t = 0; t < e->ntraj; t++) {
         if (eps) eps[t] = st[2 * t];
         if (beta) beta[t] = st[2 * t + 1];
+    }
+    return MTE_OK;
+}
+
+int mte_tea_ci(mte_engine *e, float *ci) {
+    if (!e || !ci) return MTE_ERR_ARG;
+    if (!e->P.tea_on || !e->tea_valid) return fail(e, MTE_ERR_STATE, "no TEA refresh has run");
+    HIPCHK(hipSetDevice(e->device));
+    std::vector<float4> c(e->nb);
+    HIPCHK(hipMemcpyAsync(c.data(), e->tea.ci, sizeof(float4) * e->nb, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int b = 0; b < e->nb; b++) {
+        ci[3 * b] = c[b].x;
+        ci[3 * b + 1] = c[b].y;
+        ci[3 * b + 2] = c[b].z;
     }
     return MTE_OK;
 }

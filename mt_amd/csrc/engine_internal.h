@@ -144,9 +144,9 @@ struct mte_engine {
     bool tea_valid = false;
 
     bool timing = false;
-    // Captured step sequences of the streamed path, keyed by
-    // (step mod rebuild period, step mod epsilon period, length, buffer parity).
-    std::map<std::tuple<int64_t, int64_t, int64_t, int>, hipGraphExec_t> graphs;
+    // Captured step sequences of the streamed path (list rebuilds run before
+    // them), keyed by (epsilon/MLP-check phase, length, buffer parity).
+    std::map<std::tuple<int64_t, int64_t, int>, hipGraphExec_t> graphs;
     bool use_graphs = true;
     std::map<std::string, mte::TimingSlot> timers;
     std::vector<void *> allocations;
@@ -181,6 +181,8 @@ float candidate_radius(const mte_params &P);
 hipError_t launch_tea_eps(mte_engine *e);
 hipError_t tea_build_eps_units(mte_engine *e);
 hipError_t upload_resident_order(mte_engine *e, const uint8_t *extra);
+// Most mobile (neither fixed nor extra) beads in one trajectory under these extra flags.
+int resident_mobile_max(const mte_engine *e, const uint8_t *extra);
 hipError_t launch_tea_pairs(mte_engine *e);
 hipError_t launch_tea_finish(mte_engine *e, const float *noise_step);
 hipError_t launch_resident(mte_engine *e, int nsteps, const float *noise);

@@ -172,6 +172,14 @@ __global__ void __launch_bounds__(MAXT)
     k_resident(mte_params P, DevTopo T, DevState S, uint4 *seeds, const float *noise, int nsteps, LatOffsets L,
                int stride, int entry_cap, const int *order) {
     extern __shared__ float4 lds[];
+    // A rebuild that ran out of list capacity stops the dynamics: the state
+    // and RNG streams stay at that rebuild's step, and the host reports the
+    // error with the steps not taken (every thread returns before any
+    // barrier).
+    if (*(volatile const int *)T.overflow) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(T.frozen, nsteps);
+        return;
+    }
     float4 *lx = lds, *ln = lds + stride, *l1 = lds + 2 * stride, *l2 = lds + 3 * stride;
     float4 *rbuf = lds + STATE_ARRAYS * stride;
     int *hoff = reinterpret_cast<int *>(rbuf + RNG_F4);

@@ -78,6 +78,7 @@ SIGNATURES = {
     "mte_get_rng": ([_P, _P], ctypes.c_int),
     "mte_set_rng": ([_P, _P], ctypes.c_int),
     "mte_tea_stats": ([_P, _P, _P], ctypes.c_int),
+    "mte_tea_ci": ([_P, _P], ctypes.c_int),
     "mte_set_timing": ([_P, ctypes.c_int32], ctypes.c_int),
     "mte_kernel_time": ([_P, ctypes.c_char_p, _P, _P], ctypes.c_int),
     "mte_reset_timing": ([_P], ctypes.c_int),
@@ -191,9 +192,16 @@ class Engine:
         return c
 
     def set_flags(self, extra=None, gtp=None, on_tubule=None):
-        e = None if extra is None else np.ascontiguousarray(extra, np.uint8)
-        g = None if gtp is None else np.ascontiguousarray(gtp, np.int32)
-        o = None if on_tubule is None else np.ascontiguousarray(on_tubule, np.int32)
+        def flag(name, a, dt):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dt)
+            if a.size != self.nb:   # mte_set_flags reads one entry per bead
+                raise ValueError(f"{name} has {a.size} elements, the batch has {self.nb} beads")
+            return a
+        e = flag("extra", extra, np.uint8)
+        g = flag("gtp", gtp, np.int32)
+        o = flag("on_tubule", on_tubule, np.int32)
         self._chk(self.lib.mte_set_flags(self.h, _p(e), _p(g), _p(o)))
 
     def mt_length(self, rad_hi, rad_lo, cos_min, apply=False):
@@ -263,6 +271,12 @@ class Engine:
         beta = np.zeros(self.ntraj)
         self._chk(self.lib.mte_tea_stats(self.h, _p(eps), _p(beta)))
         return eps, beta
+
+    def tea_ci(self):
+        """Per-bead row sums of squares of D_hat (B, 3) from the last TEA refresh."""
+        ci = np.zeros((self.nb, 3), np.float32)
+        self._chk(self.lib.mte_tea_ci(self.h, _p(ci)))
+        return ci
 
     def tuning(self) -> dict:
         t = Tuning()

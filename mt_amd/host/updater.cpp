@@ -467,9 +467,34 @@ int run(RunSetup &s, const RunOptions &o) {
     std::vector<mte_coord> r = s.coords, f(nb);
     std::vector<double> energies(7 * (size_t)nt);
     const long long stride = std::llabs(s.stride);
-    bool resumed = !o.resume_file.empty();   // the checkpoint's step had its events before it was written
+    const long long first = step;
     while (step < s.steps) {
-        if (!resumed) {
+        // A checkpoint holds the state at the top of its step, before that
+        // step's list rebuild and events, so a resumed run replays them
+        // exactly as the uninterrupted run does.
+        if (o.checkpoint_every > 0 && step > 0 && step != first && step % o.checkpoint_every == 0) {
+            Checkpoint c;
+            c.step = step;
+            c.rand_calls = g_rand_calls;
+            c.coords.resize(nb);
+            c.rng.resize(8 * nb);
+            if ((rc = mte_get_state(E.e, c.coords.data())) || (rc = mte_get_rng(E.e, c.rng.data())))
+                return fail(E.e, "checkpoint", rc);
+            c.extra = s.extra;
+            c.gtp = s.gtp;
+            c.on_cur = s.on_tubule_cur;
+            c.on_prev = s.on_tubule_prev;
+            c.mt_len.assign(mt_len.begin(), mt_len.end());
+            write_checkpoint(o.checkpoint_file, c, nt, n);
+            if (!o.quiet) std::printf("Checkpoint at step %lld: %s\n", step, o.checkpoint_file.c_str());
+        }
+        // Lists first, as at the top of the reference's step
+        // (src/compute_cuda.cu:1141-1152): the step's dynamics use lists of
+        // the positions before hydrolysis and the concentration update,
+        // whatever the output settings.
+        if ((s.lj_on || s.is_assembly) && step % s.ljpairsupdatefreq == 0 && (rc = mte_rebuild_lists(E.e)))
+            return fail(E.e, "mte_rebuild_lists", rc);
+        {
             if (s.hydrolysis && s.hydrostep > 0 && step % s.hydrostep == 0 && step != 0) {
                 up.hydrolyse();
                 if ((rc = mte_set_flags(E.e, nullptr, s.gtp.data(), nullptr))) return fail(E.e, "mte_set_flags", rc);
@@ -511,24 +536,7 @@ int run(RunSetup &s, const RunOptions &o) {
                     update();
                 }
             }
-            if (o.checkpoint_every > 0 && step > 0 && step % o.checkpoint_every == 0) {
-                Checkpoint c;
-                c.step = step;
-                c.rand_calls = g_rand_calls;
-                c.coords.resize(nb);
-                c.rng.resize(8 * nb);
-                if ((rc = mte_get_state(E.e, c.coords.data())) || (rc = mte_get_rng(E.e, c.rng.data())))
-                    return fail(E.e, "checkpoint", rc);
-                c.extra = s.extra;
-                c.gtp = s.gtp;
-                c.on_cur = s.on_tubule_cur;
-                c.on_prev = s.on_tubule_prev;
-                c.mt_len.assign(mt_len.begin(), mt_len.end());
-                write_checkpoint(o.checkpoint_file, c, nt, n);
-                if (!o.quiet) std::printf("Checkpoint at step %lld: %s\n", step, o.checkpoint_file.c_str());
-            }
         }
-        resumed = false;
         long long next = (step / stride + 1) * stride;
         if (s.hydrolysis && s.hydrostep > 0) next = std::min(next, (step / s.hydrostep + 1) * (long long)s.hydrostep);
         next = std::min(next, s.steps);

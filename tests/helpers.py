@@ -46,3 +46,28 @@ def base_params(**over):
 
 def noise(rng, nsteps, nb):
     return rng.standard_normal((nsteps, nb, 6)).astype(np.float32)
+
+
+# Force-field branches the default workload leaves idle: GDP and GTP bending
+# offsets both non-zero (src/compute_cuda.cu:161-175, :282-283), psi and fi
+# offsets, seam weakening (:447-452), repulsive walls on the cylinder and at
+# both ends (:497-517).
+BRANCH = dict(theta0_gdp=0.2, theta0_gtp=-0.07, psi0=0.05, fi0=-0.03, seam_coeff=2.0, repulsive_walls=True,
+              rep_r=8.0, rep_eps=0.5, rep_leftborder=2.0)
+
+
+def branch_params(**over):
+    return base_params(**dict(BRANCH, **over))
+
+
+def branch_batch(ntraj=2, jitter=0.2, seed=31, n_dimers=12, wall_z=40.0, ang_jitter=0.03):
+    """c1 geometry with per-bead GTP/GDP and on/off-tubule flags drawn at
+    random (so every barrier and theta0 selection case occurs), angle jitter,
+    and an end wall at z = wall_z + rep_leftborder."""
+    b = c1_batch(ntraj=ntraj, jitter=jitter, seed=seed, n_dimers=n_dimers)
+    rng = np.random.default_rng(seed + 1)
+    b.coords[:, 3:6] += rng.normal(0.0, ang_jitter, (b.nb, 3)).astype(np.float32)
+    b.gtp = rng.integers(0, 2, b.nb).astype(np.int32)
+    b.on_tub = rng.integers(0, 2, b.nb).astype(np.int32)
+    b.zs = np.full(b.ntraj, wall_z, np.float32)
+    return b

@@ -163,14 +163,17 @@ def test_driver_mt_len_file_matches_frames(tmp_path, capfd):
         os.chdir(old)
 
 
-def test_checkpoint_resume_continues_exactly(tmp_path, capfd):
-    """engine_checkpoint writes the run's state at step 500 (engine state,
-    HybridTaus streams, flags, the host rand() count); engine_resume continues
-    from it, and every later DCD frame, mt_len.dat row and printed energy
-    equals the uninterrupted run's, with hydrolysis and the tubule test on."""
-    print("case host driver: checkpoint at 500, resume to 1000", flush=True)
-    over = ("tubule_length=yes", "hydrolysis=yes")
-    runs = {}
+@pytest.mark.parametrize("events", [("tubule_length=yes", "hydrolysis=yes"),
+                                    ("tubule_length=yes", "is_const_conc=yes")])
+def test_checkpoint_resume_continues_exactly(tmp_path, capfd, events):
+    """engine_checkpoint writes the run's state at the top of step 500, before
+    that step's list rebuild and events (engine state, HybridTaus streams,
+    flags, the host rand() count); engine_resume replays step 500 from it, and
+    every DCD frame, mt_len.dat row and printed energy from step 500 on equals
+    the uninterrupted run's, with hydrolysis or the constant-concentration
+    update on."""
+    print(f"case host driver: checkpoint at 500, resume to 1000, {events}", flush=True)
+    over = tuple(events)
     for name in ("full", "resumed"):
         d = tmp_path / name
         d.mkdir()
@@ -192,14 +195,44 @@ def test_checkpoint_resume_continues_exactly(tmp_path, capfd):
         for name in ("traj_0.dcd", "traj_1.dcd", "traj_0.dcd_ang", "traj_1.dcd_ang"):
             _, full = read_dcd(str(tmp_path / "full" / name))
             _, res = read_dcd(str(tmp_path / "resumed" / name))
-            assert len(full) == 10 and len(res) == 4
-            for k in range(4):
-                np.testing.assert_array_equal(res[k], full[6 + k])
+            assert len(full) == 10 and len(res) == 5
+            for k in range(5):
+                np.testing.assert_array_equal(res[k], full[5 + k])
         rows_full = open(tmp_path / "full" / "mt_len.dat").read().splitlines()
         rows_res = open(tmp_path / "resumed" / "mt_len.dat").read().splitlines()
-        assert rows_res == rows_full[6:]
+        assert len(rows_full) == 10 and rows_res == rows_full[5:]
         e_full = [l for l in out_full.splitlines() if l.startswith("Energies[")]
         e_res = [l for l in out_res.splitlines() if l.startswith("Energies[")]
-        assert len(e_full) == 20 and e_res == e_full[12:]
+        assert len(e_full) == 20 and e_res == e_full[10:]
     finally:
         os.chdir(old)
+
+
+def test_const_conc_dynamics_do_not_depend_on_output(tmp_path, capfd):
+    """With is_const_conc on, the fixture's two reservoir dimers are put back
+    in the box at step 100, which is also a list-rebuild step.  The lists of
+    that step are built from the positions before the update (the reference
+    rebuilds at the top of the step, src/compute_cuda.cu:1141-1152), so the
+    trajectory is the same whether energies are printed or not."""
+    print("case host driver: is_const_conc yes, output_energy yes vs no", flush=True)
+    frames = {}
+    for flag in ("yes", "no"):
+        d = tmp_path / flag
+        d.mkdir()
+        for f in os.listdir(RUN):
+            shutil.copy(os.path.join(RUN, f), d / f)
+        old = os.getcwd()
+        try:
+            os.chdir(d)
+            capfd.readouterr()
+            host.run("main.conf", overrides=("tubule_length=yes", "is_const_conc=yes", f"output_energy={flag}"),
+                     mode=MODE_STREAMED, max_lj=256, quiet=True)
+            capfd.readouterr()
+            frames[flag] = [read_dcd(f"traj_{t}.dcd")[1] for t in range(2)]
+        finally:
+            os.chdir(old)
+    for t in range(2):
+        # the reservoir beads moved at step 100: the update fired
+        assert np.abs(frames["yes"][t][1][-4:] - frames["yes"][t][0][-4:]).max() > 1.0
+        for k in range(10):
+            np.testing.assert_array_equal(frames["yes"][t][k], frames["no"][t][k])
