@@ -253,6 +253,17 @@ int mte_mlp_get_weights(mte_engine *e, float *w, int64_t n);
 int mte_mlp_local(mte_engine *e, float *features, float *labels);
 /* Inference with the bf16-rounded weights on n feature rows. */
 int mte_mlp_forward(mte_engine *e, const float *features, int64_t n, float *out);
+/* The same inference for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  No counterpart in the
+ * reference.  mte_mlp_forward sizes its work space to the call and may
+ * allocate it; this variant allocates nothing: features and outputs are
+ * staged in a fixed device array of MTE_MLP_NOALLOC_ROWS rows (3.4 MB per
+ * GPU, held until the process ends), and n above that limit is MTE_ERR_ARG.
+ * The results equal mte_mlp_forward's bit for bit.  Every call on a GPU
+ * shares that GPU's array, so calls that overlap on one GPU (two threads, or
+ * two engines on one device) race. */
+#define MTE_MLP_NOALLOC_ROWS 8192
+int mte_mlp_forward_noalloc(mte_engine *e, const float *features, int64_t n, float *out);
 /* Mean squared error and its gradient (MTE_MLP_NPARAM floats) with the FP32
  * weights over n samples. */
 int mte_mlp_loss_grad(mte_engine *e, const float *features, const float *labels, int64_t n, double *loss,

@@ -297,6 +297,10 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
         e->err = "mte_create: stream creation failed";
         return bail(MTE_ERR_HIP);
     }
+    if (noalloc_scratch_init(e->device, &e->noalloc) != hipSuccess) {
+        e->err = "mte_create: hipGetSymbolAddress failed for the inference staging array";
+        return bail(MTE_ERR_HIP);
+    }
 
     // Lateral offsets p1, p2 (src/mt.h:44-50) in double, rounded once.
     const double c = std::cos(2.0 * M_PI / 13.0), s = std::sin(2.0 * M_PI / 13.0);
@@ -1090,6 +1094,23 @@ int mte_mlp_forward(mte_engine *e, const float *features, int64_t n, float *out)
     HIPCHK(hipMemcpyAsync(e->mlp.x, features, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice, e->stream));
     HIPCHK(launch_mlp_forward(e, e->mlp.x, (int)n, e->mlp.y));
     HIPCHK(hipMemcpyAsync(out, e->mlp.y, sizeof(float) * MLP_OUT * n, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return MTE_OK;
+}
+
+int mte_mlp_forward_noalloc(mte_engine *e, const float *features, int64_t n, float *out) {
+    int rc = need_mlp(e);
+    if (rc) return rc;
+    if (!features || !out || n <= 0) return fail(e, MTE_ERR_ARG, "mte_mlp_forward_noalloc: bad arguments");
+    if (n > MLP_NOALLOC_ROWS)
+        return fail(e, MTE_ERR_ARG, "mte_mlp_forward_noalloc: at most MTE_MLP_NOALLOC_ROWS (8192) rows per call");
+    HIPCHK(hipSetDevice(e->device));
+    // every row the kernel reads or writes is set by this call, so nothing
+    // in the shared array needs clearing
+    float *x = e->noalloc, *y = e->noalloc + (size_t)MLP_NOALLOC_ROWS * MLP_IN;
+    HIPCHK(hipMemcpyAsync(x, features, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(launch_mlp_forward(e, x, (int)n, y));
+    HIPCHK(hipMemcpyAsync(out, y, sizeof(float) * MLP_OUT * n, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     return MTE_OK;
 }

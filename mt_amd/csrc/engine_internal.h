@@ -67,6 +67,10 @@ constexpr int CELL_SCAN_SPAN = 4096;
 // MLP-HI network and work buffers (mlp.hip).
 constexpr int MLP_NN = 32, MLP_IN = 96, MLP_H = 128, MLP_OUT = 9;
 constexpr int MLP_NPARAM = MLP_IN * MLP_H + MLP_H + 2 * (MLP_H * MLP_H + MLP_H) + MLP_H * MLP_OUT + MLP_OUT;
+// mte_mlp_forward_noalloc: rows per call (the fixed staging array's size)
+// and the device ordinals whose array address is kept.
+constexpr int MLP_NOALLOC_ROWS = MTE_MLP_NOALLOC_ROWS;
+constexpr int MLP_NOALLOC_MAX_DEVICES = 64;
 struct MlpBuffers {
     float *w = nullptr, *wi = nullptr;        // FP32 master weights, bf16-rounded copy for inference
     uint4 *img = nullptr;                     // inference image: bf16 MFMA fragments + biases (mlp_image_bytes())
@@ -129,6 +133,7 @@ struct mte_engine {
     mte::CellTables cells;
     bool use_cells = false;
     mte::MlpBuffers mlp;
+    float *noalloc = nullptr;   // this device's staging array of mte_mlp_forward_noalloc
     int hi_mode = 0;            // MTE_HI_*
     int mlp_check_every = 100;
     float mlp_tol = 0.02f;
@@ -169,6 +174,9 @@ bool rebuild_traj_plan(int nmax, int lds_limit, int *threads, size_t *lds);
 hipError_t launch_rebuild_cells(mte_engine *e);
 hipError_t launch_mlp_local(mte_engine *e, float *feat, float *targ);
 hipError_t launch_mlp_forward(mte_engine *e, const float *feat, int n, float *out);
+// Address of the device's fixed staging array for mte_mlp_forward_noalloc
+// (hipGetSymbolAddress once per device; call with `device` current).
+hipError_t noalloc_scratch_init(int device, float **ptr);
 hipError_t launch_mlp_step(mte_engine *e, const float *noise_step, float *yout);
 hipError_t launch_mlp_error(mte_engine *e, const float *y, const float *t);
 hipError_t launch_mlp_grad(mte_engine *e, const float *feat, const float *targ, int n);

@@ -19,6 +19,8 @@
 // stored activations, backward, Adam) in FP32 on the vector ALUs.
 #include "engine_internal.h"
 
+#include <mutex>
+
 namespace mte {
 
 constexpr int MLP_BLOCK = 64;
@@ -812,5 +814,32 @@ hipError_t launch_mlp_pack(mte_engine *e) {
 }
 
 size_t mlp_image_bytes() { return NET_IMG_BYTES; }
+
+// ------------------------------------------- inference without allocation
+// Staging of mte_mlp_forward_noalloc: features (MLP_NOALLOC_ROWS x 96) then
+// outputs (MLP_NOALLOC_ROWS x 9), in one fixed device array.  Each GPU the
+// library runs on holds its own copy until the process ends.  Host code
+// never uses the array's name as an address: the per-device address comes
+// from hipGetSymbolAddress below.
+static __device__ __align__(256) float scratch[(size_t)MLP_NOALLOC_ROWS * (MLP_IN + MLP_OUT)];
+
+namespace {
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[MLP_NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+hipError_t noalloc_scratch_init(int device, float **ptr) {
+    *ptr = nullptr;
+    if (device < 0 || device >= MLP_NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {   // once per device, with that device current
+        float *p = nullptr;
+        const hipError_t s = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+        if (s != hipSuccess) return s;
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return hipSuccess;
+}
 
 }  // namespace mte

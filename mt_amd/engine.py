@@ -90,12 +90,14 @@ SIGNATURES = {
     "mte_mlp_get_weights": ([_P, _P, ctypes.c_int64], ctypes.c_int),
     "mte_mlp_local": ([_P, _P, _P], ctypes.c_int),
     "mte_mlp_forward": ([_P, _P, ctypes.c_int64, _P], ctypes.c_int),
+    "mte_mlp_forward_noalloc": ([_P, _P, ctypes.c_int64, _P], ctypes.c_int),
     "mte_mlp_loss_grad": ([_P, _P, _P, ctypes.c_int64, _P, _P], ctypes.c_int),
     "mte_mlp_train": ([_P, _P, _P, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, _P], ctypes.c_int),
     "mte_set_hi_mode": ([_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_float], ctypes.c_int),
     "mte_mlp_status": ([_P, _P, _P, _P], ctypes.c_int),
 }
 MLP_NPARAM, MLP_IN, MLP_OUT = 46601, 96, 9
+MLP_NOALLOC_ROWS = 8192
 HI_DEFAULT, HI_MLP = 0, 1
 
 _LIB = None
@@ -328,6 +330,14 @@ class Engine:
         x = np.ascontiguousarray(feat, np.float32)
         out = np.zeros((len(x), MLP_OUT), np.float32)
         self._chk(self.lib.mte_mlp_forward(self.h, _p(x), len(x), _p(out)))
+        return out
+
+    def mlp_forward_noalloc(self, feat):
+        """mte_mlp_forward_noalloc: the same inference through the device's
+        fixed staging array (at most MLP_NOALLOC_ROWS rows, no allocation)."""
+        x = np.ascontiguousarray(feat, np.float32)
+        out = np.zeros((len(x), MLP_OUT), np.float32)
+        self._chk(self.lib.mte_mlp_forward_noalloc(self.h, _p(x), len(x), _p(out)))
         return out
 
     def mlp_loss_grad(self, feat, targ):
