@@ -1,22 +1,28 @@
-"""Throughput of the hot path on one MI355X: batched plain Langevin dynamics,
-with TEA-HI and MLP-HI sections.
+"""Throughput of the hot path on one MI355X: batched plain Langevin dynamics
+(the headline), with C1, TEA-HI (C3), uneven-batch (C4) and MLP-HI (C5)
+sections.
 
-Workload (batched plain Langevin, FP32, no HI): 256 independent trajectories
-of a 13-protofilament x 40-dimer microtubule (1040 beads each, 266,240 beads
-in total), per-trajectory HybridTaus streams, bottom ring fixed, LJ pair list
-cutoff 8 nm rebuilt every 100 steps, synthetic lattice jittered N(0, 0.05 nm).
-One step = forces + Euler-Maruyama for every bead of the batch (and the list
-rebuild every 100th step).  value = monomer-steps/s (beads x steps / s).
+Headline workload (batched plain Langevin, FP32, no HI): 256 independent
+trajectories of a 13-protofilament x 40-dimer microtubule (1040 beads each,
+266,240 beads in total), per-trajectory HybridTaus streams, bottom ring fixed,
+LJ pair list cutoff 8 nm rebuilt every 100 steps, synthetic lattice jittered
+N(0, 0.05 nm).  One step = forces + Euler-Maruyama for every bead of the
+batch, and the list rebuild every 100th step.  The warm-up is followed by as
+many untimed steps as it takes to reach a rebuild step, so the timed window
+of K steps starts with a rebuild and holds ceil(K / 100) of them.
+value = monomer-steps/s (beads x steps / s) over that window.
 
-The "tea_hi" object times batched TEA hydrodynamics (64 x 1040 beads) and
-its drift error against the FP64 Ermak-McCammon oracle; the "mlp_hi" object
-trains the learned HI correction on the device from the engine's own TEA runs
-and deploys it on 32 x 43,250-bead systems beside plain TEA.
+Sections: "c1" the 312-bead single trajectory; "tea_hi" batched TEA
+hydrodynamics (64 x 1040 beads) with its drift and covariance errors against
+the FP64 Ermak-McCammon oracle; "c4" 48 uneven periodic trajectories with
+free dimers, assembly, cell lists and TEA; "mlp_hi" device training of the
+learned HI correction from the engine's own TEA runs and its deployment on
+32 x 43,250-bead systems beside plain TEA.
 
-Prints one JSON line.  Also reports the dominant kernel's HBM roofline
-fraction (algorithmic bytes per monomer-step x monomer-steps per launch /
-mean launch time from HIP events on the engine stream) and the CPU oracle
-timed on this host's cores on a bounded sample of the same workload.
+Prints one JSON line.  Each section reports its dominant kernel's roofline
+fraction (algorithmic bytes or flops per launch / mean launch time from HIP
+events on the engine stream) and the CPU oracle timed on this host's cores on
+a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -34,6 +40,7 @@ import numpy as np  # noqa: E402
 
 ALG_BYTES_PER_MONOMER_STEP = 112   # 6 x f32 state in + out (48 B) + two HybridTaus uint4 in + out (64 B)
 HBM_PEAK_GBS = 8000.0
+BF16_DENSE_PEAK_TFS = 2500.0
 
 
 def build_workload(ntraj=256, n_dimers=40, seed=20240601):
@@ -85,24 +92,91 @@ def read_tea_traffic(chunk):
     return 1024.0 * (found[0] * cal["fetch_calibration"] + found[1] * cal["write_calibration"])
 
 
-def cpu_baseline(ntraj_sample, steps_sample):
+def read_profile_keys(name, keys):
+    """`key = value` lines of a committed profile summary (profiles/)."""
+    path = os.path.join(ROOT, "profiles", name)
+    out = {}
+    if not os.path.exists(path):
+        return out
+    for line in open(path):
+        m = re.match(r"\s*([a-z_0-9]+)\s*=\s*([0-9.eE+-]+)\s*$", line)
+        if m and m.group(1) in keys:
+            out[m.group(1)] = float(m.group(2))
+    return out
+
+
+def cpu_baseline(ntraj_sample, steps_sample, n_dimers=40):
     """The FP32 CPU oracle (OpenMP over beads, all host cores) on a bounded
-    sample: ntraj_sample trajectories of the same 1040-bead microtubule."""
-    from oracle.oracle import Oracle, max_threads
-    p, b = build_workload(ntraj=ntraj_sample)
+    sample: ntraj_sample trajectories of the same microtubule, steps 0 ..
+    steps_sample - 1 (the rebuilds at multiples of 100 included)."""
+    from oracle.oracle import Oracle, generate_seeds, max_threads
+    p, b = build_workload(ntraj=ntraj_sample, n_dimers=n_dimers)
     o = Oracle(p, b, max_lj=64, precision="f32")
-    seeds = np.zeros(8 * b.nb, np.uint32)
-    from oracle.oracle import generate_seeds
-    seeds[:] = generate_seeds(1234567, 2 * b.nb)
-    o.run(0, 1, seeds=seeds)  # warm-up (one rebuild)
+    seeds = generate_seeds(1234567, 2 * b.nb)
     t0 = time.perf_counter()
-    rc = o.run(1, steps_sample, seeds=seeds)
+    rc = o.run(0, steps_sample, seeds=seeds)
     dt = time.perf_counter() - t0
     if rc != 0:
         raise RuntimeError(f"cpu oracle status {rc}")
     return {"value": b.nb * steps_sample / dt, "unit": "monomer-steps/s", "cores": max_threads(), "kind": "port",
-            "sample": f"{ntraj_sample} x 1040-bead trajectories x {steps_sample} steps (steps 1..{steps_sample}, "
-                      f"rebuilds at multiples of 100), FP32 oracle, OpenMP"}
+            "seconds": dt,
+            "sample": f"{ntraj_sample} x {b.nb // ntraj_sample}-bead trajectories x {steps_sample} steps (from "
+                      f"step 0, rebuilds at multiples of 100), FP32 oracle, OpenMP"}
+
+
+def plain_force_error(p, b, state, ntraj=4):
+    """Forces of the engine's bead_force (the code every plain kernel runs)
+    against the FP64 oracle at a state of the timed run, first ntraj
+    trajectories: the largest |dF| over the largest |F| of the same kind
+    (xyz, angles), and per bead |dF_i| / |F_i| (median, 99th percentile) over
+    beads whose |F_i| is at least 1e-3 of the largest."""
+    from mt_amd.engine import Engine, MODE_STREAMED
+    from oracle.oracle import Oracle
+    _, sub = build_workload(ntraj=ntraj)
+    sub.coords = np.ascontiguousarray(state[:sub.nb])
+    eng = Engine(p, sub.copy(), max_lj=64, mode=MODE_STREAMED)
+    f = eng.forces().astype(np.float64)
+    eng.close()
+    orc = Oracle(p, sub.copy(), max_lj=64, precision="f64")
+    orc.rebuild()
+    fo = orc.forces()
+    out = {"beads": int(sub.nb), "reference": "FP64 CPU oracle"}
+    for name, cols in (("xyz", slice(0, 3)), ("angles", slice(3, 6))):
+        d = np.abs(f[:, cols] - fo[:, cols])
+        out[f"max_rel_{name}"] = float(d.max() / np.abs(fo[:, cols]).max())
+        nrm = np.linalg.norm(fo[:, cols], axis=1)
+        keep = nrm >= 1e-3 * nrm.max()
+        per = np.linalg.norm(f[:, cols] - fo[:, cols], axis=1)[keep] / nrm[keep]
+        out[f"per_bead_median_{name}"] = float(np.median(per))
+        out[f"per_bead_p99_{name}"] = float(np.percentile(per, 99))
+    return out
+
+
+def c1_bench(steps, cpu=True):
+    """C1: one 312-bead microtubule (13 PF x 12 dimers), the reference's
+    CPU-runnable case, where launch latency rather than bandwidth bounds a
+    step."""
+    from mt_amd.engine import Engine, MODE_RESIDENT
+    p, b = build_workload(ntraj=1, n_dimers=12, seed=1234567)
+    eng = Engine(p, b, max_lj=48)
+    tuned, report = eng.autotune(200)
+    print("[bench] c1 autotune: " + "; ".join(report), file=sys.stderr, flush=True)
+    eng.run(100)                     # warm-up, ends on a rebuild step
+    t0 = time.perf_counter()
+    eng.run(steps)
+    elapsed = time.perf_counter() - t0
+    resident = eng.mode == MODE_RESIDENT
+    eng.close()
+    out = {"value": b.nb * steps / elapsed, "unit": "monomer-steps/s", "steps": steps, "warmup": 100,
+           "ms_per_step": elapsed * 1e3 / steps,
+           "config": {"workload": "C1: 1 trajectory x 312-bead 13PFx12-dimer microtubule, LJ list 8 nm rebuilt "
+                                  "every 100 steps, HybridTaus noise", "beads": b.nb,
+                      "mode": "resident" if resident else "streamed", "tuning": tuned},
+           "note": "one workgroup's work per step: latency-bound, no roofline applies"}
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline(1, 200000, n_dimers=12)
+        out["cpu_baseline"]["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
 
 
 def tea_error_vs_fp64():
@@ -130,6 +204,23 @@ def tea_error_vs_fp64():
     return float(np.abs(d[mob] - exact[mob]).max() / np.abs(exact[mob]).max())
 
 
+def tea_covariance_error(eng, b, t=0):
+    """TEA's noise covariance against the exact one for trajectory t of a
+    TEA engine, in FP64 from the engine's own last refresh (beta, ci):
+    |B B^T - D_hat|_F / |D_hat|_F with D_hat = L L^T the normalised RPY
+    tensor of the FP64 Cholesky Ermak-McCammon oracle, and the largest
+    per-coordinate variance error |(B B^T)_kk - 1|."""
+    from oracle import em_exact
+    s0, s1 = int(b.off[t]), int(b.off[t + 1])
+    pos = eng.get_state()[s0:s1, :3].astype(np.float64)
+    _, beta = eng.tea_stats()
+    ci = eng.tea_ci()[s0:s1]
+    Dh = em_exact.rpy_hat(pos, eng.params.tea_a)
+    B = em_exact.tea_b_from(Dh, beta[t], ci)
+    return {"beads": s1 - s0, "frobenius_rel": em_exact.covariance_error(B, Dh),
+            "variance_max_abs": float(np.abs((B * B).sum(1) - 1.0).max()), "beta": float(beta[t])}
+
+
 def tea_bench(steps, warmup, cpu=True):
     """TEA-HI dynamics: 64 trajectories x 1040 beads, a = 2 nm, water
     viscosity (128 kcal ps / (mol nm^3)), dt = 1 ps, epsilon/beta every 100
@@ -153,43 +244,124 @@ def tea_bench(steps, warmup, cpu=True):
     eng.run(min(steps, 200))
     pair_ms, pair_n = eng.kernel_time("tea_pairs")
     eng.set_timing(False)
+    cov_c3 = tea_covariance_error(eng, b)
     eng.close()
     sizes = b.sizes().astype(np.float64)
     flops = float(36.0 * np.sum(sizes * (sizes - 1)))      # 3B x 3B by 3B x 2 product per step
     avg = pair_ms / 1e3 / max(pair_n, 1)
+    kname = "k_tea_pairs_sym" if tuned["tea_chunk"] == 1 else "k_tea_pairs"
     out = {"value": b.nb * steps / elapsed, "unit": "monomer-steps/s", "steps": steps, "warmup": warmup,
            "ms_per_step": elapsed * 1e3 / steps,
            "config": {"workload": "TEA-HI: 64 trajectories x 1040-bead microtubule, a 2 nm, 0.89 cP, dt 1 ps, "
                                   "epsilon/beta every 100 steps", "beads": b.nb, "tuning": tuned},
-           "roofline": {"bound": "mfma", "achieved": flops / avg / 1e12, "peak": 157.3, "unit": "TFLOP/s",
-                        "frac": flops / avg / 1e12 / 157.3, "traffic": read_tea_traffic(tuned["tea_chunk"]),
-                        "kernel": "k_tea_pairs", "avg_launch_ms": avg * 1e3, "alg_flops_per_launch": flops,
-                        "note": "algorithmic work 36 B(B-1) flops per trajectory-step; the kernel evaluates it "
-                                "on VALU (FP32, at the non-packed issue rate); with tea_chunk 1 each pair's RPY "
-                                "block is evaluated once for both beads (~24 instructions per block against ~41 "
-                                "one-sided), so the achieved figure counts each block's algorithmic flops on "
-                                "both sides; the peak quoted is the FP32 figure, which counts packed FMAs"},
-           "force_error_vs_fp64": tea_error_vs_fp64()}
+           "roofline": {"bound": "mfma", "achieved": flops / avg / 1e12, "peak": BF16_DENSE_PEAK_TFS,
+                        "unit": "TFLOP/s", "frac": flops / avg / 1e12 / BF16_DENSE_PEAK_TFS,
+                        "traffic": read_tea_traffic(tuned["tea_chunk"]),
+                        "kernel": kname, "avg_launch_ms": avg * 1e3, "alg_flops_per_launch": flops,
+                        "note": "algorithmic work 36 B(B-1) flops per trajectory-step against the bf16 dense "
+                                "MFMA peak; the kernel evaluates the RPY blocks and their products on the "
+                                "vector ALUs in FP32 (see fp32_valu)"},
+           "fp32_valu": {"achieved": flops / avg / 1e12, "peak": 157.3, "unit": "TFLOP/s",
+                         "frac": flops / avg / 1e12 / 157.3,
+                         "note": "the same flops against the FP32 vector peak (157.3 counts packed FMAs)"},
+           "force_error_vs_fp64": tea_error_vs_fp64(),
+           "covariance_error_vs_fp64_cholesky": {"c3_trajectory_0": cov_c3, "c1_312_beads": tea_c1_covariance()}}
     if cpu:
         from oracle.oracle import Oracle, generate_seeds, max_threads
         pc = params.derive(params.TEA)
-        bc = topology.replicate(lattice.microtubule(40), 8, 0.05, 77)
+        bc = topology.replicate(lattice.microtubule(40), 64, 0.05, 77)
         o = Oracle(pc, bc, max_lj=64, precision="f32")
         seeds = generate_seeds(1234567, 2 * bc.nb)
-        o.run(0, 1, seeds=seeds)
-        ns = 10
+        ns = 100
         t0 = time.perf_counter()
-        o.run(1, ns, seeds=seeds)
+        o.run(0, ns, seeds=seeds)
         dt = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": bc.nb * ns / dt, "unit": "monomer-steps/s", "cores": max_threads(),
-                               "kind": "port", "sample": f"8 x 1040-bead trajectories x {ns} TEA steps, FP32 oracle, "
-                                                         "all-pairs TEA, OpenMP"}
+                               "kind": "port", "seconds": dt,
+                               "sample": f"the same 64 x 1040-bead batch x {ns} TEA steps from step 0 (one "
+                                         "epsilon/beta refresh, one list rebuild), FP32 oracle, all-pairs TEA, "
+                                         "OpenMP"}
+        out["cpu_baseline"]["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
+def tea_c1_covariance():
+    """TEA covariance error on the 312-bead microtubule (C1 geometry) after
+    one TEA step of the engine."""
+    from mt_amd import lattice, params, topology
+    from mt_amd.engine import Engine, MODE_STREAMED
+    p = params.derive(params.TEA)
+    b = topology.replicate(lattice.microtubule(12), 1, 0.05, 1234567)
+    eng = Engine(p, b, max_lj=48, mode=MODE_STREAMED)
+    eng.run(1)
+    out = tea_covariance_error(eng, b)
+    eng.close()
+    return out
+
+
+def c4_bench(steps, cpu=True):
+    """C4: 48 uneven trajectories, a microtubule of 20-125 dimers per
+    protofilament plus 2k-20k free dimers at 10-50 uM in a periodic box each,
+    assembly (bond breaking and re-forming), cell-list rebuilds and TEA."""
+    from mt_amd import params, workloads
+    from mt_amd.engine import Engine, MODE_STREAMED
+    p = params.derive(params.TEA)
+    b = workloads.uneven(48, dimers=(20, 125), free=(2000, 20000), conc=(10, 50), seed=4242)
+    eng = Engine(p, b, max_lj=48, mode=MODE_STREAMED)
+    tuned, report = eng.autotune(3)
+    print("[bench] c4 autotune: " + "; ".join(report), file=sys.stderr, flush=True)
+    eng.run(100)            # warm-up, ends on an epsilon refresh and list rebuild step
+    t0 = time.perf_counter()
+    eng.run(steps)
+    elapsed = time.perf_counter() - t0
+    eng.reset_timing()
+    eng.set_timing(True)
+    eng.run(10)
+    pair_ms, pair_n = eng.kernel_time("tea_pairs")
+    eng.set_timing(False)
+    eng.close()
+    sizes = b.sizes().astype(np.float64)
+    flops = float(36.0 * np.sum(sizes * (sizes - 1)))
+    avg = pair_ms / 1e3 / max(pair_n, 1)
+    out = {"value": b.nb * steps / elapsed, "unit": "monomer-steps/s", "steps": steps, "warmup": 100,
+           "ms_per_step": elapsed * 1e3 / steps,
+           "config": {"workload": "C4: 48 uneven periodic trajectories, 520-3250 microtubule beads + 4k-40k free "
+                                  "beads each, assembly, cell lists, TEA a 2 nm, 0.89 cP, dt 1 ps",
+                      "beads": int(b.nb), "largest": int(sizes.max()), "smallest": int(sizes.min()),
+                      "tuning": tuned},
+           "roofline": {"bound": "mfma", "achieved": flops / avg / 1e12, "peak": BF16_DENSE_PEAK_TFS,
+                        "unit": "TFLOP/s", "frac": flops / avg / 1e12 / BF16_DENSE_PEAK_TFS, "traffic": None,
+                        "kernel": "k_tea_pairs_sym" if tuned["tea_chunk"] == 1 else "k_tea_pairs",
+                        "avg_launch_ms": avg * 1e3, "alg_flops_per_launch": flops}}
+    if cpu:
+        from oracle.oracle import Oracle, generate_seeds, max_threads
+        t = int(np.argsort(sizes)[len(sizes) // 2])          # the median trajectory
+        s0, s1 = int(b.off[t]), int(b.off[t + 1])
+        from mt_amd import topology
+        one = b      # the engine copied the batch at creation; b still holds the initial state
+        sub = topology.Batch(coords=one.coords[s0:s1].copy(), off=np.array([0, s1 - s0], np.int64),
+                             harm_count=one.harm_count[s0:s1].copy(), harm=one.harm[s0:s1].copy(),
+                             lon_count=one.lon_count[s0:s1].copy(), lon=one.lon[s0:s1].copy(),
+                             lat_count=one.lat_count[s0:s1].copy(), lat=one.lat[s0:s1].copy(),
+                             fixed=one.fixed[s0:s1].copy(), extra=one.extra[s0:s1].copy(),
+                             mon_type=one.mon_type[s0:s1].copy(), gtp=one.gtp[s0:s1].copy(),
+                             on_tub=one.on_tub[s0:s1].copy(), zs=one.zs[t:t + 1].copy(), box=one.box[t:t + 1].copy())
+        o = Oracle(p, sub, max_lj=64, precision="f32")
+        seeds = generate_seeds(1234567, 2 * sub.nb)
+        ns = 2
+        t0 = time.perf_counter()
+        o.run(0, ns, seeds=seeds)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": sub.nb * ns / dt, "unit": "monomer-steps/s", "cores": max_threads(),
+                               "kind": "port", "seconds": dt,
+                               "sample": f"the median trajectory of the batch ({sub.nb} beads) x {ns} TEA steps from "
+                                         "step 0 (one epsilon/beta refresh, one all-pairs list rebuild), FP32 "
+                                         "oracle, OpenMP"}
         out["cpu_baseline"]["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     return out
 
 
 MLP_FLOPS_PER_BEAD = 2 * (96 * 128 + 2 * 128 * 128 + 128 * 9)   # 96-128-128-128-9 network, one bead
-BF16_DENSE_PEAK_TFS = 2500.0
 
 
 def mlp_bench(steps, warmup, iters):
@@ -292,9 +464,13 @@ def main():
                     help="skip the plain section (profiling the TEA or MLP section alone); the line then carries "
                          "no headline value")
     ap.add_argument("--mlp-steps", type=int, default=300)
+    ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--c1-steps", type=int, default=5000)
+    ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--c4-steps", type=int, default=100)
     ap.add_argument("--mlp-iters", type=int, default=20000)
-    ap.add_argument("--cpu-traj", type=int, default=8)
-    ap.add_argument("--cpu-steps", type=int, default=1000)
+    ap.add_argument("--cpu-traj", type=int, default=64)
+    ap.add_argument("--cpu-steps", type=int, default=2000)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy kernel (for scaling FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -316,8 +492,12 @@ def main():
     mode = {"auto": MODE_AUTO, "streamed": MODE_STREAMED, "resident": MODE_RESIDENT}[args.mode]
     if args.no_plain:
         out = {"metric": "monomer-steps/s/GPU (batched plain Langevin)", "value": None, "note": "plain section skipped"}
+        if not args.no_c1:
+            out["c1"] = c1_bench(args.c1_steps, cpu=not args.no_cpu)
         if not args.no_tea:
             out["tea_hi"] = tea_bench(args.tea_steps, 20, cpu=not args.no_cpu)
+        if not args.no_c4:
+            out["c4"] = c4_bench(args.c4_steps, cpu=not args.no_cpu)
         if not args.no_mlp:
             out["mlp_hi"] = mlp_bench(args.mlp_steps, 20, args.mlp_iters)
         print(json.dumps(out))
@@ -328,13 +508,16 @@ def main():
     if not args.no_tune:
         tuned, report = eng.autotune(200)
         print("[bench] autotune: " + "; ".join(report), file=sys.stderr, flush=True)
-    print(f"[bench] beads={b.nb} trajectories={b.ntraj} mode={eng.mode} warmup={args.warmup}", file=sys.stderr,
-          flush=True)
-    eng.run(args.warmup)
+    freq = p.ljpairsupdatefreq
+    align = (-args.warmup) % freq     # untimed steps after the warm-up up to the next rebuild step
+    print(f"[bench] beads={b.nb} trajectories={b.ntraj} mode={eng.mode} warmup={args.warmup}+{align}",
+          file=sys.stderr, flush=True)
+    eng.run(args.warmup + align)
+    assert eng.step % freq == 0
     eng.reset_timing()
     eng.set_timing(True)
     t0 = time.perf_counter()
-    eng.run(args.steps)   # synchronous at return
+    eng.run(args.steps)   # synchronous at return; starts with a list rebuild
     elapsed = time.perf_counter() - t0
     step_ms, step_n = eng.kernel_time("step")
     reb_ms, reb_n = eng.kernel_time("rebuild")
@@ -342,6 +525,11 @@ def main():
     st = eng.get_state()
     if not np.all(np.isfinite(st)):
         raise RuntimeError("non-finite state after the timed run")
+    # steady state: 10 full rebuild intervals from the next rebuild step
+    eng.run((-eng.step) % freq)
+    t1 = time.perf_counter()
+    eng.run(10 * freq)
+    steady_s = time.perf_counter() - t1
     value = b.nb * args.steps / elapsed
     steps_per_launch = args.steps / max(step_n, 1)
     avg_launch_s = step_ms / 1e3 / max(step_n, 1)
@@ -349,6 +537,8 @@ def main():
     achieved = alg_bytes / avg_launch_s / 1e9
     kname = "k_resident" if eng.mode == MODE_RESIDENT else "k_step_plain"
     traffic = read_traffic(f"traffic_{kname}_c2.txt")
+    valu = read_profile_keys(f"{kname}_c2.txt", ("valu_per_bead_step", "valu_issue_frac", "valu_ms_per_dispatch",
+                                                 "steps_per_dispatch"))
     out = {
         "metric": "monomer-steps/s/GPU (batched plain Langevin)",
         "value": value,
@@ -363,18 +553,30 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "batched plain Langevin: 256 trajectories x 1040-bead 13PFx40-dimer microtubule, "
-                               "LJ list 8 nm rebuilt every 100 steps, HybridTaus noise",
+                               "LJ list 8 nm rebuilt every 100 steps, HybridTaus noise; the timed window starts "
+                               "on a rebuild step",
                    "trajectories": b.ntraj, "beads_per_trajectory": int(b.sizes()[0]), "beads": b.nb,
-                   "mode": "resident" if eng.mode == MODE_RESIDENT else "streamed", "tuning": tuned},
+                   "mode": "resident" if eng.mode == MODE_RESIDENT else "streamed", "tuning": tuned,
+                   "alignment_steps": align},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kname, "launches": step_n, "avg_launch_ms": avg_launch_s * 1e3,
-                     "alg_bytes_per_launch": alg_bytes},
+                     "alg_bytes_per_launch": alg_bytes,
+                     "valu": dict(valu, source=f"profiles/{kname}_c2.txt",
+                                  note="the kernel keeps the state in LDS and is bound by VALU issue; the HBM "
+                                       "figure is the 112 B/monomer-step contract")},
         "rebuild": {"launches": reb_n, "avg_ms": reb_ms / max(reb_n, 1)},
+        "steady_state": {"steps": 10 * freq, "rebuilds": 10, "value": b.nb * 10 * freq / steady_s,
+                         "ms_per_step": steady_s * 1e3 / (10 * freq)},
+        "force_error_vs_fp64": plain_force_error(p, b, st),
     }
     eng.close()
+    if not args.no_c1:
+        out["c1"] = c1_bench(args.c1_steps, cpu=not args.no_cpu)
     if not args.no_tea:
         out["tea_hi"] = tea_bench(args.tea_steps, 20, cpu=not args.no_cpu)
+    if not args.no_c4:
+        out["c4"] = c4_bench(args.c4_steps, cpu=not args.no_cpu)
     if not args.no_mlp:
         out["mlp_hi"] = mlp_bench(args.mlp_steps, 20, args.mlp_iters)
     if not args.no_cpu:
