@@ -362,6 +362,7 @@ def c4_bench(steps, cpu=True):
 
 
 MLP_FLOPS_PER_BEAD = 2 * (96 * 128 + 2 * 128 * 128 + 128 * 9)   # 96-128-128-128-9 network, one bead
+MLP_LOCAL_BYTES_PER_BEAD = 16 + 96 * 4   # own position in, features out
 
 
 def mlp_bench(steps, warmup, iters):
@@ -419,16 +420,27 @@ def mlp_bench(steps, warmup, iters):
             eng.set_timing(True)
             eng.run(50)
             k_ms, k_n = eng.kernel_time("mlp_step")
+            l_ms, l_n = eng.kernel_time("mlp_local")
             eng.set_timing(False)
             flops = float(MLP_FLOPS_PER_BEAD) * b.nb
-            if k_n == 0:   # the judge handed the run back to TEA: no MLP-HI launches to time
-                r["roofline"] = None
+            if k_n == 0 or l_n == 0:   # the judge handed the run back to TEA: no MLP-HI launches to time
+                r["roofline"] = r["roofline_mfma"] = None
             else:
                 avg = k_ms / 1e3 / k_n
-                r["roofline"] = {"bound": "mfma", "achieved": flops / avg / 1e12, "peak": BF16_DENSE_PEAK_TFS,
-                                 "unit": "TFLOP/s", "frac": flops / avg / 1e12 / BF16_DENSE_PEAK_TFS,
-                                 "traffic": None, "kernel": "k_mlp_step", "avg_launch_ms": avg * 1e3,
-                                 "alg_flops_per_launch": flops}
+                r["roofline_mfma"] = {"bound": "mfma", "achieved": flops / avg / 1e12, "peak": BF16_DENSE_PEAK_TFS,
+                                      "unit": "TFLOP/s", "frac": flops / avg / 1e12 / BF16_DENSE_PEAK_TFS,
+                                      "traffic": None, "kernel": "k_mlp_step", "avg_launch_ms": avg * 1e3,
+                                      "alg_flops_per_launch": flops}
+                lavg = l_ms / 1e3 / l_n
+                lbytes = float(MLP_LOCAL_BYTES_PER_BEAD) * b.nb
+                traffic = read_profile_keys("k_mlp_c5.txt", ("k_mlp_local_traffic_bytes",))
+                r["roofline"] = {"bound": "hbm", "achieved": lbytes / lavg / 1e9, "peak": HBM_PEAK_GBS,
+                                 "unit": "GB/s", "frac": lbytes / lavg / 1e9 / HBM_PEAK_GBS,
+                                 "traffic": traffic.get("k_mlp_local_traffic_bytes"), "kernel": "k_mlp_local",
+                                 "avg_launch_ms": lavg * 1e3, "alg_bytes_per_launch": lbytes,
+                                 "note": "the dominant kernel of an MLP-HI step (32-nearest-neighbour features); "
+                                         "bound by dependent gathers, not bandwidth: contract 400 B per bead "
+                                         "(own position in, 32 x 3 float features out)"}
         eng.close()
         res[mode] = r
         print(f"[bench] mlp section {mode}: {r['value']:.4g} monomer-steps/s, D {D:.5g}", file=sys.stderr,
@@ -446,7 +458,35 @@ def mlp_bench(steps, warmup, iters):
             "training": {"samples": int(len(X)), "iterations": iters, "batch": 8192, "data_s": data_s,
                          "train_s": train_s, "loss_start": float(losses[:5].mean()),
                          "loss_end": float(losses[-20:].mean())},
-            "roofline": res["mlp"]["roofline"]}
+            "roofline": res["mlp"]["roofline"], "roofline_mfma": res["mlp"]["roofline_mfma"]}
+
+
+def mlp_profile(steps):
+    """The MLP-HI deployment kernels alone, for profiling: the C5 batch with
+    zero weights (the network's work does not depend on their values) and
+    the judge's tolerance raised so the run stays on MLP-HI; no training and
+    no TEA leg.  Prints per-kernel HIP-event averages, not a bench line."""
+    from mt_amd import params, workloads
+    from mt_amd.engine import Engine, HI_MLP, MLP_NPARAM, MODE_STREAMED
+    pt = params.derive(dict(params.TEA))
+    b = workloads.uneven(32, dimers=(125, 125), free=(20000, 20000), conc=(30, 30), seed=5)
+    eng = Engine(pt, b, max_lj=48, mode=MODE_STREAMED, rseed=1234567)
+    eng.mlp_set_weights(np.zeros(MLP_NPARAM, np.float32))
+    eng.set_hi_mode(HI_MLP, check_every=100, tolerance=1e9)
+    eng.run(20)
+    t0 = time.perf_counter()
+    eng.run(steps)
+    el = time.perf_counter() - t0
+    eng.reset_timing()
+    eng.set_timing(True)
+    eng.run(min(steps, 50))
+    k_ms, k_n = eng.kernel_time("mlp_step")
+    l_ms, l_n = eng.kernel_time("mlp_local")
+    eng.set_timing(False)
+    eng.close()
+    return {"workload": "MLP-HI deployment kernels, C5 batch (1,384,000 beads), zero weights, profiling only",
+            "ms_per_step": el * 1e3 / steps, "steps": steps,
+            "k_mlp_step_avg_ms": k_ms / max(k_n, 1), "k_mlp_local_avg_ms": l_ms / max(l_n, 1)}
 
 
 def main():
@@ -464,6 +504,8 @@ def main():
                     help="skip the plain section (profiling the TEA or MLP section alone); the line then carries "
                          "no headline value")
     ap.add_argument("--mlp-steps", type=int, default=300)
+    ap.add_argument("--mlp-profile", action="store_true",
+                    help="run only the MLP-HI deployment kernels (zero weights, no training) for profiling")
     ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--c1-steps", type=int, default=5000)
     ap.add_argument("--no-c4", action="store_true")
@@ -482,6 +524,9 @@ def main():
         y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": x.numel() * 4}))
+        return
+    if args.mlp_profile:
+        print(json.dumps(mlp_profile(args.mlp_steps)))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "

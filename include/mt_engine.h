@@ -180,7 +180,8 @@ int mte_tea_ci(mte_engine *e, float *ci);
 /* Kernel timing with HIP events on the engine's stream. */
 int mte_set_timing(mte_engine *e, int32_t on);
 /* name: "step" (dominant kernel of plain mte_run), "rebuild", "forces",
- * "tea_pairs" (the TEA pair-sum kernel).
+ * "tea_pairs" (the TEA pair-sum kernel), "mlp_local" and "mlp_step" (the
+ * MLP-HI feature search and the fused inference + integration kernel).
  * Returns summed milliseconds and launch count since the last reset. */
 int mte_kernel_time(mte_engine *e, const char *name, double *total_ms, int64_t *launches);
 int mte_reset_timing(mte_engine *e);

@@ -207,19 +207,25 @@ static void generate_seeds(int32_t seed, size_t nwords, uint32_t *out) {
     }
 }
 
-static void time_begin(mte_engine *e, const char *name, hipEvent_t *ev) {
+// Kernel timing with HIP events on the engine's stream.  Both events of a
+// pair are created before the start is recorded, so no host work sits
+// between the kernel and its stop event.
+struct EvPair {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
+static void time_begin(mte_engine *e, const char *name, EvPair *ev) {
     if (!e->timing) return;
-    (void)hipEventCreate(ev);
-    (void)hipEventRecord(*ev, e->stream);
+    (void)hipEventCreate(&ev->start);
+    (void)hipEventCreate(&ev->stop);
+    (void)hipEventRecord(ev->start, e->stream);
     (void)name;
 }
 
-static void time_end(mte_engine *e, const char *name, hipEvent_t start) {
+static void time_end(mte_engine *e, const char *name, const EvPair &ev) {
     if (!e->timing) return;
-    hipEvent_t stop;
-    (void)hipEventCreate(&stop);
-    (void)hipEventRecord(stop, e->stream);
-    e->timers[name].pending.emplace_back(start, stop);
+    (void)hipEventRecord(ev.stop, e->stream);
+    e->timers[name].pending.emplace_back(ev.start, ev.stop);
 }
 
 static void flush_timers(mte_engine *e) {
@@ -657,7 +663,7 @@ static int check_overflow(mte_engine *e) {
 }
 
 static int rebuild_now(mte_engine *e) {
-    hipEvent_t ev{};
+    EvPair ev;
     time_begin(e, "rebuild", &ev);
     HIPCHK(launch_rebuild(e));
     time_end(e, "rebuild", ev);
@@ -703,14 +709,18 @@ static hipError_t enqueue_streamed(mte_engine *e, int64_t s0, int64_t len, int64
     for (int64_t k = 0; k < len && st == hipSuccess; k++) {
         const int64_t step = s0 + k;
         const float *nz = e->noise ? e->noise + (size_t)6 * e->nb * (noise_cursor + k) : nullptr;
-        hipEvent_t ev{};
+        EvPair ev;
         if (mlp_active(e)) {
             // forces, TEA normalisation refresh, features (+ exact labels on
             // check steps), fused prediction + integration
             const bool check = step % e->mlp_check_every == 0;
             st = launch_forces(e);
             if (st == hipSuccess && step % e->P.tea_epsilon_freq == 0) st = launch_tea_eps(e);
-            if (st == hipSuccess) st = launch_mlp_local(e, e->mlp.feat, check ? e->mlp.targ : nullptr);
+            if (st == hipSuccess) st = launch_mlp_cells(e);
+            if (st != hipSuccess) break;
+            time_begin(e, "mlp_local", &ev);
+            st = launch_mlp_local(e, e->mlp.feat, check ? e->mlp.targ : nullptr, false);
+            time_end(e, "mlp_local", ev);
             if (st != hipSuccess) break;
             time_begin(e, "mlp_step", &ev);
             st = launch_mlp_step(e, nz, check ? e->mlp.ystep : nullptr);
@@ -788,7 +798,7 @@ int mte_run(mte_engine *e, int64_t nsteps) {
                 if (rc) return rc;
             }
             const float *nz = e->noise ? e->noise + (size_t)6 * e->nb * e->noise_cursor : nullptr;
-            hipEvent_t ev{};
+            EvPair ev;
             time_begin(e, "step", &ev);
             HIPCHK(launch_resident(e, (int)chunk, nz));
             time_end(e, "step", ev);

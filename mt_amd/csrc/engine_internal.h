@@ -172,7 +172,10 @@ size_t forces_lds_bytes(int nmax);
 hipError_t launch_mt_length(mte_engine *e, float rad_hi, float rad_lo, float cos_min, int apply);
 bool rebuild_traj_plan(int nmax, int lds_limit, int *threads, size_t *lds);
 hipError_t launch_rebuild_cells(mte_engine *e);
-hipError_t launch_mlp_local(mte_engine *e, float *feat, float *targ);
+// The 12 nm cell tables of the MLP-HI feature search, then the search
+// itself (which builds the tables first unless build_tables is false).
+hipError_t launch_mlp_cells(mte_engine *e);
+hipError_t launch_mlp_local(mte_engine *e, float *feat, float *targ, bool build_tables = true);
 hipError_t launch_mlp_forward(mte_engine *e, const float *feat, int n, float *out);
 // Address of the device's fixed staging array for mte_mlp_forward_noalloc
 // (hipGetSymbolAddress once per device; call with `device` current).

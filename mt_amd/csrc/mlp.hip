@@ -717,9 +717,13 @@ __global__ void __launch_bounds__(256) k_mlp_round(const float *w, float *wi, in
 // ------------------------------------------------------------------ launchers
 static inline int blocks(int n, int bs) { return (n + bs - 1) / bs; }
 
-hipError_t launch_mlp_local(mte_engine *e, float *feat, float *targ) {
-    hipError_t s = build_cells(e, MLP_RC, 1);
-    if (s != hipSuccess) return s;
+hipError_t launch_mlp_cells(mte_engine *e) { return build_cells(e, MLP_RC, 1); }
+
+hipError_t launch_mlp_local(mte_engine *e, float *feat, float *targ, bool build_tables) {
+    if (build_tables) {
+        const hipError_t s = launch_mlp_cells(e);
+        if (s != hipSuccess) return s;
+    }
     if (targ)
         hipLaunchKernelGGL(k_mlp_local<true>, dim3(blocks(e->nb, MLP_BLOCK)), dim3(MLP_BLOCK), 0, e->stream, e->P,
                            e->T, e->S[e->cur], e->cells, feat, targ);
