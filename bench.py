@@ -71,6 +71,13 @@ def read_traffic(name):
                      + vals["write_kib_per_dispatch"] * vals.get("write_calibration", 1.0))
 
 
+TEA_KERNELS = {1: "k_tea_pairs_sym", 2: "k_tea_pairs_mfma"}   # tea_chunk -> pair kernel (else k_tea_pairs)
+
+
+def tea_kernel(chunk):
+    return TEA_KERNELS.get(chunk, "k_tea_pairs")
+
+
 def read_tea_traffic(chunk):
     """HBM bytes per k_tea_pairs dispatch for j-chunks of `chunk` beads, from
     the committed rocprofv3 --pmc summary (FETCH_SIZE, WRITE_SIZE per chunk
@@ -249,7 +256,7 @@ def tea_bench(steps, warmup, cpu=True):
     sizes = b.sizes().astype(np.float64)
     flops = float(36.0 * np.sum(sizes * (sizes - 1)))      # 3B x 3B by 3B x 2 product per step
     avg = pair_ms / 1e3 / max(pair_n, 1)
-    kname = "k_tea_pairs_sym" if tuned["tea_chunk"] == 1 else "k_tea_pairs"
+    kname = tea_kernel(tuned["tea_chunk"])
     out = {"value": b.nb * steps / elapsed, "unit": "monomer-steps/s", "steps": steps, "warmup": warmup,
            "ms_per_step": elapsed * 1e3 / steps,
            "config": {"workload": "TEA-HI: 64 trajectories x 1040-bead microtubule, a 2 nm, 0.89 cP, dt 1 ps, "
@@ -331,7 +338,7 @@ def c4_bench(steps, cpu=True):
                       "tuning": tuned},
            "roofline": {"bound": "mfma", "achieved": flops / avg / 1e12, "peak": BF16_DENSE_PEAK_TFS,
                         "unit": "TFLOP/s", "frac": flops / avg / 1e12 / BF16_DENSE_PEAK_TFS, "traffic": None,
-                        "kernel": "k_tea_pairs_sym" if tuned["tea_chunk"] == 1 else "k_tea_pairs",
+                        "kernel": tea_kernel(tuned["tea_chunk"]),
                         "avg_launch_ms": avg * 1e3, "alg_flops_per_launch": flops}}
     if cpu:
         from oracle.oracle import Oracle, generate_seeds, max_threads

@@ -195,6 +195,10 @@ int mte_get_mode(const mte_engine *e, int32_t *mode);
 /* tea_chunk value: 64-bead chunks with each (i, j) RPY block evaluated once
  * for both beads (the other chunk values evaluate it for i and for j). */
 #define MTE_TEA_SYMMETRIC 1
+/* tea_chunk value: the pair sums on the matrix cores (open boundaries only):
+ * scalar RPY kernels times per-j moment panels in split bf16 products with
+ * FP32 accumulation, pairs closer than 16 nm summed exactly on the VALU. */
+#define MTE_TEA_MFMA 2
 
 /* tea_eps values: each pair's epsilon terms evaluated once for both beads,
  * through a partial-sum buffer of 40 x ceil(n_max / 1024) bytes per bead
@@ -207,7 +211,7 @@ typedef struct {
     int32_t resident_per_thread;  /* beads per thread of the resident kernel: 1, 2 or 3 */
     int32_t stream_block;         /* workgroup size of the streamed bead kernels: 64, 128 or 256 */
     int32_t tea_chunk;            /* j-chunk of the TEA pair kernel: 64, 128 or 256 beads, or
-                                     MTE_TEA_SYMMETRIC (0 without TEA) */
+                                     MTE_TEA_SYMMETRIC or MTE_TEA_MFMA (0 without TEA) */
     int32_t cell_stencil;         /* cell-list rebuilds: 1 = 27 cells of side r, 2 = 125 cells of side r/2 */
     int32_t mode;                 /* MTE_MODE_RESIDENT or MTE_MODE_STREAMED (plain dynamics) */
     int32_t tea_eps;              /* TEA epsilon: MTE_TEA_EPS_ONEPASS or MTE_TEA_EPS_TWOSIDED (0 without TEA) */

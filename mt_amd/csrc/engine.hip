@@ -64,6 +64,10 @@ hipError_t mte::tea_build_units(mte_engine *e, int chunk) {
         if (chunk == MTE_TEA_SYMMETRIC) {
             for (int i0 = 0; i0 < n; i0 += 64, c++)
                 for (int j0 = i0; j0 < n; j0 += 64) units.push_back(make_int4(t, i0, j0, c));
+        } else if (chunk == MTE_TEA_MFMA) {
+            // every (64-bead I tile, 64-bead J chunk) pair; partial slot = J
+            for (int j0 = 0; j0 < n; j0 += 64, c++)
+                for (int i0 = 0; i0 < n; i0 += 64) units.push_back(make_int4(t, i0, j0, c));
         } else {
             for (int j0 = 0; j0 < n; j0 += chunk, c++)
                 for (int i0 = 0; i0 < n; i0 += 256) units.push_back(make_int4(t, i0, j0, c));
@@ -448,9 +452,9 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
         return bail(MTE_ERR_HIP);
     }
     // Cell lists: selection and tables.
+    for (int t = 0; t < nt; t++) e->nmax = std::max(e->nmax, tsize[t]);
     {
-        int nmax = 0;
-        for (int t = 0; t < nt; t++) nmax = std::max(nmax, tsize[t]);
+        const int nmax = e->nmax;
         const float radius = candidate_radius(e->P);
         bool cells_ok = radius > 0.f;
         for (int t = 0; t < nt; t++)
@@ -1285,9 +1289,15 @@ int mte_set_tuning(mte_engine *e, const mte_tuning *t) {
         e->tune.stream_block = t->stream_block;
     }
     if (t->tea_chunk && t->tea_chunk != e->tune.tea_chunk) {
-        if (t->tea_chunk != 64 && t->tea_chunk != 128 && t->tea_chunk != 256 && t->tea_chunk != MTE_TEA_SYMMETRIC)
-            return fail(e, MTE_ERR_ARG, "mte_set_tuning: tea_chunk must be 64, 128, 256 or MTE_TEA_SYMMETRIC");
+        if (t->tea_chunk != 64 && t->tea_chunk != 128 && t->tea_chunk != 256 && t->tea_chunk != MTE_TEA_SYMMETRIC &&
+            t->tea_chunk != MTE_TEA_MFMA)
+            return fail(e, MTE_ERR_ARG,
+                        "mte_set_tuning: tea_chunk must be 64, 128, 256, MTE_TEA_SYMMETRIC or MTE_TEA_MFMA");
         if (!e->P.tea_on) return fail(e, MTE_ERR_ARG, "mte_set_tuning: tea_chunk needs TEA");
+        if (t->tea_chunk == MTE_TEA_MFMA && e->periodic)
+            return fail(e, MTE_ERR_ARG, "mte_set_tuning: MTE_TEA_MFMA needs open boundaries");
+        if (t->tea_chunk == MTE_TEA_MFMA && e->nmax > REBUILD_ORDER_MAX)
+            return fail(e, MTE_ERR_ARG, "mte_set_tuning: MTE_TEA_MFMA needs trajectories of at most 2048 beads");
         if (t->tea_chunk != e->tea.unit_chunk) HIPCHK(tea_build_units(e, t->tea_chunk));
     }
     if (t->tea_eps) {
@@ -1407,7 +1417,11 @@ int mte_autotune(mte_engine *e, int32_t trial_steps, mte_tuning *chosen, char *r
         if ((rc = restore()) || (rc = mte_set_tuning(e, &best))) return rc;
     } else {
         if ((rc = family("stream_block", {64, 128, 256}, &e->tune.stream_block))) return rc;
-        if ((rc = family("tea_chunk", {64, 128, 256, MTE_TEA_SYMMETRIC}, &e->tune.tea_chunk))) return rc;
+        if (e->periodic || e->nmax > REBUILD_ORDER_MAX) {
+            if ((rc = family("tea_chunk", {64, 128, 256, MTE_TEA_SYMMETRIC}, &e->tune.tea_chunk))) return rc;
+        } else if ((rc = family("tea_chunk", {64, 128, 256, MTE_TEA_SYMMETRIC, MTE_TEA_MFMA}, &e->tune.tea_chunk))) {
+            return rc;
+        }
     }
     if (e->mode == MTE_MODE_STREAMED && e->use_cells && need_lists &&
         (rc = family("cell_stencil", {1, 2}, &e->tune.cell_stencil)))

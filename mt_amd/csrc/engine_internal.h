@@ -48,6 +48,10 @@ constexpr size_t EPS_PART_MAX_BYTES = size_t(6) << 30;
 // j-chunk length of the TEA pair kernel.
 constexpr int TEA_CHUNK_BEADS = 128;
 
+// Largest trajectory that k_rebuild_order sorts spatially (larger ones keep
+// the index order); k_tea_pairs_mfma needs the spatial order.
+constexpr int REBUILD_ORDER_MAX = 2048;
+
 // Hashed cell lists (cells.hip): per trajectory a power-of-two bucket table.
 struct CellTables {
     const int *toff = nullptr;   // trajectory -> first bucket
@@ -101,6 +105,7 @@ struct mte_engine {
     mte_params P{};
     mte_options O{};
     int nb = 0, ntraj = 0;
+    int nmax = 0;            // beads in the largest trajectory
     bool periodic = false;   // some trajectory has a periodic box
     std::vector<int64_t> off;
     int device = 0;

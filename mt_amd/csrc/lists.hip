@@ -37,7 +37,7 @@ constexpr int LIST_SUPER = 64;                       // beads per super-group bo
 constexpr int LIST_CAND = 32;                        // candidate slots per lane
 constexpr int TRAJ_THREADS = 1024;                   // k_rebuild_traj workgroup (at most)
 constexpr int TRAJ_CAND = 24;                        // its candidate slots per lane
-constexpr int ORDER_MAX = 2048;                      // larger trajectories keep the index order
+constexpr int ORDER_MAX = REBUILD_ORDER_MAX;         // larger trajectories keep the index order
 constexpr int ORDER_THREADS = 1024;
 
 // 7-bit coordinate spread to every third bit (Morton interleave).
@@ -51,27 +51,59 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v) {
 }
 
 // Rebuild order of each trajectory: beads sorted by the Morton code of their
-// cell (side `side`, coordinates taken modulo 128 cells), ties by index.
-// One workgroup per trajectory; keys (code << 11 | bead) bitonic-sorted in
-// LDS.  Any order gives the same lists; this one only makes the bounding-box
-// skips of k_rebuild coherent across a wave.
+// cell, ties by index.  Cells are counted from the trajectory's lower
+// bounding-box corner, with side `side` or, for a trajectory wider than 127
+// such cells, extent / 127, so no coordinate wraps: a wrap would interleave
+// beads from opposite ends of the trajectory (on a tube centred on the axis,
+// x = -0.1 would fall in cell 127), and 64 consecutive beads would no
+// longer be a compact patch.  One workgroup per trajectory; keys
+// (code << 11 | bead) bitonic-sorted in LDS.  Any order gives the same
+// lists; this one makes the bounding-box skips of k_rebuild coherent across
+// a wave and the I tiles of k_tea_pairs_mfma compact.
 __global__ void __launch_bounds__(ORDER_THREADS) k_rebuild_order(DevTopo T, DevState S, float side, int *order) {
     __shared__ uint32_t key[ORDER_MAX];
+    __shared__ float red[6][ORDER_THREADS / 64];
     const int t = blockIdx.x;
     const int base = T.tbase[t], n = T.tsize[t];
     if (n > ORDER_MAX) {
         for (int p = threadIdx.x; p < n; p += blockDim.x) order[base + p] = p;
         return;
     }
+    // bounding box: per-lane, then across the wave (xor butterfly), then the waves
+    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const float4 x = S.xyz[base + i];
+        b[0] = fminf(b[0], x.x), b[1] = fminf(b[1], x.y), b[2] = fminf(b[2], x.z);
+        b[3] = fmaxf(b[3], x.x), b[4] = fmaxf(b[4], x.y), b[5] = fmaxf(b[5], x.z);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+            const float o = __shfl_xor(b[c], off);
+            b[c] = c < 3 ? fminf(b[c], o) : fmaxf(b[c], o);
+        }
+    const int nw = (int)(blockDim.x + 63) / 64;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int c = 0; c < 6; c++) red[c][threadIdx.x >> 6] = b[c];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        b[c] = red[c][0];
+        for (int w = 1; w < nw; w++) b[c] = c < 3 ? fminf(b[c], red[c][w]) : fmaxf(b[c], red[c][w]);
+    }
+    const float ext = fmaxf(fmaxf(b[3] - b[0], b[4] - b[1]), b[5] - b[2]);
+    const float inv = 1.f / fmaxf(side, ext * (1.f / 127.f));   // NaN extent: fmaxf keeps side
     int N = 1;
     while (N < n) N <<= 1;
-    const float inv = 1.f / side;
+    // cell of one coordinate in [0, 127]; a NaN coordinate goes to cell 0
+    auto cell = [&](float v, float lo) { return (uint32_t)fminf(fmaxf((v - lo) * inv, 0.f), 127.f); };
     for (int i = threadIdx.x; i < N; i += blockDim.x) {
         uint32_t k = 0xffffffffu;   // padding sorts last
         if (i < n) {
             const float4 x = S.xyz[base + i];
-            const uint32_t cx = (uint32_t)(int)floorf(x.x * inv), cy = (uint32_t)(int)floorf(x.y * inv),
-                           cz = (uint32_t)(int)floorf(x.z * inv);
+            const uint32_t cx = cell(x.x, b[0]), cy = cell(x.y, b[1]), cz = cell(x.z, b[2]);
             k = ((spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2)) << 11) | (uint32_t)i;
         }
         key[i] = k;

@@ -608,6 +608,250 @@ __global__ void __launch_bounds__(TEA_TILE)
     Sout.fp2[b] = p2;
 }
 
+// ------------------------------------------------------------------ MFMA
+// Pair sums on the matrix cores (open boundaries).  With d = r_j - r_i,
+// c(r) = crr / r^2 and the far-branch coefficients (r > 2a)
+//   cii = p1 w + p3 w^3,  c = p1 w^3 - 3 p3 w^5,  w = 1/r, p1 = 3a/4, p3 = a^3/2,
+// the sum over j of D_hat_ij (F_j + C'_i . R_j), C' = beta C, splits into
+// scalar kernels times per-j moments taken about a centre X (r = r_j - X,
+// a = r_i - X):
+//   sum_j cii (F_j + C'_i . R_j)                        -> moments F, R
+//   sum_j c d (d . F_j)  = S[r (r.F)] - S[r F^T] a - a S[r.F] + a (a . S[F])
+//   sum_j c d_al d_ga R_ga = S[r_al r_ga R_ga] - a_ga S[r_al R_ga]
+//                            - a_al S[r_ga R_ga] + a_al a_ga S[R_ga]
+// with S[m] = sum_j kernel_ij m_j: 6 moment columns on cii and 37 on c, i.e.
+// four 16-column products per 16 x 32 block of pairs, each as three
+// v_mfma_f32_16x16x32_bf16 on split operands (hi hi, hi lo, lo hi; a value
+// is hi + lo to 2^-18 of itself), accumulated in FP32.
+// Work unit: (trajectory, 64-bead I tile, 64-bead J chunk), four waves of
+// 16 i each; X = the mean position of the I tile.  The I tiles take the
+// beads in the spatial (Morton cell) order k_rebuild_order computes when a
+// state is set, so a tile is a compact patch (at most ~18 nm from its centre
+// on a microtubule) rather than 64 beads along a protofilament; any order
+// gives the same sums in exact arithmetic, but the rounding of the moments
+// grows with the tile's radius.  Pairs closer than MF_RNEAR (which is > 2a,
+// so every product pair takes the far branch) are summed exactly on the VALU
+// instead: they carry the largest kernel values, and beyond MF_RNEAR the
+// terms of the expansion exceed the result by at most (1 + |a| / r)^2 for a
+// bead at a from the centre.  Trajectories above REBUILD_ORDER_MAX beads keep
+// the index order, so the engine does not offer this kernel for them.
+// j == i, extra beads and slots past the trajectory end enter as zero.
+// Partial sums of chunk J go to slot J of bead i, as k_tea_pairs<64> writes
+// them.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+constexpr int MF_TILE = 64;
+constexpr int MF_COLS = 64;          // moment columns: 0-47 on c (37 used), 48-63 on cii (6 used)
+constexpr float MF_RNEAR = 16.f;
+
+__device__ __forceinline__ void mf_split8(const float v[8], bf16x8_t &hi, bf16x8_t &lo) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const __bf16 h = (__bf16)v[q];
+        hi[q] = h;
+        lo[q] = (__bf16)(v[q] - (float)h);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+    k_tea_pairs_mfma(mte_params P, DevTopo T, DevState S, const float4 *fxyz, const float4 *R, const float4 *ci,
+                     const float *beta, const int4 *units, const int *order, int chunk_stride, float4 *partial) {
+    __shared__ float4 jx[MF_TILE], jf[MF_TILE], jr[MF_TILE];   // w of jx: 1 for a bead that takes part
+    __shared__ __attribute__((aligned(16))) __bf16 mhi[MF_COLS][MF_TILE];
+    __shared__ __attribute__((aligned(16))) __bf16 mlo[MF_COLS][MF_TILE];
+    __shared__ float out[4][16][MF_COLS + 1];
+    __shared__ float nr[4][4][16][3];
+    __shared__ float4 xsum[4];
+    const int4 u = units[blockIdx.x];   // traj, i0, j0, chunk index
+    const int base = T.tbase[u.x], n = T.tsize[u.x];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, il = lane & 15;
+    const float bt = beta[u.x], w2b = bt * bt;
+    // the I tile's centre
+    {
+        float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (tid < MF_TILE && u.y + tid < n) {
+            x = S.xyz[base + order[base + u.y + tid]];
+            x.w = 1.f;
+        }
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1) {
+            x.x += __shfl_xor(x.x, m);
+            x.y += __shfl_xor(x.y, m);
+            x.z += __shfl_xor(x.z, m);
+            x.w += __shfl_xor(x.w, m);
+        }
+        if (tid == 0) xsum[0] = x;
+    }
+    // the J chunk
+    if (tid < MF_TILE) {
+        const int lj = u.z + tid, bj = base + lj;
+        float4 x = make_float4(0.f, 0.f, 0.f, 0.f), f = x, r = x;
+        if (lj < n) {
+            x = S.xyz[bj];
+            x.w = T.extra[bj] ? 0.f : 1.f;
+            if (x.w != 0.f) {
+                f = fxyz[bj];
+                r = R[bj];
+            }
+        }
+        jx[tid] = x;
+        jf[tid] = f;
+        jr[tid] = r;
+    }
+    __syncthreads();
+    const float4 xs = xsum[0];
+    const float cx0 = xs.x / xs.w, cy0 = xs.y / xs.w, cz0 = xs.z / xs.w;
+    // moments about the centre: thread (column group tid / 64, j = tid % 64)
+    {
+        const int j = lane, grp = w;
+        const float4 xj = jx[j], F = jf[j], Rj = jr[j];
+        const float r0 = xj.x - cx0, r1 = xj.y - cy0, r2 = xj.z - cz0;
+        const float rr[3] = {r0, r1, r2}, Fv[3] = {F.x, F.y, F.z}, Rv[3] = {Rj.x, Rj.y, Rj.z};
+        float m[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) m[q] = 0.f;
+        if (xj.w != 0.f) {
+            const float rF = r0 * Fv[0] + r1 * Fv[1] + r2 * Fv[2];
+            if (grp == 0) {
+#pragma unroll
+                for (int al = 0; al < 3; al++) m[al] = rr[al] * rF;
+#pragma unroll
+                for (int al = 0; al < 3; al++)
+#pragma unroll
+                    for (int be = 0; be < 3; be++) m[3 + 3 * al + be] = rr[al] * Fv[be];
+                m[12] = rF;
+#pragma unroll
+                for (int be = 0; be < 3; be++) m[13 + be] = Fv[be];
+            } else if (grp == 1) {
+#pragma unroll
+                for (int al = 0; al < 3; al++)
+#pragma unroll
+                    for (int ga = 0; ga < 3; ga++) m[3 * al + ga] = rr[al] * rr[ga] * Rv[ga];
+#pragma unroll
+                for (int k = 0; k < 7; k++) m[9 + k] = rr[k / 3] * Rv[k % 3];
+            } else if (grp == 2) {
+                m[0] = rr[2] * Rv[1];
+                m[1] = rr[2] * Rv[2];
+#pragma unroll
+                for (int ga = 0; ga < 3; ga++) m[2 + ga] = Rv[ga];
+            } else {
+#pragma unroll
+                for (int be = 0; be < 3; be++) {
+                    m[be] = Fv[be];
+                    m[3 + be] = Rv[be];
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const __bf16 h = (__bf16)m[q];
+            mhi[16 * grp + q][j] = h;
+            mlo[16 * grp + q][j] = (__bf16)(m[q] - (float)h);
+        }
+    }
+    __syncthreads();
+    // this lane's bead: row il of the wave's 16
+    const int pi = u.y + 16 * w + il;
+    const int li = pi < n ? order[base + pi] : n, bi = base + li;
+    const bool vi = li < n && !T.extra[bi];
+    float3 xi = make_float3(cx0, cy0, cz0), Cp = make_float3(0.f, 0.f, 0.f);
+    if (vi) {
+        xi = f3(S.xyz[bi]);
+        const float4 cc = ci[bi];
+        Cp = make_float3(bt * rsq_fast(1.f + w2b * cc.x), bt * rsq_fast(1.f + w2b * cc.y),
+                         bt * rsq_fast(1.f + w2b * cc.z));
+    }
+    const float a = P.tea_a, ia = rcp_fast(P.tea_a);
+    const float p1 = 0.75f * a, p3 = 0.5f * a * a * a;
+    f32x4_t accA = {0.f, 0.f, 0.f, 0.f}, accC[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) accC[t] = accA;
+    float3 nacc = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int kb = 0; kb < 2; kb++) {
+        float kc[8], kd[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int jj = kb * 32 + 8 * g + q;
+            const float4 xj = jx[jj];
+            const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const bool nearp = r2 < MF_RNEAR * MF_RNEAR;
+            const float wv = rsq_fast(fmaxf(r2, 1e-12f)), wv2 = wv * wv, wv3 = wv2 * wv;
+            kd[q] = nearp ? 0.f : fmaf(p3, wv3, p1 * wv);
+            kc[q] = nearp ? 0.f : fmaf(-3.f * p3, wv3 * wv2, p1 * wv3);
+            if (nearp && vi && xj.w != 0.f && u.z + jj != li) {
+                // the exact RPY block (both branches), src/bdhitea_kernel.cu:38-58, 104-146
+                const float r2c = fmaxf(r2, 1e-12f), rinv = rsq_fast(r2c);
+                const RpyCoef c = rpy_select(r2c, rinv, a, ia);
+                const float4 F = jf[jj], Rj = jr[jj];
+                const float vx = fmaf(Cp.x, Rj.x, F.x), vy = fmaf(Cp.y, Rj.y, F.y), vz = fmaf(Cp.z, Rj.z, F.z);
+                const float sv = (c.crr * rinv * rinv) * (dx * vx + dy * vy + dz * vz);
+                nacc.x = fmaf(sv, dx, fmaf(c.cii, vx, nacc.x));
+                nacc.y = fmaf(sv, dy, fmaf(c.cii, vy, nacc.y));
+                nacc.z = fmaf(sv, dz, fmaf(c.cii, vz, nacc.z));
+            }
+        }
+        bf16x8_t dhi, dlo, chi, clo;
+        mf_split8(kd, dhi, dlo);
+        mf_split8(kc, chi, clo);
+        const int k0 = kb * 32 + 8 * g;
+        {
+            const bf16x8_t bh = *reinterpret_cast<const bf16x8_t *>(&mhi[48 + il][k0]);
+            const bf16x8_t bl = *reinterpret_cast<const bf16x8_t *>(&mlo[48 + il][k0]);
+            accA = __builtin_amdgcn_mfma_f32_16x16x32_bf16(dhi, bh, accA, 0, 0, 0);
+            accA = __builtin_amdgcn_mfma_f32_16x16x32_bf16(dhi, bl, accA, 0, 0, 0);
+            accA = __builtin_amdgcn_mfma_f32_16x16x32_bf16(dlo, bh, accA, 0, 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            const bf16x8_t bh = *reinterpret_cast<const bf16x8_t *>(&mhi[16 * t + il][k0]);
+            const bf16x8_t bl = *reinterpret_cast<const bf16x8_t *>(&mlo[16 * t + il][k0]);
+            accC[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(chi, bh, accC[t], 0, 0, 0);
+            accC[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(chi, bl, accC[t], 0, 0, 0);
+            accC[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(clo, bh, accC[t], 0, 0, 0);
+        }
+    }
+    // lane (g, il) holds rows i = 4g + r of column il of each 16-column tile
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+#pragma unroll
+        for (int t = 0; t < 3; t++) out[w][4 * g + r][16 * t + il] = accC[t][r];
+        out[w][4 * g + r][48 + il] = accA[r];
+    }
+    nr[w][g][il][0] = nacc.x;
+    nr[w][g][il][1] = nacc.y;
+    nr[w][g][il][2] = nacc.z;
+    __syncthreads();
+    if (lane >= 16 || li >= n) return;
+    float3 f = make_float3(0.f, 0.f, 0.f);
+    if (vi) {
+        const float *o = out[w][il];
+        const float av[3] = {xi.x - cx0, xi.y - cy0, xi.z - cz0}, cp[3] = {Cp.x, Cp.y, Cp.z};
+        const float aF = av[0] * o[13] + av[1] * o[14] + av[2] * o[15];
+        float fv[3];
+#pragma unroll
+        for (int al = 0; al < 3; al++) {
+            // cii part
+            float v = fmaf(cp[al], o[51 + al], o[48 + al]);
+            // c d (d . F)
+            v += o[al] - (av[0] * o[3 + 3 * al] + av[1] * o[4 + 3 * al] + av[2] * o[5 + 3 * al]) - av[al] * o[12] +
+                 av[al] * aF;
+            // sum_ga C'_ga c d_al d_ga R_ga
+#pragma unroll
+            for (int ga = 0; ga < 3; ga++) {
+                const int k = 3 * al + ga, kd = 4 * ga;   // r_al R_ga and r_ga R_ga
+                const float rR = k < 7 ? o[25 + k] : o[32 + k - 7];
+                const float rRd = kd < 7 ? o[25 + kd] : o[32 + kd - 7];
+                v = fmaf(cp[ga], o[16 + k] - av[ga] * rR - av[al] * rRd + av[al] * av[ga] * o[34 + ga], v);
+            }
+            fv[al] = v + nr[w][0][il][al] + nr[w][1][il][al] + nr[w][2][il][al] + nr[w][3][il][al];
+        }
+        f = make_float3(fv[0], fv[1], fv[2]);
+    }
+    partial[(size_t)u.w * chunk_stride + bi] = make_float4(f.x, f.y, f.z, 0.f);
+}
+
 hipError_t launch_tea_eps(mte_engine *e) {
     if (e->tune.tea_eps == MTE_TEA_EPS_ONEPASS) {
         if (e->periodic)
@@ -653,6 +897,13 @@ hipError_t launch_tea_pairs(mte_engine *e) {
             hipLaunchKernelGGL(k_tea_pairs_sym<false>, dim3(nwg), dim3(SYM_CHUNK), 0, e->stream, e->P, e->T,
                                e->S[e->cur], e->fxyz, e->tea.R, e->tea.ci, e->tea.beta, e->tea.units,
                                e->tea.nunits, e->nb, e->tea.partial);
+        return hipGetLastError();
+    }
+    if (e->tea.unit_chunk == MTE_TEA_MFMA) {
+        if (e->periodic) return hipErrorInvalidValue;   // minimum images are not in the moment form
+        hipLaunchKernelGGL(k_tea_pairs_mfma, dim3(e->tea.nunits), dim3(256), 0, e->stream, e->P, e->T, e->S[e->cur],
+                           e->fxyz, e->tea.R, e->tea.ci, e->tea.beta, e->tea.units, e->rb_order, e->nb,
+                           e->tea.partial);
         return hipGetLastError();
     }
     switch (e->tea.unit_chunk) {

@@ -20,6 +20,7 @@ MTE_OK = 0
 MODE_AUTO, MODE_STREAMED, MODE_RESIDENT = 0, 1, 2
 LISTS_AUTO, LISTS_ALLPAIRS, LISTS_CELLS = 0, 1, 2
 TEA_SYMMETRIC = 1   # tea_chunk: 64-bead chunks, each pair block evaluated once for both beads
+TEA_MFMA = 2        # tea_chunk: pair sums on the matrix cores (open boundaries)
 TEA_EPS_ONEPASS, TEA_EPS_TWOSIDED = 1, 2   # tea_eps: epsilon terms once per pair, or once per bead of it
 ERRORS = {1: "MTE_ERR_ARG", 2: "MTE_ERR_HIP", 3: "MTE_ERR_OVERFLOW", 4: "MTE_ERR_TEA", 5: "MTE_ERR_STATE",
           6: "MTE_ERR_NOISE"}
