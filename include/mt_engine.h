@@ -74,6 +74,11 @@ typedef struct {
     float tea_a;
     int32_t tea_epsilon_freq, tea_capricious;
     float tea_epsmax;
+    /* The reference's compile-time REGULARIZATION (src/compute_cuda.cu:482-486)
+     * as a run-time switch: each LJ pair force is capped at gammaR r_mon / dt
+     * (r_mon = 2 nm, src/mt.h:24), the force that moves a bead by r_mon in one
+     * step.  0 = off, as the reference builds by default. */
+    int32_t lj_regularize;
 } mte_params;
 
 /* Topology of a CSR batch.  Trajectory t owns global beads

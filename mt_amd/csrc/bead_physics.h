@@ -134,13 +134,18 @@ __device__ __forceinline__ void lj_forces(const mte_params &P, const Acc &A, con
                                           float F[6]) {
     const int nlj = Ls.nlj();
     const float s0 = P.ljscale * P.ljsigma6;
+    // REGULARIZATION (src/compute_cuda.cu:482-486): the pair force s r is
+    // capped at gammaR r_mon / dt, i.e. s at that over r (r_mon = 2 nm)
+    const bool reg = P.lj_regularize != 0;
+    const float cap = P.gammaR * 2.f / P.dt;
     float fx = F[0], fy = F[1], fz = F[2];   // same accumulation order, scalar registers
     auto pair = [&](float4 xj) {
         float3 d = sub3(I.r, f3(xj));
         if (PERIODIC) d = min_image(d, I.box);
         const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
         const float inv = rcp_fast(r2);
-        const float s = r2 < LJ_CUT * LJ_CUT ? s0 * (6.f * (inv * inv) * (inv * inv)) : 0.f;
+        float s = r2 < LJ_CUT * LJ_CUT ? s0 * (6.f * (inv * inv) * (inv * inv)) : 0.f;
+        if (reg) s = fminf(s, cap * rsq_fast(r2));
         fx += s * d.x;
         fy += s * d.y;
         fz += s * d.z;

@@ -47,27 +47,35 @@ static void release(mte_engine *e, void *p) {
 
 // Work units (trajectory, 256-bead i group, j chunk) of the TEA pair kernel
 // and its per-chunk partial buffer, for j-chunks of `chunk` beads; for
-// MTE_TEA_SYMMETRIC, units (trajectory, 64-bead chunk I, chunk J >= I).
+// MTE_TEA_SYMMETRIC and MTE_TEA_MFMA, units (trajectory, 64-bead chunk I,
+// chunk J >= I), and for MTE_TEA_MFMA the tile tables as well.
 hipError_t mte::tea_build_units(mte_engine *e, int chunk) {
     release(e, e->tea.units);
     release(e, e->tea.nchunks);
     release(e, e->tea.partial);
+    release(e, e->tea.tiles);
+    release(e, e->tea.mom);
+    release(e, e->tea.ctr);
+    release(e, e->tea.rec);
+    release(e, e->tea.rbead);
     e->tea.units = nullptr;
     e->tea.nchunks = nullptr;
     e->tea.partial = nullptr;
+    e->tea.tiles = nullptr;
+    e->tea.mom = nullptr;
+    e->tea.ctr = nullptr;
+    e->tea.rec = nullptr;
+    e->tea.rbead = nullptr;
+    e->tea.ntiles = 0;
     std::vector<int4> units;
     std::vector<int> nchunks(e->ntraj, 0);
     int maxc = 0;
     for (int t = 0; t < e->ntraj; t++) {
         const int n = (int)(e->off[t + 1] - e->off[t]);
         int c = 0;
-        if (chunk == MTE_TEA_SYMMETRIC) {
+        if (chunk == MTE_TEA_SYMMETRIC || chunk == MTE_TEA_MFMA) {
             for (int i0 = 0; i0 < n; i0 += 64, c++)
                 for (int j0 = i0; j0 < n; j0 += 64) units.push_back(make_int4(t, i0, j0, c));
-        } else if (chunk == MTE_TEA_MFMA) {
-            // every (64-bead I tile, 64-bead J chunk) pair; partial slot = J
-            for (int j0 = 0; j0 < n; j0 += 64, c++)
-                for (int i0 = 0; i0 < n; i0 += 64) units.push_back(make_int4(t, i0, j0, c));
         } else {
             for (int j0 = 0; j0 < n; j0 += chunk, c++)
                 for (int i0 = 0; i0 < n; i0 += 256) units.push_back(make_int4(t, i0, j0, c));
@@ -75,7 +83,7 @@ hipError_t mte::tea_build_units(mte_engine *e, int chunk) {
         nchunks[t] = c;
         maxc = std::max(maxc, c);
     }
-    if (chunk == MTE_TEA_SYMMETRIC) {
+    if (chunk == MTE_TEA_SYMMETRIC || chunk == MTE_TEA_MFMA) {
         // longest first: off-diagonal blocks (64 steps of both sides), then
         // diagonal blocks (one-sided, one step per bead of the chunk)
         auto cost = [&](const int4 &u) {
@@ -84,6 +92,46 @@ hipError_t mte::tea_build_units(mte_engine *e, int chunk) {
         };
         std::stable_sort(units.begin(), units.end(),
                          [&](const int4 &a, const int4 &b) { return cost(a) > cost(b); });
+    }
+    if (chunk == MTE_TEA_MFMA) {
+        // Workgroups go to the 8 XCDs in turn (workgroup k to XCD k mod 8),
+        // each with its own L2: unit k is taken from trajectory t = k mod 8
+        // (mod 8), so that every trajectory's moment tables are read through
+        // one L2.  The units of each class keep the longest-first order.
+        constexpr int NXCD = 8;
+        std::vector<std::vector<int4>> cls(NXCD);
+        for (const int4 &u : units) cls[u.x % NXCD].push_back(u);
+        std::vector<size_t> next(NXCD, 0);
+        units.clear();
+        for (bool left = true; left;) {
+            left = false;
+            for (int x = 0; x < NXCD; x++)
+                if (next[x] < cls[x].size()) {
+                    units.push_back(cls[x][next[x]++]);
+                    left = true;
+                }
+        }
+        std::vector<int2> tiles;
+        std::vector<int> tbase(e->ntraj);
+        for (int t = 0; t < e->ntraj; t++) {
+            tbase[t] = (int)tiles.size();
+            const int n = (int)(e->off[t + 1] - e->off[t]);
+            for (int p0 = 0; p0 < n; p0 += 64) tiles.push_back(make_int2(t, p0));
+        }
+        for (int4 &u : units)
+            u = make_int4(tbase[u.x] + u.y / 64, tbase[u.x] + u.z / 64, u.y / 64, u.z / 64);
+        hipError_t s;
+        e->tea.ntiles = (int)tiles.size();
+        if ((s = dalloc(e, &e->tea.tiles, tiles.size())) != hipSuccess) return s;
+        if ((s = dalloc(e, &e->tea.ctr, tiles.size())) != hipSuccess) return s;
+        if ((s = dalloc(e, &e->tea.rec, tiles.size() * 4 * 64)) != hipSuccess) return s;
+        if ((s = dalloc(e, &e->tea.rbead, tiles.size() * 64)) != hipSuccess) return s;
+        uint16_t *mom = nullptr;
+        if ((s = dalloc(e, &mom, tiles.size() * 2 * 64 * 64)) != hipSuccess) return s;
+        e->tea.mom = mom;
+        if ((s = hipMemcpyAsync(e->tea.tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice,
+                                e->stream)) != hipSuccess)
+            return s;
     }
     hipError_t s;
     if ((s = dalloc(e, &e->tea.units, units.size())) != hipSuccess) return s;

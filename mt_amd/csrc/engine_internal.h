@@ -31,6 +31,17 @@ struct TeaBuffers {
     int *nchunks = nullptr;    // per trajectory
     int max_chunks = 0;
     float4 *partial = nullptr; // max_chunks x nb pair partial sums
+    // MTE_TEA_MFMA: 64-bead tiles of the spatial order (trajectory, first
+    // position), and per tile the bf16 hi/lo moment tables [2][64 columns]
+    // [64 beads], the centre and radius, and the bead records [4][64] and
+    // indices [64] (k_tea_moments); the units are then (tile I, tile J,
+    // slot I, slot J)
+    int2 *tiles = nullptr;
+    int ntiles = 0;
+    void *mom = nullptr;
+    float4 *ctr = nullptr;
+    float4 *rec = nullptr;
+    int *rbead = nullptr;
     // epsilon with each pair evaluated once (k_tea_eps_sym): work units
     // (traj, I group, J group >= I group) of 1024-bead groups, and the
     // partial sums [side][group][5][nb]; null when they would not fit

@@ -609,40 +609,119 @@ __global__ void __launch_bounds__(TEA_TILE)
 }
 
 // ------------------------------------------------------------------ MFMA
-// Pair sums on the matrix cores (open boundaries).  With d = r_j - r_i,
-// c(r) = crr / r^2 and the far-branch coefficients (r > 2a)
+// Pair sums on the matrix cores (open boundaries).  With d = r_i - r_j and
+// the far-branch coefficients (r > 2a) of D_hat = cii I + c d d^T,
 //   cii = p1 w + p3 w^3,  c = p1 w^3 - 3 p3 w^5,  w = 1/r, p1 = 3a/4, p3 = a^3/2,
-// the sum over j of D_hat_ij (F_j + C'_i . R_j), C' = beta C, splits into
-// scalar kernels times per-j moments taken about a centre X (r = r_j - X,
-// a = r_i - X):
-//   sum_j cii (F_j + C'_i . R_j)                        -> moments F, R
-//   sum_j c d (d . F_j)  = S[r (r.F)] - S[r F^T] a - a S[r.F] + a (a . S[F])
-//   sum_j c d_al d_ga R_ga = S[r_al r_ga R_ga] - a_ga S[r_al R_ga]
-//                            - a_al S[r_ga R_ga] + a_al a_ga S[R_ga]
-// with S[m] = sum_j kernel_ij m_j: 6 moment columns on cii and 37 on c, i.e.
-// four 16-column products per 16 x 32 block of pairs, each as three
-// v_mfma_f32_16x16x32_bf16 on split operands (hi hi, hi lo, lo hi; a value
-// is hi + lo to 2^-18 of itself), accumulated in FP32.
-// Work unit: (trajectory, 64-bead I tile, 64-bead J chunk), four waves of
-// 16 i each; X = the mean position of the I tile.  The I tiles take the
-// beads in the spatial (Morton cell) order k_rebuild_order computes when a
-// state is set, so a tile is a compact patch (at most ~18 nm from its centre
-// on a microtubule) rather than 64 beads along a protofilament; any order
-// gives the same sums in exact arithmetic, but the rounding of the moments
-// grows with the tile's radius.  Pairs closer than MF_RNEAR (which is > 2a,
-// so every product pair takes the far branch) are summed exactly on the VALU
-// instead: they carry the largest kernel values, and beyond MF_RNEAR the
-// terms of the expansion exceed the result by at most (1 + |a| / r)^2 for a
-// bead at a from the centre.  Trajectories above REBUILD_ORDER_MAX beads keep
-// the index order, so the engine does not offer this kernel for them.
-// j == i, extra beads and slots past the trajectory end enter as zero.
-// Partial sums of chunk J go to slot J of bead i, as k_tea_pairs<64> writes
-// them.
+// the sum over the j of a tile of D_hat_ij (F_j + C'_i . R_j), C' = beta C,
+// splits into two scalar kernels times per-j moments about the tile's centre
+// Y (s = r_j - Y, b = r_i - Y, d = b - s):
+//   sum_j cii (F_j + C'_i . R_j)                          -> moments F, R
+//   sum_j c d (d . F_j)  = b (b . S[F]) - b S[s.F] - S[s F^T] b + S[s (s.F)]
+//   sum_j c d_al d_ga R_ga = b_al b_ga S[R_ga] - b_al S[s_ga R_ga]
+//                            - b_ga S[s_al R_ga] + S[s_al s_ga R_ga]
+// with S[m] = sum_j kernel_ij m_j: 6 moment columns on cii and 37 on c.  The
+// moments depend on the source tile alone, so k_tea_moments tabulates them
+// once per step (split into bf16 hi + lo, a value being hi + lo to 2^-18 of
+// itself), and the pair kernel only evaluates the two kernels per pair and
+// contracts them with the tables on v_mfma_f32_16x16x32_bf16 (hi hi, hi lo,
+// lo hi, accumulated in FP32).  The kernels are symmetric in (i, j), so each
+// pair's values serve both beads: tile I's sums over tile J and tile J's
+// over tile I, as k_tea_pairs_sym does with its 3 x 3 blocks.
+// Tiles are 64 consecutive beads of the spatial (Morton cell) order that
+// k_rebuild_order computes when a state is set, a compact patch (on a
+// microtubule at most ~18 nm from its centre); any order gives the same
+// sums in exact arithmetic, but the rounding of the expansion grows with
+// the tile's radius.  Pairs closer than MF_RNEAR (> 2a, so every product
+// pair takes the far branch) are summed exactly on the VALU instead: they
+// carry the largest kernel values, and beyond MF_RNEAR the terms of the
+// expansion exceed the result by at most (1 + |s| / r)^2.  Trajectories
+// above REBUILD_ORDER_MAX beads keep the index order, so the engine does
+// not offer this form for them.
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 constexpr int MF_TILE = 64;
-constexpr int MF_COLS = 64;          // moment columns: 0-47 on c (37 used), 48-63 on cii (6 used)
-constexpr float MF_RNEAR = 16.f;
+constexpr int MF_COLS = 64;     // moment columns per bead in the tables: 0-36 on c, 48-53 on cii
+constexpr int MF_USED = 43;     // the columns in use, packed (c: 0-36, cii: 37-42)
+constexpr int MF_OUT_LD = 68;   // row stride of the LDS result rows (floats)
+constexpr float MF_RNEAR = 6.f;
+
+// packed index of moment column c, or -1 for an unused column
+__device__ __forceinline__ int mf_packed(int c) { return c < 37 ? c : (c >= 48 && c < 54 ? c - 11 : -1); }
+
+// Moment tables and bead records of one 64-bead tile (one wave; lane = bead
+// of the tile): centre Y = mean position of the tile's beads, radius = the
+// largest distance of one of them from Y; per bead, with s = r - Y,
+//   0-2  s_al (s.F)     3-11 s_al F_be    12 s.F     13-15 F_be
+//   16-24 s_al s_ga R_ga  25-33 s_al R_ga  34-36 R_ga             (kernel c)
+//   48-50 F   51-53 R                                              (kernel cii)
+// Extra beads and slots past the trajectory's end have zero F and R, so all
+// their moments vanish.  Tables: mom[tile][hi, lo][column][bead] bf16;
+// records rec[tile][position (w = 1 if active), F, R, C'][bead] and
+// rbead[tile][bead] (global index, -1 past the end), so that the pair kernel
+// reads each tile with coalesced loads that depend on nothing but the unit.
+__global__ void __launch_bounds__(MF_TILE)
+    k_tea_moments(DevTopo T, DevState S, const float4 *fxyz, const float4 *R, const float4 *ci, const float *beta,
+                  const int *order, const int2 *tiles, __bf16 *mom, float4 *ctr, float4 *rec, int *rbead) {
+    const int2 tl = tiles[blockIdx.x];   // trajectory, first position
+    const int base = T.tbase[tl.x], n = T.tsize[tl.x];
+    const int lane = threadIdx.x, p = tl.y + lane;
+    const bool in = p < n;
+    const int bi = in ? base + order[base + p] : base;
+    const float4 x = in ? S.xyz[bi] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float sum[4] = {x.x, x.y, x.z, in ? 1.f : 0.f};
+#pragma unroll
+    for (int m = 1; m < MF_TILE; m <<= 1)
+#pragma unroll
+        for (int k = 0; k < 4; k++) sum[k] += __shfl_xor(sum[k], m);
+    const float cx = sum[0] / sum[3], cy = sum[1] / sum[3], cz = sum[2] / sum[3];
+    const bool act = in && !T.extra[bi];
+    const float4 F = act ? fxyz[bi] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 Rb = act ? R[bi] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float s[3] = {x.x - cx, x.y - cy, x.z - cz}, Fv[3] = {F.x, F.y, F.z}, Rv[3] = {Rb.x, Rb.y, Rb.z};
+    float rad = in ? sqrtf(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) : 0.f;
+#pragma unroll
+    for (int m = 1; m < MF_TILE; m <<= 1) rad = fmaxf(rad, __shfl_xor(rad, m));
+    const float sF = s[0] * Fv[0] + s[1] * Fv[1] + s[2] * Fv[2];
+    float m[MF_COLS];
+#pragma unroll
+    for (int c = 0; c < MF_COLS; c++) m[c] = 0.f;
+#pragma unroll
+    for (int al = 0; al < 3; al++) {
+        m[al] = s[al] * sF;
+        m[13 + al] = Fv[al];
+        m[34 + al] = Rv[al];
+        m[48 + al] = Fv[al];
+        m[51 + al] = Rv[al];
+#pragma unroll
+        for (int be = 0; be < 3; be++) {
+            m[3 + 3 * al + be] = s[al] * Fv[be];
+            m[16 + 3 * al + be] = s[al] * s[be] * Rv[be];
+            m[25 + 3 * al + be] = s[al] * Rv[be];
+        }
+    }
+    m[12] = sF;
+    __bf16 *hi = mom + (size_t)blockIdx.x * 2 * MF_COLS * MF_TILE, *lo = hi + MF_COLS * MF_TILE;
+#pragma unroll
+    for (int c = 0; c < MF_COLS; c++) {
+        const __bf16 h = (__bf16)m[c];
+        hi[c * MF_TILE + lane] = h;
+        lo[c * MF_TILE + lane] = (__bf16)(m[c] - (float)h);
+    }
+    float4 C = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (act) {
+        const float bt = beta[tl.x], w2b = bt * bt;
+        const float4 cc = ci[bi];
+        C = make_float4(bt * rsq_fast(1.f + w2b * cc.x), bt * rsq_fast(1.f + w2b * cc.y),
+                        bt * rsq_fast(1.f + w2b * cc.z), 0.f);
+    }
+    float4 *rt = rec + (size_t)blockIdx.x * 4 * MF_TILE;
+    rt[lane] = in ? make_float4(x.x, x.y, x.z, act ? 1.f : 0.f) : make_float4(1e6f, 1e6f, 1e6f, 0.f);
+    rt[MF_TILE + lane] = F;
+    rt[2 * MF_TILE + lane] = Rb;
+    rt[3 * MF_TILE + lane] = C;
+    rbead[(size_t)blockIdx.x * MF_TILE + lane] = in ? bi : -1;
+    if (lane == 0) ctr[blockIdx.x] = make_float4(cx, cy, cz, rad);
+}
 
 __device__ __forceinline__ void mf_split8(const float v[8], bf16x8_t &hi, bf16x8_t &lo) {
 #pragma unroll
@@ -653,203 +732,288 @@ __device__ __forceinline__ void mf_split8(const float v[8], bf16x8_t &hi, bf16x8
     }
 }
 
-__global__ void __launch_bounds__(256)
-    k_tea_pairs_mfma(mte_params P, DevTopo T, DevState S, const float4 *fxyz, const float4 *R, const float4 *ci,
-                     const float *beta, const int4 *units, const int *order, int chunk_stride, float4 *partial) {
-    __shared__ float4 jx[MF_TILE], jf[MF_TILE], jr[MF_TILE];   // w of jx: 1 for a bead that takes part
-    __shared__ __attribute__((aligned(16))) __bf16 mhi[MF_COLS][MF_TILE];
-    __shared__ __attribute__((aligned(16))) __bf16 mlo[MF_COLS][MF_TILE];
-    __shared__ float out[4][16][MF_COLS + 1];
-    __shared__ float nr[4][4][16][3];
-    __shared__ float4 xsum[4];
-    const int4 u = units[blockIdx.x];   // traj, i0, j0, chunk index
-    const int base = T.tbase[u.x], n = T.tsize[u.x];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, il = lane & 15;
-    const float bt = beta[u.x], w2b = bt * bt;
-    // the I tile's centre
-    {
-        float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (tid < MF_TILE && u.y + tid < n) {
-            x = S.xyz[base + order[base + u.y + tid]];
-            x.w = 1.f;
-        }
+// Sum over one source tile for a target bead from the packed moment sums o
+// (MF_USED columns), b = r_target - Y_source and the target's C'.
+__device__ __forceinline__ float3 mf_reconstruct(const float *o, const float b[3], const float cp[3]) {
+    const float bF = b[0] * o[13] + b[1] * o[14] + b[2] * o[15];
+    float f[3];
 #pragma unroll
-        for (int m = 1; m < 64; m <<= 1) {
-            x.x += __shfl_xor(x.x, m);
-            x.y += __shfl_xor(x.y, m);
-            x.z += __shfl_xor(x.z, m);
-            x.w += __shfl_xor(x.w, m);
-        }
-        if (tid == 0) xsum[0] = x;
+    for (int al = 0; al < 3; al++) {
+        // cii (F + C' . R)
+        float v = fmaf(cp[al], o[40 + al], o[37 + al]);
+        // c d (d . F)
+        v += o[al] - (b[0] * o[3 + 3 * al] + b[1] * o[4 + 3 * al] + b[2] * o[5 + 3 * al]) - b[al] * o[12] +
+             b[al] * bF;
+        // sum_ga C'_ga c d_al d_ga R_ga
+#pragma unroll
+        for (int ga = 0; ga < 3; ga++)
+            v = fmaf(cp[ga],
+                     o[16 + 3 * al + ga] - b[ga] * o[25 + 3 * al + ga] - b[al] * o[25 + 4 * ga] +
+                         b[al] * b[ga] * o[34 + ga],
+                     v);
+        f[al] = v;
     }
-    // the J chunk
-    if (tid < MF_TILE) {
-        const int lj = u.z + tid, bj = base + lj;
-        float4 x = make_float4(0.f, 0.f, 0.f, 0.f), f = x, r = x;
-        if (lj < n) {
-            x = S.xyz[bj];
-            x.w = T.extra[bj] ? 0.f : 1.f;
-            if (x.w != 0.f) {
-                f = fxyz[bj];
-                r = R[bj];
-            }
-        }
-        jx[tid] = x;
-        jf[tid] = f;
-        jr[tid] = r;
-    }
-    __syncthreads();
-    const float4 xs = xsum[0];
-    const float cx0 = xs.x / xs.w, cy0 = xs.y / xs.w, cz0 = xs.z / xs.w;
-    // moments about the centre: thread (column group tid / 64, j = tid % 64)
-    {
-        const int j = lane, grp = w;
-        const float4 xj = jx[j], F = jf[j], Rj = jr[j];
-        const float r0 = xj.x - cx0, r1 = xj.y - cy0, r2 = xj.z - cz0;
-        const float rr[3] = {r0, r1, r2}, Fv[3] = {F.x, F.y, F.z}, Rv[3] = {Rj.x, Rj.y, Rj.z};
-        float m[16];
+    return make_float3(f[0], f[1], f[2]);
+}
+
+// Pair kernel.  Work unit: (tile I, tile J >= I of the same trajectory,
+// partial slots I and J), 256 threads.  Wave w takes the 32 x 32 block
+// (i half w / 2, j half w % 2) of the unit's 64 x 64 pairs; lane (g, il)
+// evaluates the pairs (i = 16 rb + il, j = 8 g + q) of its block, rb < 2,
+// q < 8, which is the A operand layout of two 16 x 32 MFMA tiles (row il,
+// k = 8 g + q).  I side: those tiles times tile J's moment tables (16 bytes
+// per lane from global memory, requested with the tiles' records before
+// anything waits) give the 64 moment sums of each i over the block's 32 j.
+// J side: the wave stores its kernel values transposed in LDS (rows j, 8
+// consecutive i per 16 bytes, the 16-byte groups XOR-swizzled by row so that
+// the four lane groups of a store hit different banks) and reads them back
+// as A operands against tile I's tables.  The two halves of each target's
+// sums meet in LDS; one thread per target adds them, reconstructs the sum
+// over the source tile and writes it to the target's partial slot (I side:
+// slot J, J side: slot I), as k_tea_pairs_sym does.  A diagonal unit
+// (I == J) sums every ordered pair on the I side alone.  Near pairs
+// (r < MF_RNEAR) enter the products as zero and are added exactly per lane,
+// on both sides, and reduced across the lanes in fixed order: the result
+// does not depend on scheduling.  A unit whose tiles' bounding spheres lie
+// more than MF_RNEAR apart holds no near pair and skips those tests.
+constexpr int MF_KT_WAVE = 4 * 32 * 32;   // bf16 per wave: kd hi, kd lo, kc hi, kc lo; 32 j x 32 i
+struct MfLds {
+    float4 rec[2][4][MF_TILE];   // [I, J][position (w: active), F, R, C'][bead]
+    int bead[2][MF_TILE];
+    float near_i[2][MF_TILE][3];   // exact near-pair sums, I side, per j half
+    float near_j[2][MF_TILE][3];   // J side, per i half
+    union {
+        __bf16 kt[4][MF_KT_WAVE];
+        float out[2][2][MF_USED][MF_OUT_LD];   // [side][half][packed column][target]
+    } u;
+};
+
+__device__ __forceinline__ int mf_kt_index(int type, int jj, int ii) {
+    // row jj of 32 i; 16-byte group (8 i) swizzled by (jj / 8) % 4
+    return (type * 32 + jj) * 32 + (ii ^ (8 * ((jj >> 3) & 3)));
+}
+
+// Kernels of one lane's 16 pairs as MFMA A operands; with NEAR, the pairs
+// closer than MF_RNEAR enter as zero and their exact blocks are summed into
+// ni (I side, per row block) and nj (J side, per q).
+template <bool NEAR>
+__device__ __forceinline__ void mf_pairs(const MfLds &L, int ha, int hb, int g, int il, bool diag, float a, float ia,
+                                         bf16x8_t (&kA)[2][4], float3 (&ni)[2], float3 (&nj)[8], bool &any_near) {
+    const float p1 = 0.75f * a, p3 = 0.5f * a * a * a;
 #pragma unroll
-        for (int q = 0; q < 16; q++) m[q] = 0.f;
-        if (xj.w != 0.f) {
-            const float rF = r0 * Fv[0] + r1 * Fv[1] + r2 * Fv[2];
-            if (grp == 0) {
+    for (int rb = 0; rb < 2; rb++) {
+        const int ii = 32 * ha + 16 * rb + il;
+        const float4 xi = L.rec[0][0][ii];
+        float kd[8], kc[8];
 #pragma unroll
-                for (int al = 0; al < 3; al++) m[al] = rr[al] * rF;
-#pragma unroll
-                for (int al = 0; al < 3; al++)
-#pragma unroll
-                    for (int be = 0; be < 3; be++) m[3 + 3 * al + be] = rr[al] * Fv[be];
-                m[12] = rF;
-#pragma unroll
-                for (int be = 0; be < 3; be++) m[13 + be] = Fv[be];
-            } else if (grp == 1) {
-#pragma unroll
-                for (int al = 0; al < 3; al++)
-#pragma unroll
-                    for (int ga = 0; ga < 3; ga++) m[3 * al + ga] = rr[al] * rr[ga] * Rv[ga];
-#pragma unroll
-                for (int k = 0; k < 7; k++) m[9 + k] = rr[k / 3] * Rv[k % 3];
-            } else if (grp == 2) {
-                m[0] = rr[2] * Rv[1];
-                m[1] = rr[2] * Rv[2];
-#pragma unroll
-                for (int ga = 0; ga < 3; ga++) m[2 + ga] = Rv[ga];
-            } else {
-#pragma unroll
-                for (int be = 0; be < 3; be++) {
-                    m[be] = Fv[be];
-                    m[3 + be] = Rv[be];
+        for (int q = 0; q < 8; q++) {
+            const int jj = 32 * hb + 8 * g + q;
+            const float4 xj = L.rec[1][0][jj];
+            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const float wv = rsq_fast(NEAR ? fmaxf(r2, 1e-12f) : r2), wv2 = wv * wv, wv3 = wv2 * wv;
+            kd[q] = fmaf(p3, wv3, p1 * wv);
+            kc[q] = wv3 * fmaf(-3.f * p3, wv2, p1);
+            if (NEAR) {
+                const bool nearp = r2 < MF_RNEAR * MF_RNEAR;
+                kd[q] = nearp ? 0.f : kd[q];
+                kc[q] = nearp ? 0.f : kc[q];
+                any_near |= nearp;
+                if (nearp && xi.w != 0.f && xj.w != 0.f && !(diag && ii == jj)) {
+                    // the exact RPY block (both branches), src/bdhitea_kernel.cu:38-58, 104-146
+                    const float r2c = fmaxf(r2, 1e-12f), rinv = rsq_fast(r2c);
+                    const RpyCoef cf = rpy_select(r2c, rinv, a, ia);
+                    const float cr = cf.crr * rinv * rinv;
+                    const float4 Ci = L.rec[0][3][ii], Fj = L.rec[1][1][jj], Rj = L.rec[1][2][jj];
+                    const float vx = fmaf(Ci.x, Rj.x, Fj.x), vy = fmaf(Ci.y, Rj.y, Fj.y),
+                                vz = fmaf(Ci.z, Rj.z, Fj.z);
+                    const float sv = cr * (dx * vx + dy * vy + dz * vz);
+                    ni[rb].x = fmaf(sv, dx, fmaf(cf.cii, vx, ni[rb].x));
+                    ni[rb].y = fmaf(sv, dy, fmaf(cf.cii, vy, ni[rb].y));
+                    ni[rb].z = fmaf(sv, dz, fmaf(cf.cii, vz, ni[rb].z));
+                    if (!diag) {
+                        const float4 Cj = L.rec[1][3][jj], Fi = L.rec[0][1][ii], Ri = L.rec[0][2][ii];
+                        const float ux = fmaf(Cj.x, Ri.x, Fi.x), uy = fmaf(Cj.y, Ri.y, Fi.y),
+                                    uz = fmaf(Cj.z, Ri.z, Fi.z);
+                        const float su = cr * (dx * ux + dy * uy + dz * uz);
+                        nj[q].x = fmaf(su, dx, fmaf(cf.cii, ux, nj[q].x));
+                        nj[q].y = fmaf(su, dy, fmaf(cf.cii, uy, nj[q].y));
+                        nj[q].z = fmaf(su, dz, fmaf(cf.cii, uz, nj[q].z));
+                    }
                 }
             }
         }
+        mf_split8(kd, kA[rb][0], kA[rb][1]);
+        mf_split8(kc, kA[rb][2], kA[rb][3]);
+    }
+}
+
+// kernel x table product of one 16-column tile: hi hi + hi lo + lo hi
+__device__ __forceinline__ f32x4_t mf_product(const bf16x8_t &khi, const bf16x8_t &klo, const bf16x8_t &bh,
+                                              const bf16x8_t &bl) {
+    const f32x4_t zero4 = {0.f, 0.f, 0.f, 0.f};
+    f32x4_t s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(khi, bh, zero4, 0, 0, 0);
+    s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(khi, bl, s, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(klo, bh, s, 0, 0, 0);
+}
+
+__global__ void __launch_bounds__(256)
+    k_tea_pairs_mfma(mte_params P, const int4 *units, const __bf16 *mom, const float4 *ctr, const float4 *rec,
+                     const int *rbead, int chunk_stride, float4 *partial) {
+    __shared__ MfLds L;
+    const int4 u = units[blockIdx.x];   // tile I, tile J, slot I, slot J
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, il = lane & 15;
+    const int ha = w >> 1, hb = w & 1;   // i half, j half of this wave's block
+    const bool diag = u.x == u.y;
+    // requests that depend on the unit alone: both tiles' records, this
+    // lane's fragments of both tables (16 bytes each), the two centres
+    float4 rv[4];
+    int rbv = -1;
+    const int side0 = tid >> 6, k0 = tid & 63;
+    if (tid < 2 * MF_TILE) {
+        const float4 *rt = rec + (size_t)(side0 ? u.y : u.x) * 4 * MF_TILE;
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const __bf16 h = (__bf16)m[q];
-            mhi[16 * grp + q][j] = h;
-            mlo[16 * grp + q][j] = (__bf16)(m[q] - (float)h);
+        for (int c = 0; c < 4; c++) rv[c] = rt[c * MF_TILE + k0];
+        rbv = rbead[(size_t)(side0 ? u.y : u.x) * MF_TILE + k0];
+    }
+    bf16x8_t tb[2][4][2];   // [I side: tile J's table, J side: tile I's table][column tile][hi, lo]
+#pragma unroll
+    for (int sd = 0; sd < 2; sd++) {
+        if (sd == 1 && diag) break;
+        const __bf16 *mt = mom + (size_t)(sd ? u.x : u.y) * 2 * MF_COLS * MF_TILE;
+        const int kk = 32 * (sd ? ha : hb) + 8 * g;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int col = 16 * t + il;
+            tb[sd][t][0] = *reinterpret_cast<const bf16x8_t *>(mt + col * MF_TILE + kk);
+            tb[sd][t][1] = *reinterpret_cast<const bf16x8_t *>(mt + (MF_COLS + col) * MF_TILE + kk);
         }
     }
-    __syncthreads();
-    // this lane's bead: row il of the wave's 16
-    const int pi = u.y + 16 * w + il;
-    const int li = pi < n ? order[base + pi] : n, bi = base + li;
-    const bool vi = li < n && !T.extra[bi];
-    float3 xi = make_float3(cx0, cy0, cz0), Cp = make_float3(0.f, 0.f, 0.f);
-    if (vi) {
-        xi = f3(S.xyz[bi]);
-        const float4 cc = ci[bi];
-        Cp = make_float3(bt * rsq_fast(1.f + w2b * cc.x), bt * rsq_fast(1.f + w2b * cc.y),
-                         bt * rsq_fast(1.f + w2b * cc.z));
+    const float4 YI = ctr[u.x], YJ = ctr[u.y];
+    if (tid < 2 * MF_TILE) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) L.rec[side0][c][k0] = rv[c];
+        L.bead[side0][k0] = rbv;
     }
+    __syncthreads();
     const float a = P.tea_a, ia = rcp_fast(P.tea_a);
-    const float p1 = 0.75f * a, p3 = 0.5f * a * a * a;
-    f32x4_t accA = {0.f, 0.f, 0.f, 0.f}, accC[3];
+    bf16x8_t kA[2][4];   // [rb][kd hi, kd lo, kc hi, kc lo]
+    float3 ni[2] = {make_float3(0.f, 0.f, 0.f), make_float3(0.f, 0.f, 0.f)};
+    float3 nj[8];
 #pragma unroll
-    for (int t = 0; t < 3; t++) accC[t] = accA;
-    float3 nacc = make_float3(0.f, 0.f, 0.f);
+    for (int q = 0; q < 8; q++) nj[q] = make_float3(0.f, 0.f, 0.f);
+    bool any_near = false;
+    const float gap = sqrtf((YI.x - YJ.x) * (YI.x - YJ.x) + (YI.y - YJ.y) * (YI.y - YJ.y) +
+                            (YI.z - YJ.z) * (YI.z - YJ.z)) - YI.w - YJ.w;
+    const bool near_unit = diag || !(gap > MF_RNEAR + 0.01f);
+    if (near_unit) mf_pairs<true>(L, ha, hb, g, il, diag, a, ia, kA, ni, nj, any_near);
+    else mf_pairs<false>(L, ha, hb, g, il, diag, a, ia, kA, ni, nj, any_near);
+    // I side: the block's pairs times tile J's tables (k = j of half hb)
+    f32x4_t acc[2][4], accj[2][4];
 #pragma unroll
-    for (int kb = 0; kb < 2; kb++) {
-        float kc[8], kd[8];
+    for (int t = 0; t < 4; t++) {
+        const int kh = t < 3 ? 2 : 0;   // kernel c for the 48 columns 0-47, cii for 48-63
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++) acc[rb][t] = mf_product(kA[rb][kh], kA[rb][kh + 1], tb[0][t][0], tb[0][t][1]);
+    }
+    // J side, first the transposed kernels of this wave's block
+    if (!diag) {
+        __bf16 *kt = L.u.kt[w];
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+#pragma unroll
+                for (int ty = 0; ty < 4; ty++) kt[mf_kt_index(ty, 8 * g + q, 16 * rb + il)] = kA[rb][ty][q];
+    }
+    __syncthreads();
+    if (!diag) {
+        const __bf16 *kt = L.u.kt[w];
+        bf16x8_t kB[2][4];
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+            for (int ty = 0; ty < 4; ty++)
+                kB[rb][ty] = *reinterpret_cast<const bf16x8_t *>(kt + mf_kt_index(ty, 16 * rb + il, 8 * g));
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int kh = t < 3 ? 2 : 0;
+#pragma unroll
+            for (int rb = 0; rb < 2; rb++)
+                accj[rb][t] = mf_product(kB[rb][kh], kB[rb][kh + 1], tb[1][t][0], tb[1][t][1]);
+        }
+    }
+    // near sums: I side over the four lane groups, J side over the 16 lanes of a row
+    if (near_unit && __ballot(any_near) != 0) {
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+            for (int m = 16; m < 64; m <<= 1) {
+                ni[rb].x += __shfl_xor(ni[rb].x, m);
+                ni[rb].y += __shfl_xor(ni[rb].y, m);
+                ni[rb].z += __shfl_xor(ni[rb].z, m);
+            }
+        if (!diag)
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+#pragma unroll
+                for (int m = 1; m < 16; m <<= 1) {
+                    nj[q].x += __shfl_xor(nj[q].x, m);
+                    nj[q].y += __shfl_xor(nj[q].y, m);
+                    nj[q].z += __shfl_xor(nj[q].z, m);
+                }
+    }
+    __syncthreads();   // every wave is done with its kt before out overwrites them
+    if (g == 0)
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++) {
+            float *o = L.near_i[hb][32 * ha + 16 * rb + il];
+            o[0] = ni[rb].x;
+            o[1] = ni[rb].y;
+            o[2] = ni[rb].z;
+        }
+    if (il == 0 && !diag)
 #pragma unroll
         for (int q = 0; q < 8; q++) {
-            const int jj = kb * 32 + 8 * g + q;
-            const float4 xj = jx[jj];
-            const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            const bool nearp = r2 < MF_RNEAR * MF_RNEAR;
-            const float wv = rsq_fast(fmaxf(r2, 1e-12f)), wv2 = wv * wv, wv3 = wv2 * wv;
-            kd[q] = nearp ? 0.f : fmaf(p3, wv3, p1 * wv);
-            kc[q] = nearp ? 0.f : fmaf(-3.f * p3, wv3 * wv2, p1 * wv3);
-            if (nearp && vi && xj.w != 0.f && u.z + jj != li) {
-                // the exact RPY block (both branches), src/bdhitea_kernel.cu:38-58, 104-146
-                const float r2c = fmaxf(r2, 1e-12f), rinv = rsq_fast(r2c);
-                const RpyCoef c = rpy_select(r2c, rinv, a, ia);
-                const float4 F = jf[jj], Rj = jr[jj];
-                const float vx = fmaf(Cp.x, Rj.x, F.x), vy = fmaf(Cp.y, Rj.y, F.y), vz = fmaf(Cp.z, Rj.z, F.z);
-                const float sv = (c.crr * rinv * rinv) * (dx * vx + dy * vy + dz * vz);
-                nacc.x = fmaf(sv, dx, fmaf(c.cii, vx, nacc.x));
-                nacc.y = fmaf(sv, dy, fmaf(c.cii, vy, nacc.y));
-                nacc.z = fmaf(sv, dz, fmaf(c.cii, vz, nacc.z));
-            }
+            float *o = L.near_j[ha][32 * hb + 8 * g + q];
+            o[0] = nj[q].x;
+            o[1] = nj[q].y;
+            o[2] = nj[q].z;
         }
-        bf16x8_t dhi, dlo, chi, clo;
-        mf_split8(kd, dhi, dlo);
-        mf_split8(kc, chi, clo);
-        const int k0 = kb * 32 + 8 * g;
-        {
-            const bf16x8_t bh = *reinterpret_cast<const bf16x8_t *>(&mhi[48 + il][k0]);
-            const bf16x8_t bl = *reinterpret_cast<const bf16x8_t *>(&mlo[48 + il][k0]);
-            accA = __builtin_amdgcn_mfma_f32_16x16x32_bf16(dhi, bh, accA, 0, 0, 0);
-            accA = __builtin_amdgcn_mfma_f32_16x16x32_bf16(dhi, bl, accA, 0, 0, 0);
-            accA = __builtin_amdgcn_mfma_f32_16x16x32_bf16(dlo, bh, accA, 0, 0, 0);
-        }
+    // lane (g, il) holds rows 4 g + r of column il of each 16-column tile
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
-            const bf16x8_t bh = *reinterpret_cast<const bf16x8_t *>(&mhi[16 * t + il][k0]);
-            const bf16x8_t bl = *reinterpret_cast<const bf16x8_t *>(&mlo[16 * t + il][k0]);
-            accC[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(chi, bh, accC[t], 0, 0, 0);
-            accC[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(chi, bl, accC[t], 0, 0, 0);
-            accC[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(clo, bh, accC[t], 0, 0, 0);
+    for (int t = 0; t < 4; t++) {
+        const int pc = mf_packed(16 * t + il);
+        if (pc < 0) continue;
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++) {
+            *reinterpret_cast<f32x4_t *>(&L.u.out[0][hb][pc][32 * ha + 16 * rb + 4 * g]) = acc[rb][t];
+            if (!diag) *reinterpret_cast<f32x4_t *>(&L.u.out[1][ha][pc][32 * hb + 16 * rb + 4 * g]) = accj[rb][t];
         }
     }
-    // lane (g, il) holds rows i = 4g + r of column il of each 16-column tile
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-#pragma unroll
-        for (int t = 0; t < 3; t++) out[w][4 * g + r][16 * t + il] = accC[t][r];
-        out[w][4 * g + r][48 + il] = accA[r];
-    }
-    nr[w][g][il][0] = nacc.x;
-    nr[w][g][il][1] = nacc.y;
-    nr[w][g][il][2] = nacc.z;
     __syncthreads();
-    if (lane >= 16 || li >= n) return;
+    if (tid >= 2 * MF_TILE) return;
+    const int side = tid >> 6, k = tid & 63;
+    if (side == 1 && diag) return;
+    const int bi = L.bead[side][k];
+    if (bi < 0) return;
     float3 f = make_float3(0.f, 0.f, 0.f);
-    if (vi) {
-        const float *o = out[w][il];
-        const float av[3] = {xi.x - cx0, xi.y - cy0, xi.z - cz0}, cp[3] = {Cp.x, Cp.y, Cp.z};
-        const float aF = av[0] * o[13] + av[1] * o[14] + av[2] * o[15];
-        float fv[3];
+    const float4 xt = L.rec[side][0][k];
+    if (xt.w != 0.f) {
+        float o[MF_USED];
 #pragma unroll
-        for (int al = 0; al < 3; al++) {
-            // cii part
-            float v = fmaf(cp[al], o[51 + al], o[48 + al]);
-            // c d (d . F)
-            v += o[al] - (av[0] * o[3 + 3 * al] + av[1] * o[4 + 3 * al] + av[2] * o[5 + 3 * al]) - av[al] * o[12] +
-                 av[al] * aF;
-            // sum_ga C'_ga c d_al d_ga R_ga
-#pragma unroll
-            for (int ga = 0; ga < 3; ga++) {
-                const int k = 3 * al + ga, kd = 4 * ga;   // r_al R_ga and r_ga R_ga
-                const float rR = k < 7 ? o[25 + k] : o[32 + k - 7];
-                const float rRd = kd < 7 ? o[25 + kd] : o[32 + kd - 7];
-                v = fmaf(cp[ga], o[16 + k] - av[ga] * rR - av[al] * rRd + av[al] * av[ga] * o[34 + ga], v);
-            }
-            fv[al] = v + nr[w][0][il][al] + nr[w][1][il][al] + nr[w][2][il][al] + nr[w][3][il][al];
-        }
-        f = make_float3(fv[0], fv[1], fv[2]);
+        for (int c = 0; c < MF_USED; c++) o[c] = L.u.out[side][0][c][k] + L.u.out[side][1][c][k];
+        const float4 Y = side ? YI : YJ;
+        const float4 cc = L.rec[side][3][k];
+        const float b[3] = {xt.x - Y.x, xt.y - Y.y, xt.z - Y.z}, cp[3] = {cc.x, cc.y, cc.z};
+        f = mf_reconstruct(o, b, cp);
+        const float *n0 = side ? L.near_j[0][k] : L.near_i[0][k], *n1 = side ? L.near_j[1][k] : L.near_i[1][k];
+        f.x += n0[0] + n1[0];
+        f.y += n0[1] + n1[1];
+        f.z += n0[2] + n1[2];
     }
-    partial[(size_t)u.w * chunk_stride + bi] = make_float4(f.x, f.y, f.z, 0.f);
+    const int slot = side ? u.z : u.w;
+    partial[(size_t)slot * chunk_stride + bi] = make_float4(f.x, f.y, f.z, 0.f);
 }
 
 hipError_t launch_tea_eps(mte_engine *e) {
@@ -901,8 +1065,11 @@ hipError_t launch_tea_pairs(mte_engine *e) {
     }
     if (e->tea.unit_chunk == MTE_TEA_MFMA) {
         if (e->periodic) return hipErrorInvalidValue;   // minimum images are not in the moment form
-        hipLaunchKernelGGL(k_tea_pairs_mfma, dim3(e->tea.nunits), dim3(256), 0, e->stream, e->P, e->T, e->S[e->cur],
-                           e->fxyz, e->tea.R, e->tea.ci, e->tea.beta, e->tea.units, e->rb_order, e->nb,
+        hipLaunchKernelGGL(k_tea_moments, dim3(e->tea.ntiles), dim3(MF_TILE), 0, e->stream, e->T, e->S[e->cur],
+                           e->fxyz, e->tea.R, e->tea.ci, e->tea.beta, e->rb_order, e->tea.tiles,
+                           static_cast<__bf16 *>(e->tea.mom), e->tea.ctr, e->tea.rec, e->tea.rbead);
+        hipLaunchKernelGGL(k_tea_pairs_mfma, dim3(e->tea.nunits), dim3(256), 0, e->stream, e->P, e->tea.units,
+                           static_cast<const __bf16 *>(e->tea.mom), e->tea.ctr, e->tea.rec, e->tea.rbead, e->nb,
                            e->tea.partial);
         return hipGetLastError();
     }

@@ -201,6 +201,7 @@ RunSetup load_run(const std::string &path, const std::vector<std::string> &ov) {
         s.ljscale = c.real("LJScale");
         s.ljsigma6 = (float)std::pow(c.real("LJSigma"), 6);
     }
+    s.lj_regularize = c.yesno("regularization", false);
     s.is_wall = c.yesno("repulsive_walls");
     s.rep_leftborder = c.real("rep_leftborder");
     s.rep_h = c.real("rep_h");
@@ -280,6 +281,7 @@ mte_params engine_params(const RunSetup &s) {
     p.tea_epsilon_freq = s.tea_on ? s.tea_epsilon_freq : 1;
     p.tea_capricious = s.tea_capricious;
     p.tea_epsmax = s.tea_epsmax;
+    p.lj_regularize = s.lj_regularize;
     return p;
 }
 

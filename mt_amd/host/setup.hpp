@@ -47,6 +47,7 @@ struct RunSetup {
     float a_barr_long = 0, r_barr_long = 0, w_barr_long = 0, a_barr_lat = 0, r_barr_lat = 0, w_barr_lat = 0;
     float C = 0, B_fi = 0, B_psi = 0, B_theta = 0, fi_0 = 0, psi_0 = 0, theta0_gdp = 0, theta0_gtp = 0;
     bool lj_on = false, is_wall = false;
+    bool lj_regularize = false;   // the reference's compile-time REGULARIZATION, key "regularization"
     float ljscale = 0, ljsigma6 = 0, rep_leftborder = 0, rep_h = 0, rep_r = 0, rep_eps = 0;
     // conditions file
     bool is_const_conc = false, hydrolysis = false;

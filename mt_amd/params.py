@@ -45,6 +45,7 @@ class Params(ctypes.Structure):
         ("tea_a", ctypes.c_float),
         ("tea_epsilon_freq", ctypes.c_int32), ("tea_capricious", ctypes.c_int32),
         ("tea_epsmax", ctypes.c_float),
+        ("lj_regularize", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -101,6 +102,7 @@ def derive(cfg: dict) -> Params:
     p.tea_epsilon_freq = int(cfg.get("tea_epsilon_freq", 0)) if hdi else 1
     p.tea_capricious = int(bool(cfg.get("tea_capricious", True)))
     p.tea_epsmax = f32(cfg.get("tea_epsmax", 999.0))
+    p.lj_regularize = int(bool(cfg.get("regularization", False)))
     return p
 
 

@@ -64,6 +64,7 @@ typedef struct {
     float tea_a;
     int32_t tea_epsilon_freq, tea_capricious;
     float tea_epsmax;
+    int32_t lj_regularize;   /* REGULARIZATION, src/compute_cuda.cu:482-486 */
 } or_params;
 
 /* Topology in CSR-batched form: trajectory t owns global beads
@@ -286,7 +287,8 @@ void or_forces(const or_params *p, const or_topo *t, const or_coord *rc, int64_t
                 if (dr != 0) dUdr += dbarr(p->a_barr_lat, p->r_barr_lat, p->w_barr_lat, dr) / dr;
             for (int q = 0; q < 6; q++) F[q] -= dUdr * gr[q];
         }
-        /* LJ: :470-495 (no REGULARIZATION) */
+        /* LJ: :470-495; with lj_regularize, the REGULARIZATION cap of :482-486
+         * (r_mon = 2 nm, src/mt.h:24) */
         if (p->lj_on) {
             for (int k = 0; k < t->lj_count[b]; k++) {
                 const int j = t->lj[b * t->max_lj + k];
@@ -296,8 +298,11 @@ void or_forces(const or_params *p, const or_topo *t, const or_coord *rc, int64_t
                 const real dx = d[0], dy = d[1], dz = d[2];
                 const real dr = sqrt(dx * dx + dy * dy + dz * dz);
                 if (dr < O_LJ_CUTOFF) {
-                    const real df = 6.0 / pow(dr, 8);
-                    const real s = (real)p->ljscale * (real)p->ljsigma6 * df;
+                    real df = 6.0 / pow(dr, 8);
+                    const real s6 = (real)p->ljscale * (real)p->ljsigma6;
+                    if (p->lj_regularize && s6 * df * dr > (real)p->gammaR * O_RMON / p->dt)
+                        df = (real)p->gammaR * O_RMON / (p->dt * dr * s6);
+                    const real s = s6 * df;
                     F[0] += s * dx; F[1] += s * dy; F[2] += s * dz;
                 }
             }

@@ -8,7 +8,8 @@ kind, each term family alone and all together):
     813-817), in both R_MON branches;
   * the barrier only between off-tubule beads, flags mixed (:259, :454);
   * seam weakening with seam_coeff = 2 (:447-452, :875-880);
-  * cylindrical and end walls (:497-517).
+  * cylindrical and end walls (:497-517);
+  * the REGULARIZATION cap on LJ pair forces (:482-486) as a run-time switch.
 Trajectories of 300 steps with injected noise run against the FP32 oracle in
 streamed and resident modes.
 """
@@ -105,6 +106,56 @@ def test_branch_trajectory_injected_noise(mode):
     da = np.abs(g[:, 3:6] - o[:, 3:6]).max()
     print("max |dx| nm", dx, "max |dang| rad", da, flush=True)
     assert dx < 1e-3 and da < 1e-3
+    eng.close()
+
+
+def _overlap_batch():
+    """uneven_batch with trajectory 1's first reservoir dimer moved rigidly so
+    that its bead k sits 0.9 nm radially outside the mobile lattice bead i
+    (its partner lands ~0.9 nm from i's upper neighbour): LJ pairs well
+    inside the range where the REGULARIZATION cap binds (~1.02 nm at
+    dt = 200 ps)."""
+    from tests.helpers import uneven_batch
+    b = uneven_batch()
+    s = int(b.off[1])
+    i, k = s + 128, s + 130
+    assert b.extra[k] and b.extra[k + 1] and not b.extra[i] and not b.fixed[i]
+    x = b.coords[i, :3].astype(np.float64)
+    u = np.array([x[0], x[1], 0.0]) / np.hypot(x[0], x[1])
+    shift = x + 0.9 * u - b.coords[k, :3].astype(np.float64)
+    b.coords[k:k + 2, :3] = (b.coords[k:k + 2, :3].astype(np.float64) + shift).astype(np.float32)
+    return b, i
+
+
+@pytest.mark.parametrize("mode", [MODE_STREAMED, MODE_RESIDENT])
+def test_lj_regularization(mode):
+    """REGULARIZATION (src/compute_cuda.cu:482-486) as the run-time switch:
+    forces against the FP64 oracle with the cap binding on two pairs, and 50
+    steps of injected noise against the FP32 oracle, in both modes."""
+    print(f"case LJ regularization, mode {mode}", flush=True)
+    from tests.helpers import base_params
+    b, i = _overlap_batch()
+    p = base_params(regularization=True)
+    cap = 2.0 * float(p.gammaR) / float(p.dt)
+    off = Oracle(base_params(), b.copy(), max_lj=64, precision="f64")
+    orc = Oracle(p, b.copy(), max_lj=64, precision="f64")
+    assert off.rebuild() == 0 and orc.rebuild() == 0
+    fo, f_off = orc.forces(), off.forces()
+    # the cap binds: bead i's unclamped push is ~2.4x the cap
+    assert np.linalg.norm(f_off[i, :3] - fo[i, :3]) > cap
+    eng = Engine(p, b.copy(), max_lj=64, mode=mode)
+    assert eng.mode == mode
+    if mode == MODE_STREAMED:
+        eng.rebuild()
+        assert rel_err(eng.forces(), fo, XYZ) < 1e-5
+    o32 = Oracle(p, b.copy(), max_lj=64, precision="f32")
+    nz = noise(np.random.default_rng(8), 50, b.nb)
+    eng.set_noise(nz)
+    eng.run(50)
+    assert o32.run(0, 50, noise=nz) == 0
+    dx = np.abs(eng.get_state()[:, :3] - o32.b.coords[:, :3]).max()
+    print("max |dx| nm", dx, flush=True)
+    assert dx < 1e-3
     eng.close()
 
 

@@ -118,8 +118,13 @@ def test_setup_params_match_python_derivation(in_run_dir):
     w2 = P.derive(dict(cfg, tea_on=True, tea_a=2.5, tea_epsilon_freq=10)).as_dict()
     for k in ("gammaR", "varR", "tea_a", "tea_epsilon_freq", "tea_on"):
         assert np.float32(s2.params.as_dict()[k]) == np.float32(w2[k]), k
+    # the reference's compile-time REGULARIZATION as the run-time key "regularization"
+    assert got["lj_regularize"] == 0
+    s3 = host.Setup("main.conf", ["regularization=yes"])
+    assert s3.params.as_dict()["lj_regularize"] == 1 == P.derive(dict(cfg, regularization=True)).lj_regularize
     s.close()
     s2.close()
+    s3.close()
 
 
 @pytest.mark.parametrize("assembly", [True, False])

@@ -228,3 +228,46 @@ def test_periodic_image_invariance():
         c[idx, :3] = (c[idx, :3].astype(np.float64) + shift).astype(np.float32)
     F1 = o.forces(c)
     assert np.abs(F1 - F0).max() < 5e-3 * np.abs(F0).max()
+
+
+def _lj_only(jitter=0.3, **over):
+    """312-bead tube with every bonded partner removed: LJ repulsion alone."""
+    p = _cfg(LJ_on=True, barrier=False, is_assembly=False, **over)
+    b = c1_batch(jitter=jitter)
+    b.harm_count[:] = 0
+    b.harm[:] = -1
+    b.lon_count[:] = 0
+    b.lat_count[:] = 0
+    return p, b
+
+
+def test_lj_regularization_cap():
+    # REGULARIZATION (src/compute_cuda.cu:482-486): each LJ pair force is
+    # capped at gammaR r_mon / dt (r_mon = 2 nm, src/mt.h:24).
+    p_off, b = _lj_only()
+    o = Oracle(p_off, b.copy(), precision="f64")
+    assert o.rebuild() == 0
+    F_off = o.forces()
+    assert np.abs(F_off[:, 3:6]).max() == 0 and np.abs(F_off[:, :3]).max() > 0
+    # at dt = 200 ps the cap (~1e4) lies far above every pair force of the
+    # lattice: the switch changes nothing
+    p_on, _ = _lj_only(regularization=True)
+    o = Oracle(p_on, b.copy(), precision="f64")
+    assert o.rebuild() == 0
+    np.testing.assert_array_equal(o.forces(), F_off)
+    # at dt = 1e12 ps the cap (~2e-6) lies below every pair force: each pair
+    # within the 6 nm force cutoff pushes with exactly the cap along the unit
+    # vector, whatever its distance
+    p_cap, _ = _lj_only(regularization=True, dt=1e12)
+    cap = 2.0 * float(p_cap.gammaR) / float(p_cap.dt)
+    o = Oracle(p_cap, b.copy(), precision="f64")
+    assert o.rebuild() == 0
+    F = o.forces()[:, :3]
+    x = b.coords[:, :3].astype(np.float64)
+    d = x[:, None, :] - x[None, :, :]
+    r = np.linalg.norm(d, axis=2)
+    np.fill_diagonal(r, np.inf)
+    near = r < 6.0
+    assert near.sum() > 1000
+    expect = cap * (np.where(near, 1.0 / r, 0.0)[:, :, None] * d).sum(1)
+    np.testing.assert_allclose(F, expect, rtol=1e-9, atol=1e-12 * cap)
