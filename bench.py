@@ -228,7 +228,7 @@ def tea_covariance_error(eng, b, t=0):
             "variance_max_abs": float(np.abs((B * B).sum(1) - 1.0).max()), "beta": float(beta[t])}
 
 
-def tea_bench(steps, warmup, cpu=True):
+def tea_bench(steps, warmup, cpu=True, chunk=0):
     """TEA-HI dynamics: 64 trajectories x 1040 beads, a = 2 nm, water
     viscosity (128 kcal ps / (mol nm^3)), dt = 1 ps, epsilon/beta every 100
     steps."""
@@ -238,7 +238,12 @@ def tea_bench(steps, warmup, cpu=True):
     b = topology.replicate(lattice.microtubule(40), 64, 0.05, 77)
     b.coords = lattice.wrap_angles(b.coords)
     eng = Engine(p, b, max_lj=48, mode=MODE_STREAMED)
-    tuned, report = eng.autotune(100)   # 100-step trials (~8 ms each): the shapes differ by ~10%
+    if chunk:
+        # a forced pair kernel (profiling one of them), no autotune
+        eng.set_tuning(tea_chunk=chunk)
+        tuned, report = eng.tuning(), [f"tea_chunk forced to {chunk}"]
+    else:
+        tuned, report = eng.autotune(100)   # 100-step trials (~8 ms each): the shapes differ by ~10%
     print("[bench] tea autotune: " + "; ".join(report), file=sys.stderr, flush=True)
     eng.run(warmup)
     # timed region: graph-replayed step segments
@@ -261,6 +266,7 @@ def tea_bench(steps, warmup, cpu=True):
            "ms_per_step": elapsed * 1e3 / steps,
            "config": {"workload": "TEA-HI: 64 trajectories x 1040-bead microtubule, a 2 nm, 0.89 cP, dt 1 ps, "
                                   "epsilon/beta every 100 steps", "beads": b.nb, "tuning": tuned},
+           "autotune": report,
            "roofline": {"bound": "mfma", "achieved": flops / avg / 1e12, "peak": BF16_DENSE_PEAK_TFS,
                         "unit": "TFLOP/s", "frac": flops / avg / 1e12 / BF16_DENSE_PEAK_TFS,
                         "traffic": read_tea_traffic(tuned["tea_chunk"]),
@@ -505,6 +511,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tea", action="store_true")
     ap.add_argument("--tea-steps", type=int, default=2000)
+    ap.add_argument("--tea-chunk", type=int, default=0,
+                    help="force the TEA pair kernel (64/128/256 one-sided, 1 symmetric, 2 MFMA) instead of autotuning")
     ap.add_argument("--no-mlp", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no mte_autotune)")
     ap.add_argument("--no-plain", action="store_true",
@@ -547,7 +555,7 @@ def main():
         if not args.no_c1:
             out["c1"] = c1_bench(args.c1_steps, cpu=not args.no_cpu)
         if not args.no_tea:
-            out["tea_hi"] = tea_bench(args.tea_steps, 20, cpu=not args.no_cpu)
+            out["tea_hi"] = tea_bench(args.tea_steps, 20, cpu=not args.no_cpu, chunk=args.tea_chunk)
         if not args.no_c4:
             out["c4"] = c4_bench(args.c4_steps, cpu=not args.no_cpu)
         if not args.no_mlp:
@@ -626,7 +634,7 @@ def main():
     if not args.no_c1:
         out["c1"] = c1_bench(args.c1_steps, cpu=not args.no_cpu)
     if not args.no_tea:
-        out["tea_hi"] = tea_bench(args.tea_steps, 20, cpu=not args.no_cpu)
+        out["tea_hi"] = tea_bench(args.tea_steps, 20, cpu=not args.no_cpu, chunk=args.tea_chunk)
     if not args.no_c4:
         out["c4"] = c4_bench(args.c4_steps, cpu=not args.no_cpu)
     if not args.no_mlp:
