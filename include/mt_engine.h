@@ -200,9 +200,11 @@ int mte_get_mode(const mte_engine *e, int32_t *mode);
 /* tea_chunk value: 64-bead chunks with each (i, j) RPY block evaluated once
  * for both beads (the other chunk values evaluate it for i and for j). */
 #define MTE_TEA_SYMMETRIC 1
-/* tea_chunk value: the pair sums on the matrix cores (open boundaries only):
- * scalar RPY kernels times per-j moment panels in split bf16 products with
- * FP32 accumulation, pairs closer than 16 nm summed exactly on the VALU. */
+/* tea_chunk value: the pair sums on the matrix cores (open boundaries and
+ * trajectories of at most 2048 beads; MTE_ERR_ARG otherwise): two scalar RPY
+ * kernels per pair times per-tile moment tables in split bf16 products with
+ * FP32 accumulation, each pair once for both beads, pairs closer than 6 nm
+ * summed exactly on the VALU. */
 #define MTE_TEA_MFMA 2
 
 /* tea_eps values: each pair's epsilon terms evaluated once for both beads,

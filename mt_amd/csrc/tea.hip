@@ -629,7 +629,7 @@ __global__ void __launch_bounds__(TEA_TILE)
 // over tile I, as k_tea_pairs_sym does with its 3 x 3 blocks.
 // Tiles are 64 consecutive beads of the spatial (Morton cell) order that
 // k_rebuild_order computes when a state is set, a compact patch (on a
-// microtubule at most ~18 nm from its centre); any order gives the same
+// microtubule at most ~24 nm from its centre); any order gives the same
 // sums in exact arithmetic, but the rounding of the expansion grows with
 // the tile's radius.  Pairs closer than MF_RNEAR (> 2a, so every product
 // pair takes the far branch) are summed exactly on the VALU instead: they
