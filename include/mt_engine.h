@@ -214,6 +214,14 @@ int mte_get_mode(const mte_engine *e, int32_t *mode);
 #define MTE_TEA_EPS_ONEPASS 1
 #define MTE_TEA_EPS_TWOSIDED 2
 
+/* forces values (the force pass of TEA and MLP-HI steps): each workgroup
+ * stages its bead's whole trajectory in LDS and reads neighbours there (only
+ * for trajectories that fit), or neighbours are gathered from global memory
+ * by workgroups of stream_block beads, each XCD taking a contiguous eighth of
+ * the beads so that their trajectories' state stays in that XCD's L2. */
+#define MTE_FORCES_LDS 1
+#define MTE_FORCES_GLOBAL 2
+
 typedef struct {
     int32_t resident_per_thread;  /* beads per thread of the resident kernel: 1, 2 or 3 */
     int32_t stream_block;         /* workgroup size of the streamed bead kernels: 64, 128 or 256 */
@@ -222,7 +230,8 @@ typedef struct {
     int32_t cell_stencil;         /* cell-list rebuilds: 1 = 27 cells of side r, 2 = 125 cells of side r/2 */
     int32_t mode;                 /* MTE_MODE_RESIDENT or MTE_MODE_STREAMED (plain dynamics) */
     int32_t tea_eps;              /* TEA epsilon: MTE_TEA_EPS_ONEPASS or MTE_TEA_EPS_TWOSIDED (0 without TEA) */
-    int32_t reserved[2];
+    int32_t forces;               /* TEA / MLP-HI force pass: MTE_FORCES_LDS or MTE_FORCES_GLOBAL */
+    int32_t reserved;
 } mte_tuning;
 
 int mte_get_tuning(const mte_engine *e, mte_tuning *t);

@@ -569,6 +569,7 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
             e->rb_traj_threads = 0;
         // forces with the trajectory staged in LDS when it fits
         e->forces_lds = forces_lds_bytes(nmax) <= (size_t)lds_limit ? forces_lds_bytes(nmax) : 0;
+        e->tune.forces = e->forces_lds > 0 ? MTE_FORCES_LDS : MTE_FORCES_GLOBAL;
     }
     ResidentPlan plan;
     const bool fits = resident_fits(e, &plan);
@@ -1361,6 +1362,13 @@ int mte_set_tuning(mte_engine *e, const mte_tuning *t) {
             return fail(e, MTE_ERR_ARG, "mte_set_tuning: cell_stencil must be 1 or 2");
         e->tune.cell_stencil = t->cell_stencil;
     }
+    if (t->forces) {
+        if (t->forces != MTE_FORCES_LDS && t->forces != MTE_FORCES_GLOBAL)
+            return fail(e, MTE_ERR_ARG, "mte_set_tuning: forces must be MTE_FORCES_LDS or MTE_FORCES_GLOBAL");
+        if (t->forces == MTE_FORCES_LDS && e->forces_lds == 0)
+            return fail(e, MTE_ERR_ARG, "mte_set_tuning: MTE_FORCES_LDS needs every trajectory to fit in LDS");
+        e->tune.forces = t->forces;
+    }
     clear_graphs(e);   // captured launches hold the old shapes
     return MTE_OK;
 }
@@ -1405,11 +1413,15 @@ int mte_autotune(mte_engine *e, int32_t trial_steps, mte_tuning *chosen, char *r
         log += line;
         return MTE_OK;
     };
-    auto trial = [&](const char *what, int value, double *ms) -> int {
-        mte_tuning t{};
+    auto pick = [](mte_tuning &t, const char *what, int value) {
         if (!strcmp(what, "stream_block")) t.stream_block = value;
         else if (!strcmp(what, "tea_chunk")) t.tea_chunk = value;
+        else if (!strcmp(what, "forces")) t.forces = value;
         else t.cell_stencil = value;
+    };
+    auto trial = [&](const char *what, int value, double *ms) -> int {
+        mte_tuning t{};
+        pick(t, what, value);
         return trial_with(t, what, value, ms);
     };
     auto family = [&](const char *what, std::initializer_list<int> values, int *field) -> int {
@@ -1427,9 +1439,7 @@ int mte_autotune(mte_engine *e, int32_t trial_steps, mte_tuning *chosen, char *r
         // mte_set_tuning applies the choice (and rebuilds what depends on it)
         (void)field;
         mte_tuning t{};
-        if (!strcmp(what, "stream_block")) t.stream_block = best;
-        else if (!strcmp(what, "tea_chunk")) t.tea_chunk = best;
-        else t.cell_stencil = best;
+        pick(t, what, best);
         return mte_set_tuning(e, &t);
     };
     // plain dynamics: every resident plan against every streamed workgroup size
@@ -1464,6 +1474,8 @@ int mte_autotune(mte_engine *e, int32_t trial_steps, mte_tuning *chosen, char *r
         }
         if ((rc = restore()) || (rc = mte_set_tuning(e, &best))) return rc;
     } else {
+        if (e->forces_lds > 0 && (rc = family("forces", {MTE_FORCES_LDS, MTE_FORCES_GLOBAL}, &e->tune.forces)))
+            return rc;
         if ((rc = family("stream_block", {64, 128, 256}, &e->tune.stream_block))) return rc;
         if (e->periodic || e->nmax > REBUILD_ORDER_MAX) {
             if ((rc = family("tea_chunk", {64, 128, 256, MTE_TEA_SYMMETRIC}, &e->tune.tea_chunk))) return rc;

@@ -10,6 +10,17 @@ namespace mte {
 // engine's tuning (64, 128 or 256).
 constexpr int STREAM_BLOCK = 256;
 
+// Bead of this thread in a grid of 8 x nb8 workgroups (grid_xcd).  The
+// dispatcher hands workgroup k to XCD k mod 8, so workgroup k takes the
+// (k / 8)-th block of the (k mod 8)-th eighth of the beads: each XCD works
+// on a contiguous eighth, whose trajectories' state then stays in that
+// XCD's L2 while their neighbours are gathered.  The bead a thread computes
+// does not change any result.
+__device__ __forceinline__ int xcd_bead(int bs) {
+    const int nb8 = (int)gridDim.x >> 3;
+    return (((int)blockIdx.x & 7) * nb8 + ((int)blockIdx.x >> 3)) * bs + (int)threadIdx.x;
+}
+
 constexpr int FORCE_TILE = 256;   // k_forces_lds: beads per workgroup (the 256-bead tiles of e->tiles)
 
 struct GlobalAcc {
@@ -43,7 +54,7 @@ template <int BS>
 __global__ void __launch_bounds__(BS)
     k_step_plain(mte_params P, DevTopo T, DevState Sin, DevState Sout, uint4 *seeds, const float *noise,
                  LatOffsets L) {
-    const int b = blockIdx.x * BS + threadIdx.x;
+    const int b = xcd_bead(BS);
     if (b >= T.nb) return;
     float4 x = Sin.xyz[b], a = Sin.ang[b];
     const bool extra = T.extra[b];
@@ -97,7 +108,7 @@ struct TeaDraw {
 template <int BS>
 __global__ void __launch_bounds__(BS)
     k_forces(mte_params P, DevTopo T, DevState S, float4 *fxyz, float4 *fang, TeaDraw D) {
-    const int b = blockIdx.x * BS + threadIdx.x;
+    const int b = xcd_bead(BS);
     if (b >= T.nb) return;
     float F[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (!T.extra[b]) {
@@ -167,7 +178,7 @@ size_t forces_lds_bytes(int nmax) { return (size_t)4 * sizeof(float4) * nmax; }
 template <int BS>
 __global__ void __launch_bounds__(BS)
     k_energies(mte_params P, DevTopo T, DevState S, double *en) {
-    const int b = blockIdx.x * BS + threadIdx.x;
+    const int b = xcd_bead(BS);
     if (b >= T.nb) return;
     float U[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (!T.extra[b]) {
@@ -199,17 +210,19 @@ __global__ void __launch_bounds__(256) k_traj_sum(DevTopo T, const double *en, d
 }
 
 static inline int grid_for(int n, int bs = STREAM_BLOCK) { return (n + bs - 1) / bs; }
+// Grid of a kernel that takes its beads through xcd_bead: a multiple of 8.
+static inline int grid_xcd(int n, int bs) { return 8 * ((grid_for(n, bs) + 7) / 8); }
 
 // Launch of a per-bead kernel template at the tuned workgroup size.
 #define MTE_LAUNCH_BS(KERNEL, e, ...)                                                                              \
     do {                                                                                                           \
         const int bs_ = (e)->tune.stream_block;                                                                    \
         if (bs_ == 64)                                                                                             \
-            hipLaunchKernelGGL(KERNEL<64>, dim3(grid_for((e)->nb, 64)), dim3(64), 0, (e)->stream, __VA_ARGS__);   \
+            hipLaunchKernelGGL(KERNEL<64>, dim3(grid_xcd((e)->nb, 64)), dim3(64), 0, (e)->stream, __VA_ARGS__);   \
         else if (bs_ == 128)                                                                                       \
-            hipLaunchKernelGGL(KERNEL<128>, dim3(grid_for((e)->nb, 128)), dim3(128), 0, (e)->stream, __VA_ARGS__); \
+            hipLaunchKernelGGL(KERNEL<128>, dim3(grid_xcd((e)->nb, 128)), dim3(128), 0, (e)->stream, __VA_ARGS__); \
         else                                                                                                       \
-            hipLaunchKernelGGL(KERNEL<256>, dim3(grid_for((e)->nb, 256)), dim3(256), 0, (e)->stream, __VA_ARGS__); \
+            hipLaunchKernelGGL(KERNEL<256>, dim3(grid_xcd((e)->nb, 256)), dim3(256), 0, (e)->stream, __VA_ARGS__); \
     } while (0)
 
 // The updater's tubule classification (mt_length, src/updater.cpp:164-205)
@@ -253,7 +266,7 @@ hipError_t launch_step_plain(mte_engine *e, const float *noise_step) {
 }
 
 static hipError_t launch_forces_draw(mte_engine *e, TeaDraw D) {
-    if (e->forces_lds > 0) {
+    if (e->tune.forces == MTE_FORCES_LDS && e->forces_lds > 0) {
         const hipError_t s = hipFuncSetAttribute((const void *)k_forces_lds,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->forces_lds);
         if (s != hipSuccess) return s;

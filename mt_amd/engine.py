@@ -21,6 +21,8 @@ MODE_AUTO, MODE_STREAMED, MODE_RESIDENT = 0, 1, 2
 LISTS_AUTO, LISTS_ALLPAIRS, LISTS_CELLS = 0, 1, 2
 TEA_SYMMETRIC = 1   # tea_chunk: 64-bead chunks, each pair block evaluated once for both beads
 TEA_MFMA = 2        # tea_chunk: pair sums on the matrix cores (open boundaries)
+FORCES_LDS = 1      # forces: the trajectory staged in LDS (TEA / MLP-HI force pass)
+FORCES_GLOBAL = 2   # forces: neighbours gathered from global memory, XCD-aware block order
 TEA_EPS_ONEPASS, TEA_EPS_TWOSIDED = 1, 2   # tea_eps: epsilon terms once per pair, or once per bead of it
 ERRORS = {1: "MTE_ERR_ARG", 2: "MTE_ERR_HIP", 3: "MTE_ERR_OVERFLOW", 4: "MTE_ERR_TEA", 5: "MTE_ERR_STATE",
           6: "MTE_ERR_NOISE"}
@@ -51,7 +53,7 @@ class Tuning(ctypes.Structure):
     """mte_tuning: launch shapes chosen by mte_autotune (0 = keep)."""
     _fields_ = [("resident_per_thread", ctypes.c_int32), ("stream_block", ctypes.c_int32),
                 ("tea_chunk", ctypes.c_int32), ("cell_stencil", ctypes.c_int32), ("mode", ctypes.c_int32),
-                ("tea_eps", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
+                ("tea_eps", ctypes.c_int32), ("forces", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

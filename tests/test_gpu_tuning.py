@@ -8,7 +8,8 @@ import numpy as np
 import pytest
 
 from mt_amd import params as P, workloads
-from mt_amd.engine import Engine, EngineError, MODE_RESIDENT, MODE_STREAMED, TEA_SYMMETRIC
+from mt_amd.engine import (Engine, EngineError, FORCES_GLOBAL, FORCES_LDS, MODE_RESIDENT, MODE_STREAMED,
+                           TEA_SYMMETRIC)
 from tests.helpers import c1_batch
 
 pytestmark = pytest.mark.gpu
@@ -94,6 +95,33 @@ def test_autotune_tea_and_cells():
     t.close()
 
 
+def test_tea_force_kernels_agree_bit_for_bit():
+    """The TEA force pass with the trajectory staged in LDS and with global
+    gathers in the XCD-aware block order evaluates the same bead_force on the
+    same values: trajectories agree bit for bit, at every workgroup size, and
+    the autotuner times both."""
+    print("case TEA force pass: LDS-staged vs global gathers", flush=True)
+    p = P.derive(dict(P.TEA, tea_epsilon_freq=10))
+    b = c1_batch(ntraj=5, n_dimers=20)
+    ref = Engine(p, b.copy(), max_lj=48, mode=MODE_STREAMED, rseed=3)
+    assert ref.tuning()["forces"] == FORCES_LDS
+    ref.run(60)
+    want = ref.get_state()
+    ref.close()
+    for bs in (64, 128, 256):
+        e = Engine(p, b.copy(), max_lj=48, mode=MODE_STREAMED, rseed=3)
+        e.set_tuning(forces=FORCES_GLOBAL, stream_block=bs)
+        e.run(60)
+        np.testing.assert_array_equal(e.get_state(), want)
+        e.close()
+    t = Engine(p, b.copy(), max_lj=48, mode=MODE_STREAMED, rseed=3)
+    chosen, report = t.autotune(5)
+    print(chosen, report, flush=True)
+    assert sum(l.startswith("forces") for l in report) == 2
+    assert chosen["forces"] in (FORCES_LDS, FORCES_GLOBAL)
+    t.close()
+
+
 def test_set_tuning_rejects_unusable_shapes():
     print("case set_tuning argument checks", flush=True)
     p = P.derive(P.BASE)
@@ -108,6 +136,8 @@ def test_set_tuning_rejects_unusable_shapes():
         e.set_tuning(tea_chunk=128)              # no TEA in this run
     with pytest.raises(EngineError):
         e.set_tuning(tea_eps=1)                  # no TEA in this run
+    with pytest.raises(EngineError):
+        e.set_tuning(forces=3)
     e.set_tuning(resident_per_thread=3)
     assert e.tuning()["resident_per_thread"] == 3
     e.close()
