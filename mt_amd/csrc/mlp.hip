@@ -49,16 +49,20 @@ __device__ __forceinline__ float taper(float r) {
 // test rejects the beads of other cells sharing a bucket.  Keeps the 32
 // nearest (distance, then index) and sums the exact label over every
 // neighbour within 12 nm (LABELS: only on check steps and for training data).
-template <bool LABELS>
+// The 32-nearest lists live in LDS with trajectory-local indices of type JT:
+// 16-bit where every trajectory has at most 65536 beads (12 KB per wave
+// instead of 16, so more waves per CU hide the candidates' load latency).
+template <bool LABELS, typename JT>
 __global__ void __launch_bounds__(MLP_BLOCK)
     k_mlp_local(mte_params P, DevTopo T, DevState S, CellTables C, float *feat, float *targ) {
     __shared__ float sd[MLP_BLOCK * MLP_NN];
-    __shared__ int sj[MLP_BLOCK * MLP_NN];
+    __shared__ JT sj[MLP_BLOCK * MLP_NN];
     const int b = blockIdx.x * MLP_BLOCK + threadIdx.x;
     if (b >= T.nb) return;
     float *nd = sd + threadIdx.x * MLP_NN;
-    int *nj = sj + threadIdx.x * MLP_NN;
+    JT *nj = sj + threadIdx.x * MLP_NN;
     const int t = T.tid[b];
+    const int base = T.tbase[t];
     const float box = T.tbox[t];
     const float3 ri = f3(S.xyz[b]);
     float A[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -140,15 +144,16 @@ __global__ void __launch_bounds__(MLP_BLOCK)
                         for (int y = 0; y < 3; y++) A[3 * x + y] += w * (crr * e[x] * e[y] + (x == y ? cii : 0.f));
                 }
                 // 32 nearest, ordered by (r2, j)
-                if (cnt < MLP_NN || r2 < nd[cnt - 1] || (r2 == nd[cnt - 1] && j < nj[cnt - 1])) {
+                const int jl = j - base;
+                if (cnt < MLP_NN || r2 < nd[cnt - 1] || (r2 == nd[cnt - 1] && jl < (int)nj[cnt - 1])) {
                     int k = cnt < MLP_NN ? cnt++ : MLP_NN - 1;
-                    while (k > 0 && (nd[k - 1] > r2 || (nd[k - 1] == r2 && nj[k - 1] > j))) {
+                    while (k > 0 && (nd[k - 1] > r2 || (nd[k - 1] == r2 && (int)nj[k - 1] > jl))) {
                         nd[k] = nd[k - 1];
                         nj[k] = nj[k - 1];
                         k--;
                     }
                     nd[k] = r2;
-                    nj[k] = j;
+                    nj[k] = (JT)jl;
                 }
             }
         }
@@ -165,7 +170,7 @@ __global__ void __launch_bounds__(MLP_BLOCK)
                 const int k = k0 + u;
                 float3 d = make_float3(0.f, 0.f, 0.f);
                 if (k < cnt) {
-                    d = min_image(sub3(f3(S.xyz[nj[k]]), ri), box);
+                    d = min_image(sub3(f3(S.xyz[base + (int)nj[k]]), ri), box);
                     d = make_float3(d.x / MLP_RC, d.y / MLP_RC, d.z / MLP_RC);
                 }
                 v[3 * u] = d.x;
@@ -724,12 +729,20 @@ hipError_t launch_mlp_local(mte_engine *e, float *feat, float *targ, bool build_
         const hipError_t s = launch_mlp_cells(e);
         if (s != hipSuccess) return s;
     }
-    if (targ)
-        hipLaunchKernelGGL(k_mlp_local<true>, dim3(blocks(e->nb, MLP_BLOCK)), dim3(MLP_BLOCK), 0, e->stream, e->P,
-                           e->T, e->S[e->cur], e->cells, feat, targ);
+    const dim3 grid(blocks(e->nb, MLP_BLOCK)), block(MLP_BLOCK);
+    const bool narrow = e->nmax <= 65536;   // trajectory-local indices fit 16 bits
+    if (targ && narrow)
+        hipLaunchKernelGGL((k_mlp_local<true, uint16_t>), grid, block, 0, e->stream, e->P, e->T, e->S[e->cur],
+                           e->cells, feat, targ);
+    else if (targ)
+        hipLaunchKernelGGL((k_mlp_local<true, int>), grid, block, 0, e->stream, e->P, e->T, e->S[e->cur], e->cells,
+                           feat, targ);
+    else if (narrow)
+        hipLaunchKernelGGL((k_mlp_local<false, uint16_t>), grid, block, 0, e->stream, e->P, e->T, e->S[e->cur],
+                           e->cells, feat, targ);
     else
-        hipLaunchKernelGGL(k_mlp_local<false>, dim3(blocks(e->nb, MLP_BLOCK)), dim3(MLP_BLOCK), 0, e->stream, e->P,
-                           e->T, e->S[e->cur], e->cells, feat, targ);
+        hipLaunchKernelGGL((k_mlp_local<false, int>), grid, block, 0, e->stream, e->P, e->T, e->S[e->cur], e->cells,
+                           feat, targ);
     return hipGetLastError();
 }
 
