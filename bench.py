@@ -459,6 +459,7 @@ def mlp_bench(steps, warmup, iters):
         print(f"[bench] mlp section {mode}: {r['value']:.4g} monomer-steps/s, D {D:.5g}", file=sys.stderr,
               flush=True)
     tea = res["tea"]
+    diss = mlp_dissociation(w)
     return {"value": res["mlp"]["value"], "unit": "monomer-steps/s", "steps": steps, "warmup": warmup,
             "ms_per_step": res["mlp"]["ms_per_step"],
             "config": {"workload": "MLP-HI: 32 trajectories x (125-dimer microtubule, 3250 beads + 20,000 free "
@@ -468,10 +469,71 @@ def mlp_bench(steps, warmup, iters):
             "diffusion_error_vs_tea": res["mlp"]["D_nm2_per_ps"] / tea["D_nm2_per_ps"] - 1.0,
             "diffusion_error_no_hi_vs_tea": res["none"]["D_nm2_per_ps"] / tea["D_nm2_per_ps"] - 1.0,
             "runs": res,
+            "dissociation": diss,
             "training": {"samples": int(len(X)), "iterations": iters, "batch": 8192, "data_s": data_s,
                          "train_s": train_s, "loss_start": float(losses[:5].mean()),
                          "loss_end": float(losses[-20:].mean())},
             "roofline": res["mlp"]["roofline"], "roofline_mfma": res["mlp"]["roofline_mfma"]}
+
+
+def mlp_dissociation(w, ntraj=16, steps=5000):
+    """Dissociation rate under MLP-HI against TEA and against no HI: 16 GDP
+    microtubules of 312 beads with the weakened wells and temperature at which
+    protofilaments peel within 1 us (D_lat 1.5, D_long 6.0, 600 K), the
+    reference's viscosity and dt (200 ps), a 2 nm, 5000 steps from the same
+    seeds.  The rate is the count of bonded-list entries (longitudinal and
+    lateral) lost since step 0 per trajectory per us, from the engine's own
+    pair rebuild (src/compute_cuda.cu:527-674; the count src/updater.cpp:154-227
+    turns into mt_length).  The network is the one trained for C5: the HI
+    tensor it predicts depends on the geometry only.  MLP-HI runs twice: as
+    deployed (the judge samples the network's error every 100 steps and hands
+    the run to TEA above 2%), and with the judge's tolerance at 100% so that
+    the network drives the whole run; the largest sampled error is reported
+    for both."""
+    from mt_amd import lattice, params, topology, mlp_hi
+    from mt_amd.engine import Engine, HI_MLP, HI_DEFAULT, MODE_STREAMED
+    over = dict(D_lat=1.5, D_long=6.0, Temp=600.0)
+    pt = params.derive(dict(params.BASE, tea_on=True, tea_a=2.0, tea_epsilon_freq=100, tea_capricious=True,
+                            tea_epsmax=999.0, **over))
+    pn = params.derive(dict(params.BASE, **over))
+    b = topology.replicate(lattice.microtubule(12), ntraj, 0.05, 99)
+    b.coords = lattice.wrap_angles(b.coords)
+    b.gtp[:] = 0
+    n = b.nb // ntraj
+    out, lost = {}, {}
+    for mode in ("tea", "mlp", "mlp_no_judge", "none"):
+        eng = Engine(pn if mode == "none" else pt, b.copy(), max_lj=64, mode=MODE_STREAMED, rseed=4321)
+        eng.rebuild()
+        L0 = eng.lists()
+        if mode.startswith("mlp"):
+            eng.mlp_set_weights(w)
+            eng.set_hi_mode(HI_MLP, check_every=100, tolerance=0.02 if mode == "mlp" else 1.0)
+        elif mode == "tea":
+            eng.set_hi_mode(HI_DEFAULT)
+        eng.run(steps)
+        eng.rebuild()
+        L1 = eng.lists()
+        k = ((L0["lat_count"] - L1["lat_count"]) + (L0["lon_count"] - L1["lon_count"])).reshape(ntraj, n).sum(1)
+        t_us = steps * pt.dt * 1e-6
+        lost[mode] = k.astype(np.float64) / t_us
+        out[mode] = {"per_trajectory_per_us": float(lost[mode].mean()),
+                     "stderr": float(lost[mode].std(ddof=1) / np.sqrt(ntraj))}
+        if mode.startswith("mlp"):
+            err, fb, bad = eng.mlp_status()
+            out[mode].update(sampled_error_max=float(np.max(err)), fallback=bool(fb))
+        eng.close()
+    for mode in ("mlp", "mlp_no_judge", "none"):
+        d = lost[mode] - lost["tea"]
+        out[f"error_{mode}_vs_tea"] = float(d.mean() / lost["tea"].mean())
+        out[f"error_{mode}_vs_tea_stderr"] = float(d.std(ddof=1) / np.sqrt(ntraj) / lost["tea"].mean())
+    out["workload"] = (f"{ntraj} GDP microtubules x 312 beads, D_lat 1.5, D_long 6.0, 600 K, viscosity 2.85e4, "
+                       f"dt 200 ps, a 2 nm, {steps} steps from the same seeds; bonded-list entries lost per "
+                       "trajectory per us; MLP-HI as deployed (judge at 2%) and with the judge's tolerance at 100%")
+    print(f"[bench] dissociation per trajectory per us: TEA {out['tea']['per_trajectory_per_us']:.1f}, "
+          f"MLP-HI {out['mlp']['per_trajectory_per_us']:.1f} (network throughout "
+          f"{out['mlp_no_judge']['per_trajectory_per_us']:.1f}), no HI {out['none']['per_trajectory_per_us']:.1f}",
+          file=sys.stderr, flush=True)
+    return out
 
 
 def mlp_profile(steps):
