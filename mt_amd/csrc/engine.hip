@@ -259,17 +259,38 @@ static void generate_seeds(int32_t seed, size_t nwords, uint32_t *out) {
     }
 }
 
+namespace mte {
+// The list-capacity flags into the host's pinned, device-mapped words.
+__global__ void k_publish_flags(const int *overflow, const int *frozen, int *out) {
+    out[0] = *(volatile const int *)overflow;
+    out[1] = *(volatile const int *)frozen;
+}
+}  // namespace mte
+
 // Kernel timing with HIP events on the engine's stream.  Both events of a
-// pair are created before the start is recorded, so no host work sits
-// between the kernel and its stop event.
+// pair are taken before the start is recorded, so no host work sits
+// between the kernel and its stop event; read events go back to a pool.
 struct EvPair {
     hipEvent_t start = nullptr, stop = nullptr;
 };
 
+static hipEvent_t take_event(mte_engine *e) {
+    if (!e->ev_pool.empty()) {
+        const hipEvent_t ev = e->ev_pool.back();
+        e->ev_pool.pop_back();
+        return ev;
+    }
+    // Timing only: without the system-scope fence a marker writes back no
+    // caches, so it neither costs the following work nor skews the times.
+    hipEvent_t ev = nullptr;
+    (void)hipEventCreateWithFlags(&ev, hipEventDisableSystemFence);
+    return ev;
+}
+
 static void time_begin(mte_engine *e, const char *name, EvPair *ev) {
     if (!e->timing) return;
-    (void)hipEventCreate(&ev->start);
-    (void)hipEventCreate(&ev->stop);
+    ev->start = take_event(e);
+    ev->stop = take_event(e);
     (void)hipEventRecord(ev->start, e->stream);
     (void)name;
 }
@@ -286,8 +307,8 @@ static void flush_timers(mte_engine *e) {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) kv.second.total_ms += ms;
             kv.second.launches++;
-            (void)hipEventDestroy(pr.first);
-            (void)hipEventDestroy(pr.second);
+            e->ev_pool.push_back(pr.first);
+            e->ev_pool.push_back(pr.second);
         }
         kv.second.pending.clear();
     }
@@ -302,6 +323,8 @@ int mte_destroy(mte_engine *e) {
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     flush_timers(e);
+    for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
+    if (e->flags_host) (void)hipHostFree(e->flags_host);
     for (auto &kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
     for (void *p : e->allocations) (void)hipFree(p);
     for (void *p : {(void *)e->mlp.x, (void *)e->mlp.t, (void *)e->mlp.hid, (void *)e->mlp.y, (void *)e->mlp.g4,
@@ -495,6 +518,13 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
         e->err = "mte_create: memset failed";
         return bail(MTE_ERR_HIP);
     }
+    if (hipHostMalloc((void **)&e->flags_host, 2 * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent) !=
+            hipSuccess ||
+        hipHostGetDevicePointer((void **)&e->flags_dev, e->flags_host, 0) != hipSuccess) {
+        e->err = "mte_create: pinned allocation failed for the list-capacity flags";
+        return bail(MTE_ERR_HIP);
+    }
+    e->flags_host[0] = e->flags_host[1] = 0;
     if (e->P.tea_on && hipMemsetAsync(e->tea.status, 0, sizeof(int), e->stream) != hipSuccess) {
         e->err = "mte_create: memset failed";
         return bail(MTE_ERR_HIP);
@@ -694,11 +724,20 @@ int mte_get_step(const mte_engine *e, int64_t *step) {
 // the step counter and the injected-noise cursor go back by that many; the
 // next run rebuilds the lists again (and overflows again unless the state
 // changed).
-static int check_overflow(mte_engine *e) {
-    int ov = 0, frozen = 0;
-    HIPCHK(hipMemcpyAsync(&ov, e->T.overflow, sizeof(int), hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipMemcpyAsync(&frozen, e->T.frozen, sizeof(int), hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipStreamSynchronize(e->stream));
+//
+// The flags reach the host through k_publish_flags, enqueued after the work
+// that may set them and read once the stream has drained: one small kernel
+// in place of two device-to-host copies and a synchronisation of their own.
+static hipError_t publish_flags(mte_engine *e) {
+    hipLaunchKernelGGL(mte::k_publish_flags, dim3(1), dim3(1), 0, e->stream, e->T.overflow, e->T.frozen,
+                       e->flags_dev);
+    return hipGetLastError();
+}
+
+// The published flags, after a stream synchronisation that followed
+// publish_flags.
+static int eval_overflow(mte_engine *e) {
+    const int ov = ((volatile int *)e->flags_host)[0], frozen = ((volatile int *)e->flags_host)[1];
     if (!ov) return MTE_OK;
     HIPCHK(hipMemsetAsync(e->T.overflow, 0, sizeof(int), e->stream));
     HIPCHK(hipMemsetAsync(e->T.frozen, 0, sizeof(int), e->stream));
@@ -713,6 +752,12 @@ static int check_overflow(mte_engine *e) {
                   "step %lld; the state is that step's",
                   (long long)e->step);
     return fail(e, MTE_ERR_OVERFLOW, buf);
+}
+
+static int check_overflow(mte_engine *e) {
+    HIPCHK(publish_flags(e));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return eval_overflow(e);
 }
 
 static int rebuild_now(mte_engine *e) {
@@ -918,9 +963,10 @@ int mte_run(mte_engine *e, int64_t nsteps) {
         e->step += seg;
         done += seg;
     }
+    if (need_lists) HIPCHK(publish_flags(e));
     HIPCHK(hipStreamSynchronize(e->stream));
     flush_timers(e);
-    if (need_lists) return check_overflow(e);
+    if (need_lists) return eval_overflow(e);
     return MTE_OK;
 }
 

@@ -144,7 +144,10 @@ struct mte_engine {
     size_t forces_lds = 0;     // > 0: forces by k_forces_lds with this much dynamic LDS
     int rb_traj_threads = 0;   // > 0: list rebuilds by k_rebuild_traj with this workgroup size
     size_t rb_traj_lds = 0;    // and this much dynamic LDS
-    int *d_overflow = nullptr;
+    // {overflow, frozen} copied by k_publish_flags at the end of a run into
+    // pinned, device-mapped host memory, so that the run's one stream
+    // synchronisation also delivers them (flags_dev is its device address).
+    int *flags_host = nullptr, *flags_dev = nullptr;
     mte::TeaBuffers tea;
     mte::CellTables cells;
     bool use_cells = false;
@@ -170,6 +173,7 @@ struct mte_engine {
     std::map<std::tuple<int64_t, int64_t, int>, hipGraphExec_t> graphs;
     bool use_graphs = true;
     std::map<std::string, mte::TimingSlot> timers;
+    std::vector<hipEvent_t> ev_pool;   // timing events whose pair was read, for reuse
     std::vector<void *> allocations;
     std::string err;
 };

@@ -4,7 +4,8 @@
 // One workgroup owns one trajectory.  LDS holds, per bead, the position and
 // the three rotated offsets with the Euler angles packed into their spare
 // lanes (64 B), plus the bead's neighbour lists compacted into a uint16 CSR
-// (local index << 1 | flag).  Each thread owns PT beads and keeps their two
+// (local index << 1 | flag) whose places are sized for the full lists, so a
+// distance-filtered list and its later refill share one layout.  Each thread owns PT beads and keeps their two
 // HybridTaus states in registers.  A launch runs every step up to the next
 // list-rebuild boundary (LJPairsUpdateFreq steps; 100 in the batched
 // workload), after the rebuild kernel of lists.hip has refreshed the lists
@@ -219,7 +220,7 @@ __global__ void __launch_bounds__(MAXT)
     // has work (its lanes past the mobile beads hold the immobile ones)
     const int m2 = n - (int)blockDim.x;
     // (flags[0] was cleared before the first barrier; the skin test clears it
-    // again in compact before its first use)
+    // again in fill before its first use)
     __syncthreads();
     if (PT == 2 && threadIdx.x >= 256 && mob[PT - 1]) flags[0] = 1;
     __syncthreads();
@@ -251,84 +252,99 @@ __global__ void __launch_bounds__(MAXT)
         const float3 d = min_image(sub3(f3(lx[i]), f3(lx[j])), box);
         return d.x * d.x + d.y * d.y + d.z * d.z < rf2;
     };
-    // Compact this trajectory's lists into LDS: counts -> block scan of the
-    // per-bead totals into offsets -> copy (when `copy` and they fit).
-    // Returns the number of entries.
-    auto compact = [&](float rf2, bool copy) {
+    // Offsets of this trajectory's full lists in LDS: the per-bead totals go
+    // through a block scan into hoff.  Returns the number of entries.  Every
+    // later fill writes at these offsets (a filtered list uses a prefix of
+    // its bead's place), so the layout is computed once per launch.
+    auto layout = [&]() {
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
             const int b = base + i;
-            const int h = T.harm[b] >= 0 ? 1 : 0;
-            const int nlon = T.lon_n[b], nlat = T.lat_n[b];
-            int nlj = 0;
-            if (P.lj_on) {
-                const int m = T.lj_n[b];
-                if (rf2 == 0.f) nlj = m;
-                else
-                    for (int q = 0; q < m; q++) nlj += lj_keep(i, T.lj[q * nb + b] - base, rf2) ? 1 : 0;
-            }
-            hcnt[i] = pack_counts(h, nlon, nlat, nlj);
-            hoff[i] = h + nlon + nlat + nlj;
+            hoff[i] = (T.harm[b] >= 0 ? 1 : 0) + T.lon_n[b] + T.lat_n[b] + (P.lj_on ? T.lj_n[b] : 0);
         }
-        // Chunked exclusive scan: thread t owns beads [t*c, (t+1)*c); the
-        // totals go through hoff, indexed by bead.
+        // Chunked exclusive scan: thread t owns beads [t*c, (t+1)*c); a wave
+        // scans its lanes' chunk totals by shuffles, the first wave the
+        // wave totals.
         const int c = (n + blockDim.x - 1) / blockDim.x;
         __syncthreads();
         int run = 0;
         const int lo = threadIdx.x * c, hi = min(n, lo + c);
         for (int i = lo; i < hi; i++) run += hoff[i];
-        scan[threadIdx.x] = run;
-        __syncthreads();
-        for (int d = 1; d < (int)blockDim.x; d <<= 1) {
-            const int v = threadIdx.x >= (unsigned)d ? scan[threadIdx.x - d] : 0;
-            __syncthreads();
-            scan[threadIdx.x] += v;
-            __syncthreads();
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        int inc = run;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int v = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += v;
         }
-        const int total = scan[blockDim.x - 1];
-        int acc = scan[threadIdx.x] - run;   // exclusive prefix of this chunk
+        if (lane == 63) scan[wv] = inc;
+        __syncthreads();
+        if (wv == 0) {
+            int w = lane < nw ? scan[lane] : 0;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int v = __shfl_up(w, d, 64);
+                if (lane >= d) w += v;
+            }
+            if (lane < nw) scan[lane] = w;
+        }
+        __syncthreads();
+        const int total = scan[nw - 1];
+        int acc = (wv ? scan[wv - 1] : 0) + inc - run;   // exclusive prefix of this chunk
         for (int i = lo; i < hi; i++) {
             const int v = hoff[i];
             hoff[i] = acc;
             acc += v;
         }
         __syncthreads();
-        if (copy && total <= entry_cap) {
-            for (int i = threadIdx.x; i < n; i += blockDim.x) {
-                const int b = base + i;
-                int o = hoff[i];
-                const int hc = hcnt[i];
-                if (hc & 1) {
-                    const int e = T.harm[b];
-                    ent[o++] = (uint16_t)((((e >> 1) - base) << 1) | (e & 1));
-                }
-                for (int q = 0; q < ((hc >> 1) & 15); q++) {
-                    const int e = T.lon[q * nb + b];
-                    ent[o++] = (uint16_t)((((e >> 1) - base) << 1) | (e & 1));
-                }
-                for (int q = 0; q < ((hc >> 5) & 31); q++) {
-                    const int e = T.lat[q * nb + b];
-                    ent[o++] = (uint16_t)((((e >> 1) - base) << 1) | (e & 1));
-                }
-                if (P.lj_on)
-                    for (int q = 0, m = T.lj_n[b]; q < m; q++) {
-                        const int j = T.lj[q * nb + b] - base;
-                        if (lj_keep(i, j, rf2)) ent[o++] = (uint16_t)j;
-                    }
+        return total;
+    };
+    // Copy the lists into their places, of the LJ entries only those
+    // lj_keep admits (requested four at a time); hcnt records the counts
+    // copied.
+    auto fill = [&](float rf2) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const int b = base + i;
+            int o = hoff[i];
+            const int hb = T.harm[b];
+            const int nlon = T.lon_n[b], nlat = T.lat_n[b];
+            if (hb >= 0) ent[o++] = (uint16_t)((((hb >> 1) - base) << 1) | (hb & 1));
+            for (int q = 0; q < nlon; q++) {
+                const int e = T.lon[q * nb + b];
+                ent[o++] = (uint16_t)((((e >> 1) - base) << 1) | (e & 1));
             }
+            for (int q = 0; q < nlat; q++) {
+                const int e = T.lat[q * nb + b];
+                ent[o++] = (uint16_t)((((e >> 1) - base) << 1) | (e & 1));
+            }
+            int nlj = 0;
+            if (P.lj_on) {
+                const int m = T.lj_n[b];
+                for (int q0 = 0; q0 < m; q0 += 4) {
+                    int jv[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) jv[u] = q0 + u < m ? T.lj[(q0 + u) * nb + b] - base : 0;
+#pragma unroll
+                    for (int u = 0; u < 4; u++)
+                        if (q0 + u < m && lj_keep(i, jv[u], rf2)) {
+                            ent[o++] = (uint16_t)jv[u];
+                            nlj++;
+                        }
+                }
+            }
+            hcnt[i] = pack_counts(hb >= 0, nlon, nlat, nlj);
         }
         if (threadIdx.x == 0) flags[0] = flags[1] = 0;
         __syncthreads();
-        return total;
     };
     // The full lists decide between LDS and global lists; in LDS, the LJ
     // lists are filtered by distance when the list cutoff reaches beyond
     // FILTER_R (the full lists fit, so a refill always fits too).
-    const bool use_lds = compact(0.f, false) <= entry_cap;
+    const bool use_lds = layout() <= entry_cap;
     if (use_lds) {
         const bool filter = P.lj_on && P.ljpairscutoff > FILTER_R;
-        compact(filter ? FILTER_R * FILTER_R : 0.f, true);
+        fill(filter ? FILTER_R * FILTER_R : 0.f);
         auto mk = [&](int i, int) { return LdsLists{ent, hoff[i], hcnt[i]}; };
-        auto refill = [&]() { compact(0.f, true); };
+        auto refill = [&]() { fill(0.f); };
         run_steps<PT, LdsLists>(P, T, A, t, lx, ln, l1, l2, mob, s0, s1, trig, noise, nsteps, L, mk, filter, flags,
                                 refill, rbuf, own_slot, help_slot, ib);
     } else {
