@@ -247,22 +247,95 @@ __global__ void __launch_bounds__(LIST_TILE)
     if (active) rebuild_finish(T, o, do_lj, do_bonded);
 }
 
+// Candidates of the bead at ri among the m beads of a trajectory held in LDS
+// in rebuild order (sm[pos].w: the bead's local index): every bead of a
+// group (inside a super-group) whose box comes within rc and whose centre
+// distance is below rc.  They come out in rebuild order, not by index, so
+// nothing is flushed during the scan: nc counts them all, the first `cap`
+// are kept in mine[slot * stride], and every later write lands in slot
+// `cap`, which the buffer holds beyond them.
+template <bool PERIODIC>
+__device__ __forceinline__ void scan_cands(const float4 *sm, const float4 *glo, const float4 *ghi, const float4 *slo,
+                                           const float4 *shi, int m, float3 ri, float box, float rc, float rc2,
+                                           uint16_t *mine, int stride, int cap, int &nc) {
+    constexpr int GPS = LIST_SUPER / LIST_GROUP;
+    const int ng = (m + LIST_GROUP - 1) / LIST_GROUP;
+    for (int sg = 0; sg * LIST_SUPER < m; sg++) {
+        if (!PERIODIC && !tile_within(ri, slo[sg], shi[sg], rc)) continue;
+        const int gend = min(ng, (sg + 1) * GPS);
+        for (int g = sg * GPS; g < gend; g++) {
+            if (!PERIODIC && !tile_within(ri, glo[g], ghi[g], rc)) continue;
+            const int g0 = g * LIST_GROUP;
+            auto test = [&](int jj) {
+                const float4 q = sm[jj];
+                float3 d = sub3(ri, f3(q));
+                if (PERIODIC) d = min_image(d, box);
+                mine[min(nc, cap) * stride] = (uint16_t)__float_as_int(q.w);
+                nc += (d.x * d.x + d.y * d.y + d.z * d.z < rc2) ? 1 : 0;
+            };
+            if (g0 + LIST_GROUP <= m) {
+#pragma unroll
+                for (int u = 0; u < LIST_GROUP; u++) test(g0 + u);
+            } else {
+                for (int jj = g0; jj < m; jj++) test(jj);
+            }
+        }
+    }
+}
+
+// The lane's nc <= TRAJ_CAND candidates in ascending order: a bitonic
+// network over SORT_W registers (padding sorts last), with no dependent LDS
+// accesses.
+constexpr int SORT_W = 32;
+static_assert(TRAJ_CAND <= SORT_W, "the sorting network covers the candidate buffer");
+__device__ __forceinline__ void sort_cands(uint16_t *mine, int stride, int nc) {
+    uint32_t v[SORT_W];
+#pragma unroll
+    for (int q = 0; q < SORT_W; q++) v[q] = (q < TRAJ_CAND && q < nc) ? mine[q * stride] : 0xffffu;
+#pragma unroll
+    for (int k = 2; k <= SORT_W; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < SORT_W; i++) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint32_t a = min(v[i], v[l]), b = max(v[i], v[l]);
+                    const bool up = (i & k) == 0;
+                    v[i] = up ? a : b;
+                    v[l] = up ? b : a;
+                }
+            }
+#pragma unroll
+    for (int q = 0; q < TRAJ_CAND; q++)
+        if (q < nc) mine[q * stride] = (uint16_t)v[q];
+}
+
 // The same rebuild with one workgroup per trajectory and the whole
-// trajectory in LDS (positions, rotated offsets, monomer types, group and
-// super-group boxes, a uint16 candidate buffer per lane): every bead's data
-// crosses from L2 once instead of once per 256-bead i tile.  Used when the
-// largest trajectory fits (rebuild_traj_plan).
+// trajectory in LDS (positions, rotated offsets, monomer types, a uint16
+// candidate buffer per lane), used when the largest trajectory fits
+// (rebuild_traj_plan).  Every bead's data crosses from L2 once instead of
+// once per 256-bead i tile, and the neighbour search runs over a second copy
+// of the positions in rebuild order, whose groups of 8 are compact patches
+// (index order runs along a protofilament, so its groups are 28 nm rods
+// that pass the box test for most of a wave's lanes).  Each lane sorts its
+// candidates by index before the full tests, so the lists come out in the
+// reference's order; a bead with more than TRAJ_CAND candidates tests every
+// bead of its trajectory in index order instead.
 __global__ void __launch_bounds__(TRAJ_THREADS)
     k_rebuild_traj(mte_params P, DevTopo T, DevState S, const int *order, int do_lj, int do_bonded) {
     extern __shared__ float4 lds[];
     const int t = blockIdx.x;
     const int base = T.tbase[t], n = T.tsize[t];
     const int ng = (n + LIST_GROUP - 1) / LIST_GROUP, nsg = (n + LIST_SUPER - 1) / LIST_SUPER;
-    float4 *sx = lds, *sn = sx + n, *s1 = sn + n, *s2 = s1 + n;
+    float4 *sx = lds, *sm = sx + n, *sn = sm + n, *s1 = sn + n, *s2 = s1 + n;
     float4 *glo = s2 + n, *ghi = glo + ng, *slo = ghi + ng, *shi = slo + nsg;
     int *smt = reinterpret_cast<int *>(shi + nsg);
-    uint16_t *cand = reinterpret_cast<uint16_t *>(smt + n);   // [slot][thread]
+    uint16_t *cand = reinterpret_cast<uint16_t *>(smt + n);   // [slot][thread], TRAJ_CAND + 1 slots
     for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const int j = order[base + q];
+        const float4 xj = S.xyz[base + j];
+        sm[q] = make_float4(xj.x, xj.y, xj.z, __int_as_float(j));
         sx[q] = S.xyz[base + q];
         sn[q] = S.fn[base + q];
         s1[q] = S.fp1[base + q];
@@ -270,31 +343,39 @@ __global__ void __launch_bounds__(TRAJ_THREADS)
         smt[q] = T.mon_type[base + q];
     }
     __syncthreads();
-    group_boxes(sx, n, glo, ghi, slo, shi);
+    group_boxes(sm, n, glo, ghi, slo, shi);
     const float box = T.tbox[t];
     const float rc = rebuild_radius(P.ljpairscutoff, do_lj, do_bonded);
     const float rc2 = rc * rc;
     uint16_t *mine = cand + threadIdx.x;
+    const int stride = (int)blockDim.x;
     for (int pos = threadIdx.x; pos < n; pos += blockDim.x) {
-        const int li = order[base + pos];
+        const int li = __float_as_int(sm[pos].w);
         const int b = base + li;
         RebuildOwn o = rebuild_own(sx[li], sn[li], s1[li], s2[li], smt[li], T.harm_raw[b], li, b, base, box);
         if (!T.extra[b]) {
             int nc = 0;
-            auto flush = [&]() {
-                for (int q = 0; q < nc; q++) {
-                    const int jj = mine[q * blockDim.x];
-                    rebuild_pair(T, o, jj, sx[jj], sn[jj], s1[jj], s2[jj], smt[jj], P.ljpairscutoff, do_lj, do_bonded);
-                }
-                nc = 0;
-            };
             if (box > 0.f)
-                scan_groups<true>(sx, glo, ghi, slo, shi, n, o.ri, box, rc, rc2, mine, (int)blockDim.x, TRAJ_CAND, nc,
-                                  flush);
+                scan_cands<true>(sm, glo, ghi, slo, shi, n, o.ri, box, rc, rc2, mine, stride, TRAJ_CAND, nc);
             else
-                scan_groups<false>(sx, glo, ghi, slo, shi, n, o.ri, box, rc, rc2, mine, (int)blockDim.x, TRAJ_CAND, nc,
-                                   flush);
-            flush();
+                scan_cands<false>(sm, glo, ghi, slo, shi, n, o.ri, box, rc, rc2, mine, stride, TRAJ_CAND, nc);
+            // the full tests; LJ-only rebuilds read no rotated offsets or types
+            auto pair = [&](int jj) {
+                if (do_bonded)
+                    rebuild_pair(T, o, jj, sx[jj], sn[jj], s1[jj], s2[jj], smt[jj], P.ljpairscutoff, do_lj, true);
+                else
+                    rebuild_pair(T, o, jj, sx[jj], float4{}, float4{}, float4{}, 0, P.ljpairscutoff, do_lj, false);
+            };
+            if (nc <= TRAJ_CAND) {
+                sort_cands(mine, stride, nc);
+                for (int q = 0; q < nc; q++) pair(mine[q * stride]);
+            } else {
+                for (int jj = 0; jj < n; jj++) {
+                    float3 d = sub3(o.ri, f3(sx[jj]));
+                    if (box > 0.f) d = min_image(d, box);
+                    if (d.x * d.x + d.y * d.y + d.z * d.z < rc2) pair(jj);
+                }
+            }
         }
         rebuild_finish(T, o, do_lj, do_bonded);
     }
@@ -306,8 +387,8 @@ bool rebuild_traj_plan(int nmax, int lds_limit, int *threads, size_t *lds) {
     if (nmax <= 0 || nmax > 65535) return false;
     const int th = std::min(TRAJ_THREADS, ((nmax + 63) / 64) * 64);
     const size_t ng = (nmax + LIST_GROUP - 1) / LIST_GROUP, nsg = (nmax + LIST_SUPER - 1) / LIST_SUPER;
-    const size_t bytes = sizeof(float4) * (4 * (size_t)nmax + 2 * ng + 2 * nsg) + sizeof(int) * (size_t)nmax +
-                         sizeof(uint16_t) * (size_t)TRAJ_CAND * th;
+    const size_t bytes = sizeof(float4) * (5 * (size_t)nmax + 2 * ng + 2 * nsg) + sizeof(int) * (size_t)nmax +
+                         sizeof(uint16_t) * (size_t)(TRAJ_CAND + 1) * th;
     if (bytes > (size_t)lds_limit) return false;
     *threads = th;
     *lds = (bytes + 15) & ~(size_t)15;

@@ -50,6 +50,8 @@ constexpr float SKIN_HALF = 0.75f;
 constexpr float FILTER_R = LJ_CUT + 2.f * SKIN_HALF + 0.01f;
 // Minimum room for list entries per bead when sizing the plan.
 constexpr int MIN_ENTRIES_PER_BEAD = 24;
+// List entries a thread requests at once when it fills its bead's place.
+constexpr int FILL_BATCH = 8;
 
 // Packed counts: bit 0 harmonic present, bits 1-4 longitudinal, 5-9 lateral,
 // 10-19 LJ.
@@ -252,14 +254,22 @@ __global__ void __launch_bounds__(MAXT)
         const float3 d = min_image(sub3(f3(lx[i]), f3(lx[j])), box);
         return d.x * d.x + d.y * d.y + d.z * d.z < rf2;
     };
-    // Offsets of this trajectory's full lists in LDS: the per-bead totals go
+    // Offsets of this trajectory's full lists in LDS, for the mobile beads
+    // (the only ones whose forces are computed): the per-bead totals go
     // through a block scan into hoff.  Returns the number of entries.  Every
     // later fill writes at these offsets (a filtered list uses a prefix of
     // its bead's place), so the layout is computed once per launch.
     auto layout = [&]() {
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            const int b = base + i;
-            hoff[i] = (T.harm[b] >= 0 ? 1 : 0) + T.lon_n[b] + T.lat_n[b] + (P.lj_on ? T.lj_n[b] : 0);
+        for (int i = threadIdx.x; i < n; i += blockDim.x) hoff[i] = 0;   // immobile beads use no lists
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            if (!mob[k]) continue;
+            const int i = ib[k], b = base + i;
+            const int h = T.harm[b] >= 0 ? 1 : 0, nlon = T.lon_n[b], nlat = T.lat_n[b];
+            const int nlj = P.lj_on ? T.lj_n[b] : 0;
+            hcnt[i] = pack_counts(h, nlon, nlat, nlj);
+            hoff[i] = h + nlon + nlat + nlj;
         }
         // Chunked exclusive scan: thread t owns beads [t*c, (t+1)*c); a wave
         // scans its lanes' chunk totals by shuffles, the first wave the
@@ -299,39 +309,45 @@ __global__ void __launch_bounds__(MAXT)
         return total;
     };
     // Copy the lists into their places, of the LJ entries only those
-    // lj_keep admits (requested four at a time); hcnt records the counts
-    // copied.
+    // lj_keep admits; hcnt's LJ field records the number copied.  A bead's
+    // entries (harmonic, longitudinal, lateral, LJ: the full counts of the
+    // layout) are requested FILL_BATCH at a time, each from its own array.
+    // Each thread fills the places of its own mobile beads.
+    int nent = 0;   // entries of the full lists (layout)
     auto fill = [&](float rf2) {
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            const int b = base + i;
-            int o = hoff[i];
-            const int hb = T.harm[b];
-            const int nlon = T.lon_n[b], nlat = T.lat_n[b];
-            if (hb >= 0) ent[o++] = (uint16_t)((((hb >> 1) - base) << 1) | (hb & 1));
-            for (int q = 0; q < nlon; q++) {
-                const int e = T.lon[q * nb + b];
-                ent[o++] = (uint16_t)((((e >> 1) - base) << 1) | (e & 1));
-            }
-            for (int q = 0; q < nlat; q++) {
-                const int e = T.lat[q * nb + b];
-                ent[o++] = (uint16_t)((((e >> 1) - base) << 1) | (e & 1));
-            }
-            int nlj = 0;
-            if (P.lj_on) {
-                const int m = T.lj_n[b];
-                for (int q0 = 0; q0 < m; q0 += 4) {
-                    int jv[4];
 #pragma unroll
-                    for (int u = 0; u < 4; u++) jv[u] = q0 + u < m ? T.lj[(q0 + u) * nb + b] - base : 0;
+        for (int kk = 0; kk < PT; kk++) {
+            if (!mob[kk]) continue;
+            const int i = ib[kk], b = base + i;
+            const int hc = hcnt[i];
+            const int h = hc & 1, nlon = (hc >> 1) & 15, nlat = (hc >> 5) & 31;
+            const int nbond = h + nlon + nlat;
+            const int o0 = hoff[i];
+            const int nall = (i + 1 < n ? hoff[i + 1] : nent) - o0;
+            int o = o0, nlj = 0;
+            for (int k0 = 0; k0 < nall; k0 += FILL_BATCH) {
+                int v[FILL_BATCH];
 #pragma unroll
-                    for (int u = 0; u < 4; u++)
-                        if (q0 + u < m && lj_keep(i, jv[u], rf2)) {
-                            ent[o++] = (uint16_t)jv[u];
-                            nlj++;
-                        }
+                for (int u = 0; u < FILL_BATCH; u++) {
+                    const int k = k0 + u;
+                    const int *src = k < h          ? T.harm
+                                     : k < h + nlon ? T.lon + (size_t)(k - h) * nb
+                                     : k < nbond    ? T.lat + (size_t)(k - h - nlon) * nb
+                                                    : T.lj + (size_t)(k - nbond) * nb;
+                    v[u] = k < nall ? src[b] : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < FILL_BATCH; u++) {
+                    const int k = k0 + u;
+                    if (k < nbond) {
+                        ent[o++] = (uint16_t)((((v[u] >> 1) - base) << 1) | (v[u] & 1));
+                    } else if (k < nall && lj_keep(i, v[u] - base, rf2)) {
+                        ent[o++] = (uint16_t)(v[u] - base);
+                        nlj++;
+                    }
                 }
             }
-            hcnt[i] = pack_counts(hb >= 0, nlon, nlat, nlj);
+            hcnt[i] = (hc & 1023) | (nlj << 10);
         }
         if (threadIdx.x == 0) flags[0] = flags[1] = 0;
         __syncthreads();
@@ -339,7 +355,8 @@ __global__ void __launch_bounds__(MAXT)
     // The full lists decide between LDS and global lists; in LDS, the LJ
     // lists are filtered by distance when the list cutoff reaches beyond
     // FILTER_R (the full lists fit, so a refill always fits too).
-    const bool use_lds = layout() <= entry_cap;
+    nent = layout();
+    const bool use_lds = nent <= entry_cap;
     if (use_lds) {
         const bool filter = P.lj_on && P.ljpairscutoff > FILTER_R;
         fill(filter ? FILTER_R * FILTER_R : 0.f);

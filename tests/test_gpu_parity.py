@@ -31,7 +31,10 @@ def _lists_equal(eng, orc):
         np.testing.assert_array_equal(L["lon"][m, k], b.lon[m, k])
     for k in range(16):
         m = b.lat_count > k
-        np.testing.assert_array_equal(L["lat"][m, k], b.lat[m, k])
+        # a static (PDB) lateral entry 0 is bead 0 in the j <= 0 orientation
+        # (src/compute_cuda.cu:306-308), which the engine reports as -ZERO
+        want = np.where(b.lat[m, k] == 0, -999999, b.lat[m, k])
+        np.testing.assert_array_equal(L["lat"][m, k], want)
     for k in range(orc.max_lj):
         m = orc.lj_count > k
         np.testing.assert_array_equal(L["lj"][m, k], orc.lj[m, k])
@@ -388,6 +391,33 @@ def test_lists_tiled_rebuild_for_trajectories_beyond_lds():
     b = topology.make_batch(atoms, [lattice.coords7(a, 0.1, rng) for a in atoms])
     eng = Engine(p, b.copy(), mode=MODE_STREAMED, list_method=LISTS_ALLPAIRS)
     orc = Oracle(p, b.copy())
+    eng.rebuild()
+    assert orc.rebuild() == 0
+    _lists_equal(eng, orc)
+    eng.close()
+
+
+@pytest.mark.parametrize("assembly", [False, True])
+def test_lists_dense_cluster_beyond_candidate_buffer(assembly):
+    """120 free beads in a 20 nm cube, next to a 312-bead tube: a bead in the
+    cube's interior has about 30 others within the 8 nm list radius, more
+    than the 24 candidates a lane of the one-workgroup-per-trajectory rebuild
+    keeps, and takes its index-order scan; edge beads keep theirs and sort
+    them.  Both give the oracle's lists, entry for entry (bonded lists too in
+    assembly mode)."""
+    print(f"case dense cluster lists, assembly {assembly}", flush=True)
+    from mt_amd import lattice, topology
+    p = base_params(is_assembly=assembly)
+    rng = np.random.default_rng(11)
+    # chain B: free tubulin of the system, not reservoir beads (chain X), which get no lists
+    atoms = [lattice.free_dimers(60, 20.0, rng, min_sep=2.0, chain="B"), lattice.microtubule(12)]
+    b = topology.make_batch(atoms, [lattice.coords7(a, 0.1, rng) for a in atoms], is_assembly=assembly)
+    assert not b.extra.any()
+    xyz = b.coords[:120, :3]
+    within = ((xyz[:, None, :] - xyz[None, :, :]) ** 2).sum(-1) < 8.0 ** 2
+    assert (within.sum(1) - 1 > 24).sum() > 20   # the fallback is exercised
+    eng = Engine(p, b.copy(), max_lj=64, mode=MODE_STREAMED)
+    orc = Oracle(p, b.copy(), max_lj=64)
     eng.rebuild()
     assert orc.rebuild() == 0
     _lists_equal(eng, orc)
