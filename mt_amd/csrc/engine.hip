@@ -301,17 +301,24 @@ static void time_end(mte_engine *e, const char *name, const EvPair &ev) {
     e->timers[name].pending.emplace_back(ev.start, ev.stop);
 }
 
+// Reads every pending pair; an event may end one pair and start the next
+// (a rebuild and the resident launch after it), so each returns to the pool
+// once.
 static void flush_timers(mte_engine *e) {
+    std::vector<hipEvent_t> done;
     for (auto &kv : e->timers) {
         for (auto &pr : kv.second.pending) {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) kv.second.total_ms += ms;
             kv.second.launches++;
-            e->ev_pool.push_back(pr.first);
-            e->ev_pool.push_back(pr.second);
+            done.push_back(pr.first);
+            done.push_back(pr.second);
         }
         kv.second.pending.clear();
     }
+    std::sort(done.begin(), done.end());
+    done.erase(std::unique(done.begin(), done.end()), done.end());
+    e->ev_pool.insert(e->ev_pool.end(), done.begin(), done.end());
 }
 
 extern "C" {
@@ -579,6 +586,7 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
         if (dalloc(e, &M.w, MLP_NPARAM) != hipSuccess || dalloc(e, &M.wi, MLP_NPARAM) != hipSuccess ||
             dalloc(e, &M.grad, MLP_NPARAM) != hipSuccess || dalloc(e, &M.m1, MLP_NPARAM) != hipSuccess ||
             dalloc(e, &M.m2, MLP_NPARAM) != hipSuccess || dalloc(e, &M.feat, (size_t)MLP_IN * nb) != hipSuccess ||
+            dalloc(e, &M.fblk, nb) != hipSuccess ||
             dalloc(e, &M.targ, (size_t)MLP_OUT * nb) != hipSuccess ||
             dalloc(e, &M.ystep, (size_t)MLP_OUT * nb) != hipSuccess || dalloc(e, &M.err, 2 * (size_t)nt) != hipSuccess ||
             dalloc(e, &M.nbad, 1) != hipSuccess || dalloc(e, &M.img, mlp_image_bytes() / sizeof(uint4)) != hipSuccess) {
@@ -590,6 +598,25 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
             return bail(MTE_ERR_HIP);
         }
         e->mlp_last_err.assign(nt, 0.0);
+        // Neighbour-search workgroups round robin over the trajectories: the
+        // first blocks of every trajectory (a tube's crowded beads, which
+        // come first in the usual input, run longest) are dispatched first,
+        // and with a multiple of 8 trajectories workgroup k keeps to the
+        // XCD of trajectory k mod 8.
+        std::vector<int2> bl;
+        int most = 0;
+        for (int t = 0; t < nt; t++) most = std::max(most, (tsize[t] + MLP_SEARCH_BLOCK - 1) / MLP_SEARCH_BLOCK);
+        for (int k = 0; k < most; k++)
+            for (int t = 0; t < nt; t++)
+                if (k * MLP_SEARCH_BLOCK < tsize[t]) bl.push_back(make_int2(t, k));
+        M.nblocks = (int)bl.size();
+        if (dalloc(e, &M.blocks, bl.size()) != hipSuccess ||
+            hipMemcpyAsync(M.blocks, bl.data(), sizeof(int2) * bl.size(), hipMemcpyHostToDevice, e->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(e->stream) != hipSuccess) {
+            e->err = "mte_create: upload failed for the neighbour-search blocks";
+            return bail(MTE_ERR_HIP);
+        }
     }
     {   // list rebuilds with one workgroup per trajectory when the largest one fits in LDS
         int nmax = 0, lds_limit = 0;
@@ -760,11 +787,14 @@ static int check_overflow(mte_engine *e) {
     return eval_overflow(e);
 }
 
-static int rebuild_now(mte_engine *e) {
+// `end`, when given, receives the rebuild's stop event (null while timing is
+// off), for a launch that follows at once to start its timing from.
+static int rebuild_now(mte_engine *e, hipEvent_t *end = nullptr) {
     EvPair ev;
     time_begin(e, "rebuild", &ev);
     HIPCHK(launch_rebuild(e));
     time_end(e, "rebuild", ev);
+    if (end) *end = ev.stop;
     e->lists_step = e->step;
     e->lists_valid = true;
     return MTE_OK;
@@ -817,7 +847,7 @@ static hipError_t enqueue_streamed(mte_engine *e, int64_t s0, int64_t len, int64
             if (st == hipSuccess) st = launch_mlp_cells(e);
             if (st != hipSuccess) break;
             time_begin(e, "mlp_local", &ev);
-            st = launch_mlp_local(e, e->mlp.feat, check ? e->mlp.targ : nullptr, false);
+            st = launch_mlp_local(e, e->mlp.feat, check ? e->mlp.targ : nullptr, false, e->mlp.fblk);
             time_end(e, "mlp_local", ev);
             if (st != hipSuccess) break;
             time_begin(e, "mlp_step", &ev);
@@ -891,13 +921,19 @@ int mte_run(mte_engine *e, int64_t nsteps) {
             const int64_t to_boundary = freq - (e->step % freq);
             const int64_t chunk = std::min<int64_t>(nsteps - done, to_boundary);
             // lists due at this boundary are rebuilt by the streamed kernel first (same stream)
+            hipEvent_t mark = nullptr;   // the rebuild's stop marker, which also starts the step's timing
             if (need_lists && rebuild_due(e)) {
-                const int rc = rebuild_now(e);
+                const int rc = rebuild_now(e, &mark);
                 if (rc) return rc;
             }
             const float *nz = e->noise ? e->noise + (size_t)6 * e->nb * e->noise_cursor : nullptr;
             EvPair ev;
-            time_begin(e, "step", &ev);
+            if (mark) {
+                ev.start = mark;
+                ev.stop = take_event(e);
+            } else {
+                time_begin(e, "step", &ev);
+            }
             HIPCHK(launch_resident(e, (int)chunk, nz));
             time_end(e, "step", ev);
             if (e->noise) e->noise_cursor += chunk;

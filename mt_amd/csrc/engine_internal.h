@@ -81,6 +81,7 @@ constexpr int CELL_SCAN_SPAN = 4096;
 
 // MLP-HI network and work buffers (mlp.hip).
 constexpr int MLP_NN = 32, MLP_IN = 96, MLP_H = 128, MLP_OUT = 9;
+constexpr int MLP_SEARCH_BLOCK = 64;   // beads per workgroup of the neighbour search
 constexpr int MLP_NPARAM = MLP_IN * MLP_H + MLP_H + 2 * (MLP_H * MLP_H + MLP_H) + MLP_H * MLP_OUT + MLP_OUT;
 // mte_mlp_forward_noalloc: rows per call (the fixed staging array's size)
 // and the device ordinals whose array address is kept.
@@ -91,6 +92,9 @@ struct MlpBuffers {
     uint4 *img = nullptr;                     // inference image: bf16 MFMA fragments + biases (mlp_image_bytes())
     float *grad = nullptr, *m1 = nullptr, *m2 = nullptr;
     float *feat = nullptr, *targ = nullptr, *ystep = nullptr;   // per bead
+    uint8_t *fblk = nullptr;   // per bead: 24-float blocks of `feat` the step's search wrote (the rest are zeros)
+    int2 *blocks = nullptr;    // neighbour-search workgroups: (trajectory, 64-bead block), round robin over trajectories
+    int nblocks = 0;
     double *err = nullptr;                    // per trajectory: squared error, squared label norm
     int *nbad = nullptr;                      // beads whose prediction failed the Cholesky check
     // training work space for up to `cap` samples
@@ -195,7 +199,10 @@ hipError_t launch_rebuild_cells(mte_engine *e);
 // The 12 nm cell tables of the MLP-HI feature search, then the search
 // itself (which builds the tables first unless build_tables is false).
 hipError_t launch_mlp_cells(mte_engine *e);
-hipError_t launch_mlp_local(mte_engine *e, float *feat, float *targ, bool build_tables = true);
+// fblk null: all 96 features of every bead are written; else only the
+// 24-float blocks holding neighbours, their number in fblk.
+hipError_t launch_mlp_local(mte_engine *e, float *feat, float *targ, bool build_tables = true,
+                            uint8_t *fblk = nullptr);
 hipError_t launch_mlp_forward(mte_engine *e, const float *feat, int n, float *out);
 // Address of the device's fixed staging array for mte_mlp_forward_noalloc
 // (hipGetSymbolAddress once per device; call with `device` current).

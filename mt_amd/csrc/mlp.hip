@@ -23,7 +23,7 @@
 
 namespace mte {
 
-constexpr int MLP_BLOCK = 64;
+constexpr int MLP_BLOCK = MLP_SEARCH_BLOCK;
 constexpr int ROW = MLP_H + 1;   // LDS row pitch (odd: no bank conflicts between lanes)
 constexpr float MLP_RC = 12.f, MLP_RIN = 8.f;
 // flat parameter offsets
@@ -40,6 +40,33 @@ __device__ __forceinline__ float taper(float r) {
 }
 
 // ------------------------------------------------------------ features, labels
+constexpr int FEAT_BLK_NN = 8;   // neighbours per block of written features (24 floats: whole
+                                 // float4 stores and whole 8-float MFMA operand slices)
+static_assert(3 * FEAT_BLK_NN % 8 == 0 && FEAT_BLK_NN % 4 == 0, "feature blocks");
+// The 27 bucket ranges of a search as [first, end) slots of the bucket-ordered
+// positions.  A trajectory's beads hold the slots [base, base + n) of its
+// buckets, so with trajectory-local indices of 16 bits (n < 65536) both ends
+// pack into one register.
+template <typename JT>
+struct BucketRanges {
+    int2 v[27];
+    __device__ __forceinline__ void set(int o, int2 r, int) { v[o] = make_int2(r.x, r.x + r.y); }
+    __device__ __forceinline__ void clear(int o) { v[o] = make_int2(0, 0); }
+    __device__ __forceinline__ int lo(int o, int) const { return v[o].x; }
+    __device__ __forceinline__ int hi(int o, int) const { return v[o].y; }
+};
+template <>
+struct BucketRanges<uint16_t> {
+    uint32_t v[27];
+    __device__ __forceinline__ void set(int o, int2 r, int base) {
+        const uint32_t s = (uint32_t)(r.x - base);
+        v[o] = s | ((s + (uint32_t)r.y) << 16);
+    }
+    __device__ __forceinline__ void clear(int o) { v[o] = 0; }
+    __device__ __forceinline__ int lo(int o, int base) const { return base + (int)(v[o] & 0xffffu); }
+    __device__ __forceinline__ int hi(int o, int base) const { return base + (int)(v[o] >> 16); }
+};
+
 // Neighbour scan over the 27 cells of the hashed table (built with cell side
 // >= 12 nm, or 3 cells per side covering the whole box).  The 27 bucket
 // ranges are loaded up front; a bucket that two of the cells hash to is
@@ -50,19 +77,26 @@ __device__ __forceinline__ float taper(float r) {
 // nearest (distance, then index) and sums the exact label over every
 // neighbour within 12 nm (LABELS: only on check steps and for training data).
 // The 32-nearest lists live in LDS with trajectory-local indices of type JT:
-// 16-bit where every trajectory has at most 65536 beads (12 KB per wave
+// 16-bit where every trajectory has fewer than 65536 beads (12 KB per wave
 // instead of 16, so more waves per CU hide the candidates' load latency).
+// With fblk, a bead's features are written only up to the 24-float block
+// (8 neighbours) holding its last neighbour, and the number of blocks goes
+// to fblk: the step's network reads the rest as zeros, so in a dilute box
+// (a few neighbours within 12 nm) most of the 384 feature bytes per bead
+// never cross HBM.
 template <bool LABELS, typename JT>
 __global__ void __launch_bounds__(MLP_BLOCK)
-    k_mlp_local(mte_params P, DevTopo T, DevState S, CellTables C, float *feat, float *targ) {
+    k_mlp_local(mte_params P, DevTopo T, DevState S, CellTables C, const int2 *blocks, float *feat, float *targ,
+                uint8_t *fblk) {
     __shared__ float sd[MLP_BLOCK * MLP_NN];
     __shared__ JT sj[MLP_BLOCK * MLP_NN];
-    const int b = blockIdx.x * MLP_BLOCK + threadIdx.x;
-    if (b >= T.nb) return;
+    const int2 blk = blocks[blockIdx.x];
+    const int t = blk.x;
+    const int base = T.tbase[t];
+    const int b = base + blk.y * MLP_BLOCK + threadIdx.x;
+    if (b >= base + T.tsize[t]) return;
     float *nd = sd + threadIdx.x * MLP_NN;
     JT *nj = sj + threadIdx.x * MLP_NN;
-    const int t = T.tid[b];
-    const int base = T.tbase[t];
     const float box = T.tbox[t];
     const float3 ri = f3(S.xyz[b]);
     float A[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -90,9 +124,9 @@ __global__ void __launch_bounds__(MLP_BLOCK)
             const uint32_t h = ((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u);
             bk[o] = toff + (int)(h & mask);
         }
-        int2 rg[27];
+        BucketRanges<JT> rg;
 #pragma unroll
-        for (int o = 0; o < 27; o++) rg[o] = C.range[bk[o]];
+        for (int o = 0; o < 27; o++) rg.set(o, C.range[bk[o]], base);
         // a bucket two cells hash to is visited once: a 64-bit filter on the
         // low bucket bits, and the full comparison only when it hits
         uint64_t seen = 1ull << (bk[0] & 63);
@@ -103,67 +137,95 @@ __global__ void __launch_bounds__(MLP_BLOCK)
                 bool dup = false;
 #pragma unroll
                 for (int k = 0; k < o; k++) dup |= bk[k] == bk[o];
-                if (dup) rg[o].y = 0;
+                if (dup) rg.clear(o);
             }
             seen |= bit;
         }
-#pragma unroll
-        for (int c = 0; c < 27; c++) {
-            // each candidate is one 16-byte load, issued one candidate ahead
-            const int q0 = rg[c].x, q1 = q0 + rg[c].y;
-            float4 pn = q0 < q1 ? C.spos[q0] : make_float4(0.f, 0.f, 0.f, 0.f);
-            float4 pm = q0 + 1 < q1 ? C.spos[q0 + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
-            for (int q = q0; q < q1; q++) {
-                const float4 pj = pn;
-                pn = pm;
-                if (q + 2 < q1) pm = C.spos[q + 2];
-                const int jw = __float_as_int(pj.w);
-                const int j = jw & 0x7fffffff;
-                const float3 d = min_image(make_float3(pj.x - ri.x, pj.y - ri.y, pj.z - ri.z), box);
-                const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
-                // extra beads, this bead and beads beyond the radius drop out
-                if (jw < 0 || j == b || r2 >= MLP_RC * MLP_RC) continue;
-                if (LABELS) {
-                    // exact label: phi(r) D_hat(d)
-                    const float r = sqrtf(r2), rinv = 1.f / r;
-                    const float ra = r * ia;
-                    float cii, crr;
-                    if (ra > 2.f) {
-                        const float u = 1.f / ra, u2 = u * u;
-                        cii = 0.75f * u * (1.f + (2.f / 3.f) * u2);
-                        crr = 0.75f * u * (1.f - 2.f * u2);
-                    } else {
-                        cii = 1.f - 9.f * ra / 32.f;
-                        crr = 3.f * ra / 32.f;
-                    }
-                    const float w = taper(r);
-                    const float e[3] = {d.x * rinv, d.y * rinv, d.z * rinv};
-#pragma unroll
-                    for (int x = 0; x < 3; x++)
-#pragma unroll
-                        for (int y = 0; y < 3; y++) A[3 * x + y] += w * (crr * e[x] * e[y] + (x == y ? cii : 0.f));
+        auto visit = [&](const float4 pj) {
+            const int jw = __float_as_int(pj.w);
+            const int j = jw & 0x7fffffff;
+            const float3 d = min_image(make_float3(pj.x - ri.x, pj.y - ri.y, pj.z - ri.z), box);
+            const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+            // extra beads, this bead and beads beyond the radius drop out
+            if (jw < 0 || j == b || r2 >= MLP_RC * MLP_RC) return;
+            if (LABELS) {
+                // exact label: phi(r) D_hat(d)
+                const float r = sqrtf(r2), rinv = 1.f / r;
+                const float ra = r * ia;
+                float cii, crr;
+                if (ra > 2.f) {
+                    const float u = 1.f / ra, u2 = u * u;
+                    cii = 0.75f * u * (1.f + (2.f / 3.f) * u2);
+                    crr = 0.75f * u * (1.f - 2.f * u2);
+                } else {
+                    cii = 1.f - 9.f * ra / 32.f;
+                    crr = 3.f * ra / 32.f;
                 }
-                // 32 nearest, ordered by (r2, j)
-                const int jl = j - base;
-                if (cnt < MLP_NN || r2 < nd[cnt - 1] || (r2 == nd[cnt - 1] && jl < (int)nj[cnt - 1])) {
-                    int k = cnt < MLP_NN ? cnt++ : MLP_NN - 1;
-                    while (k > 0 && (nd[k - 1] > r2 || (nd[k - 1] == r2 && (int)nj[k - 1] > jl))) {
-                        nd[k] = nd[k - 1];
-                        nj[k] = nj[k - 1];
-                        k--;
-                    }
-                    nd[k] = r2;
-                    nj[k] = (JT)jl;
+                const float w = taper(r);
+                const float e[3] = {d.x * rinv, d.y * rinv, d.z * rinv};
+#pragma unroll
+                for (int x = 0; x < 3; x++)
+#pragma unroll
+                    for (int y = 0; y < 3; y++) A[3 * x + y] += w * (crr * e[x] * e[y] + (x == y ? cii : 0.f));
+            }
+            // 32 nearest, ordered by (r2, j)
+            const int jl = j - base;
+            if (cnt < MLP_NN || r2 < nd[cnt - 1] || (r2 == nd[cnt - 1] && jl < (int)nj[cnt - 1])) {
+                int k = cnt < MLP_NN ? cnt++ : MLP_NN - 1;
+                while (k > 0 && (nd[k - 1] > r2 || (nd[k - 1] == r2 && (int)nj[k - 1] > jl))) {
+                    nd[k] = nd[k - 1];
+                    nj[k] = nj[k - 1];
+                    k--;
+                }
+                nd[k] = r2;
+                nj[k] = (JT)jl;
+            }
+        };
+        // Buckets in order, their candidates in bucket order.  Most buckets
+        // hold no bead or one (a few per cell in a dilute box): the first
+        // candidate of each bucket is requested SCAN_CH buckets at a time, the
+        // next group's while this group is visited, so their loads overlap;
+        // a bucket's further candidates follow one load ahead.
+        constexpr int SCAN_CH = 9;
+        auto first = [&](int c0, float4 *p) {
+#pragma unroll
+            for (int u = 0; u < SCAN_CH; u++) {
+                const int q0 = rg.lo(c0 + u, base), q1 = rg.hi(c0 + u, base);
+                p[u] = q0 < q1 ? C.spos[q0] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        };
+        auto group = [&](int c0, const float4 *p) {
+#pragma unroll
+            for (int u = 0; u < SCAN_CH; u++) {
+                const int q0 = rg.lo(c0 + u, base), q1 = rg.hi(c0 + u, base);
+                if (q0 >= q1) continue;
+                visit(p[u]);
+                float4 pn = q0 + 1 < q1 ? C.spos[q0 + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int q = q0 + 1; q < q1; q++) {
+                    const float4 pj = pn;
+                    if (q + 1 < q1) pn = C.spos[q + 1];
+                    visit(pj);
                 }
             }
-        }
+        };
+        static_assert(27 == 3 * SCAN_CH, "three bucket groups");
+        float4 pa[SCAN_CH], pb[SCAN_CH];
+        first(0, pa);
+        first(SCAN_CH, pb);
+        group(0, pa);
+        first(2 * SCAN_CH, pa);
+        group(SCAN_CH, pb);
+        group(2 * SCAN_CH, pa);
     }
     if (feat) {
         // four neighbours (12 floats, three float4 stores) at a time, their
         // positions gathered together
         float4 *f = reinterpret_cast<float4 *>(feat + (size_t)b * MLP_IN);
+        const int nblk = (cnt + FEAT_BLK_NN - 1) / FEAT_BLK_NN;
+        if (fblk) fblk[b] = (uint8_t)nblk;
 #pragma unroll
         for (int k0 = 0; k0 < MLP_NN; k0 += 4) {
+            if (fblk && k0 >= FEAT_BLK_NN * nblk) break;
             float v[12];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
@@ -380,8 +442,10 @@ __device__ __forceinline__ void hidden_to_frags(f32x4 (&acc)[2 * KS][NET_GROUPS]
 
 // The network on beads b0 .. b0+NET_TILE-1 (features of beads >= n read as
 // zero); returns in lane t < NET_TILE the 9 outputs of bead b0 + t.
-__device__ __forceinline__ void net_tile(const u32x4 *frag, const float *bias, const float *__restrict__ feat, int n,
-                                         int b0, int lane, float y[MLP_OUT]) {
+// lim, byte grp: the features of the lane's bead in column group grp past
+// that count are zeros and are not read (feat_limits).
+__device__ __forceinline__ void net_tile(const u32x4 *frag, const float *bias, const float *__restrict__ feat,
+                                         uint32_t lim, int b0, int lane, float y[MLP_OUT]) {
     const int g = lane >> 4, col = lane & 15;
     // The fragments are the same for every tile: keep the compiler from
     // hoisting all of them (368 registers) out of the caller's tile loop.
@@ -393,7 +457,7 @@ __device__ __forceinline__ void net_tile(const u32x4 *frag, const float *bias, c
 #pragma unroll
         for (int ks = 0; ks < KS_IN; ks++) {
             float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            if (bead < n) {
+            if (32 * ks + 8 * g < (int)((lim >> (8 * grp)) & 0xffu)) {
                 const float4 *src = reinterpret_cast<const float4 *>(feat + (size_t)bead * MLP_IN + 32 * ks + 8 * g);
                 const float4 u = src[0], w = src[1];
                 v[0] = u.x; v[1] = u.y; v[2] = u.z; v[3] = u.w;
@@ -427,6 +491,21 @@ __device__ __forceinline__ void net_tile(const u32x4 *frag, const float *bias, c
     }
 }
 
+// Features present for the lane's beads of the tile at b0, one byte per
+// column group (net_tile's lim): all 96 without fblk, else fblk's blocks of
+// 24; none past the last bead.
+__device__ __forceinline__ uint32_t feat_limits(const uint8_t *__restrict__ fblk, int n, int b0, int lane) {
+    uint32_t lim = 0;
+#pragma unroll
+    for (int grp = 0; grp < NET_GROUPS; grp++) {
+        const int bead = b0 + 16 * grp + (lane & 15);
+        const int l = bead >= n ? 0 : fblk ? 3 * FEAT_BLK_NN * (int)fblk[bead] : MLP_IN;
+        lim |= (uint32_t)l << (8 * grp);
+    }
+    return lim;
+}
+static_assert(MLP_IN < 256 && NET_GROUPS <= 4, "feature limits packed in bytes");
+
 // Weight image into LDS (dynamic, NET_IMG_BYTES).
 __device__ __forceinline__ void load_image(const u32x4 *__restrict__ img, u32x4 *lds) {
     for (int i = threadIdx.x; i < (int)(NET_IMG_BYTES / 16); i += NET_THREADS) lds[i] = img[i];
@@ -442,7 +521,7 @@ __global__ void __launch_bounds__(NET_THREADS)
     const int ntiles = (n + NET_TILE - 1) / NET_TILE;
     for (int tile = blockIdx.x * NET_WAVES + wave; tile < ntiles; tile += gridDim.x * NET_WAVES) {
         float y[MLP_OUT];
-        net_tile(net_lds, bias, feat, n, tile * NET_TILE, lane, y);
+        net_tile(net_lds, bias, feat, feat_limits(nullptr, n, tile * NET_TILE, lane), tile * NET_TILE, lane, y);
         const int s = tile * NET_TILE + lane;
         if (lane < NET_TILE && s < n)
             for (int q = 0; q < MLP_OUT; q++) out[(size_t)s * MLP_OUT + q] = y[q];
@@ -568,7 +647,8 @@ __device__ __forceinline__ void mlp_integrate(const MlpStepArgs &A, int b, const
 // MLP-HI step: network on the matrix cores, then lanes 0..NET_TILE-1
 // integrate the tile's beads.
 __global__ void __launch_bounds__(NET_THREADS)
-    k_mlp_step(const u32x4 *__restrict__ img, const float *__restrict__ feat, MlpStepArgs A) {
+    k_mlp_step(const u32x4 *__restrict__ img, const float *__restrict__ feat, const uint8_t *__restrict__ fblk,
+               MlpStepArgs A) {
     extern __shared__ u32x4 net_lds[];
     load_image(img, net_lds);
     const float *bias = reinterpret_cast<const float *>(net_lds + FR_END);
@@ -576,12 +656,13 @@ __global__ void __launch_bounds__(NET_THREADS)
     const int n = A.T.nb;
     const int ntiles = (n + NET_TILE - 1) / NET_TILE;
     for (int tile = blockIdx.x * NET_WAVES + wave; tile < ntiles; tile += gridDim.x * NET_WAVES) {
+        const uint32_t lim = feat_limits(fblk, n, tile * NET_TILE, lane);
         const int b = tile * NET_TILE + lane;
         const bool integ = lane < NET_TILE && b < n;
         MlpIn in{};
         if (integ) in = mlp_load(A, b);
         float y[MLP_OUT];
-        net_tile(net_lds, bias, feat, n, tile * NET_TILE, lane, y);
+        net_tile(net_lds, bias, feat, lim, tile * NET_TILE, lane, y);
         if (integ) mlp_integrate(A, b, y, in);
     }
 }
@@ -724,25 +805,26 @@ static inline int blocks(int n, int bs) { return (n + bs - 1) / bs; }
 
 hipError_t launch_mlp_cells(mte_engine *e) { return build_cells(e, MLP_RC, 1); }
 
-hipError_t launch_mlp_local(mte_engine *e, float *feat, float *targ, bool build_tables) {
+hipError_t launch_mlp_local(mte_engine *e, float *feat, float *targ, bool build_tables, uint8_t *fblk) {
     if (build_tables) {
         const hipError_t s = launch_mlp_cells(e);
         if (s != hipSuccess) return s;
     }
-    const dim3 grid(blocks(e->nb, MLP_BLOCK)), block(MLP_BLOCK);
-    const bool narrow = e->nmax <= 65536;   // trajectory-local indices fit 16 bits
+    const dim3 grid(e->mlp.nblocks), block(MLP_BLOCK);
+    const int2 *bl = e->mlp.blocks;
+    const bool narrow = e->nmax < 65536;   // trajectory-local indices and bucket ends fit 16 bits
     if (targ && narrow)
         hipLaunchKernelGGL((k_mlp_local<true, uint16_t>), grid, block, 0, e->stream, e->P, e->T, e->S[e->cur],
-                           e->cells, feat, targ);
+                           e->cells, bl, feat, targ, fblk);
     else if (targ)
         hipLaunchKernelGGL((k_mlp_local<true, int>), grid, block, 0, e->stream, e->P, e->T, e->S[e->cur], e->cells,
-                           feat, targ);
+                           bl, feat, targ, fblk);
     else if (narrow)
         hipLaunchKernelGGL((k_mlp_local<false, uint16_t>), grid, block, 0, e->stream, e->P, e->T, e->S[e->cur],
-                           e->cells, feat, targ);
+                           e->cells, bl, feat, targ, fblk);
     else
         hipLaunchKernelGGL((k_mlp_local<false, int>), grid, block, 0, e->stream, e->P, e->T, e->S[e->cur], e->cells,
-                           feat, targ);
+                           bl, feat, targ, fblk);
     return hipGetLastError();
 }
 
@@ -771,7 +853,7 @@ hipError_t launch_mlp_step(mte_engine *e, const float *noise_step, float *yout) 
     MlpStepArgs A{e->P, e->T, e->S[e->cur], e->S[1 - e->cur], e->fxyz, e->fang, e->tea.ci, e->tea.beta,
                   e->seeds, noise_step, yout, e->mlp.nbad, e->L};
     hipLaunchKernelGGL(k_mlp_step, dim3(net_blocks(e, e->nb)), dim3(NET_THREADS), NET_IMG_BYTES, e->stream,
-                       reinterpret_cast<const u32x4 *>(e->mlp.img), e->mlp.feat, A);
+                       reinterpret_cast<const u32x4 *>(e->mlp.img), e->mlp.feat, e->mlp.fblk, A);
     e->cur = 1 - e->cur;
     return hipGetLastError();
 }
