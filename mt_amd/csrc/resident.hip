@@ -5,8 +5,9 @@
 // the three rotated offsets with the Euler angles packed into their spare
 // lanes (64 B), plus the bead's neighbour lists compacted into a uint16 CSR
 // (local index << 1 | flag) whose places are sized for the full lists, so a
-// distance-filtered list and its later refill share one layout.  Each thread owns PT beads and keeps their two
-// HybridTaus states in registers.  A launch runs every step up to the next
+// distance-filtered list and its later refill share one layout.  Each
+// thread owns PT beads and keeps their two HybridTaus states in registers.
+// A launch runs every step up to the next
 // list-rebuild boundary (LJPairsUpdateFreq steps; 100 in the batched
 // workload), after the rebuild kernel of lists.hip has refreshed the lists
 // when one is due, so HBM sees the state once per launch instead of twice
@@ -95,26 +96,27 @@ __device__ __forceinline__ float4 store_state(float4 *lx, float4 *ln, float4 *l1
     return make_float4(R.st, R.ct, R.sp, R.cp);
 }
 
-// The step loop.  With `filtered` the LJ lists in LDS hold only the pairs
-// within FILTER_R at the first step: each mobile bead's displacement from
-// its first-step position is checked every step, and once any bead of the
-// trajectory has moved SKIN_HALF or more, `refill` restores the full lists
-// before the next force evaluation.  Dropped pairs contribute exactly zero
-// force until then, so the sums are those over the full lists.
-template <int PT, class Lists, class MakeLists, class Refill>
-__device__ __forceinline__ void run_steps(const mte_params &P, const DevTopo &T, const LdsAcc &A, int t, float4 *lx,
-                                          float4 *ln, float4 *l1, float4 *l2, const bool mob[PT], uint4 s0[PT],
-                                          uint4 s1[PT], float4 trig[PT], const float *noise, int nsteps, const LatOffsets &L,
-                                          const MakeLists &make_lists, bool filtered, int *flags,
-                                          const Refill &refill, float4 *rbuf, int own_slot, int help_slot,
-                                          const int ib[PT]) {
+// The step loop over steps [first, nsteps).  FILTERED: the LJ lists in LDS
+// hold only the pairs within FILTER_R at step `first`; each mobile bead's
+// displacement from its position then is checked every step, and once any
+// bead of the trajectory has moved SKIN_HALF or more the loop returns the
+// next step, for the caller to restore the full lists before that step's
+// force evaluation (dropped pairs contribute exactly zero force until then,
+// so the sums are those over the full lists) and go on unfiltered.
+// Returns nsteps when every step ran.
+template <int PT, bool FILTERED, class Lists, class MakeLists>
+__device__ __forceinline__ int run_steps(const mte_params &P, const DevTopo &T, const LdsAcc &A, int t, float4 *lx,
+                                         float4 *ln, float4 *l1, float4 *l2, const bool mob[PT], uint4 s0[PT],
+                                         uint4 s1[PT], float4 trig[PT], const float *noise, int first, int nsteps,
+                                         const LatOffsets &L, const MakeLists &make_lists, int *flags, float4 *rbuf,
+                                         int own_slot, int help_slot, const int ib[PT]) {
     const int base = T.tbase[t], nb = T.nb;
     const float box = T.tbox[t];
     const float wall_z = P.is_wall ? T.zs[t] : 0.f;
     float3 xr[PT];
 #pragma unroll
-    for (int k = 0; k < PT; k++) xr[k] = mob[k] ? f3(lx[ib[k]]) : make_float3(0.f, 0.f, 0.f);
-    for (int s = 0; s < nsteps; s++) {
+    for (int k = 0; k < PT; k++) xr[k] = FILTERED && mob[k] ? f3(lx[ib[k]]) : make_float3(0.f, 0.f, 0.f);
+    for (int s = first; s < nsteps; s++) {
         float4 nx[PT], na[PT];
         bool moved = false;
 #pragma unroll
@@ -144,7 +146,7 @@ __device__ __forceinline__ void run_steps(const mte_params &P, const DevTopo &T,
             em_update(P, F, rx, ra, x, a);
             nx[k] = x;   // w keeps the packed flags
             na[k] = a;
-            if (filtered) {
+            if (FILTERED) {
                 const float dx = x.x - xr[k].x, dy = x.y - xr[k].y, dz = x.z - xr[k].z;
                 moved |= dx * dx + dy * dy + dz * dz >= SKIN_HALF * SKIN_HALF;
             }
@@ -153,21 +155,19 @@ __device__ __forceinline__ void run_steps(const mte_params &P, const DevTopo &T,
             rbuf[(((s + 1) & 1) * RNG_SLOTS + help_slot) * 2] = normals4(s0[PT - 1]);
             rbuf[(((s + 1) & 1) * RNG_SLOTS + help_slot) * 2 + 1] = normals4(s1[PT - 1]);
         }
-        if (filtered && moved) flags[s & 1] = 1;
+        if (FILTERED && moved) flags[s & 1] = 1;
         __syncthreads();
-        const bool left = filtered && flags[s & 1] != 0;
-        if (filtered && threadIdx.x == 0) flags[(s + 1) & 1] = 0;   // read at the next step, after its barrier
+        const bool left = FILTERED && flags[s & 1] != 0;
+        if (FILTERED && threadIdx.x == 0) flags[(s + 1) & 1] = 0;   // read at the next step, after its barrier
 #pragma unroll
         for (int k = 0; k < PT; k++) {
             if (!mob[k]) continue;
             trig[k] = store_state(lx, ln, l1, l2, ib[k], nx[k], na[k], L);
         }
         __syncthreads();
-        if (left) {   // uniform: every thread read the same flag
-            refill();
-            filtered = false;
-        }
+        if (left) return s + 1;   // uniform: every thread read the same flag
     }
+    return nsteps;
 }
 
 template <int PT, int MAXT>
@@ -361,14 +361,19 @@ __global__ void __launch_bounds__(MAXT)
         const bool filter = P.lj_on && P.ljpairscutoff > FILTER_R;
         fill(filter ? FILTER_R * FILTER_R : 0.f);
         auto mk = [&](int i, int) { return LdsLists{ent, hoff[i], hcnt[i]}; };
-        auto refill = [&]() { fill(0.f); };
-        run_steps<PT, LdsLists>(P, T, A, t, lx, ln, l1, l2, mob, s0, s1, trig, noise, nsteps, L, mk, filter, flags,
-                                refill, rbuf, own_slot, help_slot, ib);
+        int s = 0;
+        if (filter) {
+            s = run_steps<PT, true, LdsLists>(P, T, A, t, lx, ln, l1, l2, mob, s0, s1, trig, noise, 0, nsteps, L, mk,
+                                              flags, rbuf, own_slot, help_slot, ib);
+            if (s < nsteps) fill(0.f);   // a bead left its skin: the full lists from step s on
+        }
+        if (s < nsteps)
+            run_steps<PT, false, LdsLists>(P, T, A, t, lx, ln, l1, l2, mob, s0, s1, trig, noise, s, nsteps, L, mk,
+                                           flags, rbuf, own_slot, help_slot, ib);
     } else {
         auto mk = [&](int, int b) { return GlobalLists{T, b, base}; };
-        auto refill = [&]() {};
-        run_steps<PT, GlobalLists>(P, T, A, t, lx, ln, l1, l2, mob, s0, s1, trig, noise, nsteps, L, mk, false, flags,
-                                   refill, rbuf, own_slot, help_slot, ib);
+        run_steps<PT, false, GlobalLists>(P, T, A, t, lx, ln, l1, l2, mob, s0, s1, trig, noise, 0, nsteps, L, mk,
+                                          flags, rbuf, own_slot, help_slot, ib);
     }
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int b = base + i;
