@@ -49,22 +49,24 @@ static_assert(3 * FEAT_BLK_NN % 8 == 0 && FEAT_BLK_NN % 4 == 0, "feature blocks"
 // pack into one register.
 template <typename JT>
 struct BucketRanges {
-    int2 v[27];
+    using W = int2;
+    W v[27];
     __device__ __forceinline__ void set(int o, int2 r, int) { v[o] = make_int2(r.x, r.x + r.y); }
     __device__ __forceinline__ void clear(int o) { v[o] = make_int2(0, 0); }
-    __device__ __forceinline__ int lo(int o, int) const { return v[o].x; }
-    __device__ __forceinline__ int hi(int o, int) const { return v[o].y; }
+    __device__ __forceinline__ static int lo(W w, int) { return w.x; }
+    __device__ __forceinline__ static int hi(W w, int) { return w.y; }
 };
 template <>
 struct BucketRanges<uint16_t> {
-    uint32_t v[27];
+    using W = uint32_t;
+    W v[27];
     __device__ __forceinline__ void set(int o, int2 r, int base) {
         const uint32_t s = (uint32_t)(r.x - base);
         v[o] = s | ((s + (uint32_t)r.y) << 16);
     }
     __device__ __forceinline__ void clear(int o) { v[o] = 0; }
-    __device__ __forceinline__ int lo(int o, int base) const { return base + (int)(v[o] & 0xffffu); }
-    __device__ __forceinline__ int hi(int o, int base) const { return base + (int)(v[o] >> 16); }
+    __device__ __forceinline__ static int lo(W w, int base) { return base + (int)(w & 0xffffu); }
+    __device__ __forceinline__ static int hi(W w, int base) { return base + (int)(w >> 16); }
 };
 
 // Neighbour scan over the 27 cells of the hashed table (built with cell side
@@ -76,9 +78,16 @@ struct BucketRanges<uint16_t> {
 // test rejects the beads of other cells sharing a bucket.  Keeps the 32
 // nearest (distance, then index) and sums the exact label over every
 // neighbour within 12 nm (LABELS: only on check steps and for training data).
-// The 32-nearest lists live in LDS with trajectory-local indices of type JT:
-// 16-bit where every trajectory has fewer than 65536 beads (12 KB per wave
-// instead of 16, so more waves per CU hide the candidates' load latency).
+// The 32 nearest live in registers as 64-bit keys (the squared distance's
+// float bits, which order as integers for non-negative floats, over the
+// trajectory-local index), ascending, empty slots all ones.  A candidate
+// that precedes the last key is inserted without branches: its place is the
+// count of smaller keys and every slot takes its own, its left neighbour's
+// or the new key.  A wave's lanes then never wait on one another's shift
+// loops, which on a crowded tube (60-130 candidates per lane, most of them
+// inserted) had made a wave run the longest lane's shifts for every
+// candidate.  JT selects the bucket-range packing (16-bit where every
+// trajectory has fewer than 65536 beads).
 // With fblk, a bead's features are written only up to the 24-float block
 // (8 neighbours) holding its last neighbour, and the number of blocks goes
 // to fblk: the step's network reads the rest as zeros, so in a dilute box
@@ -88,15 +97,18 @@ template <bool LABELS, typename JT>
 __global__ void __launch_bounds__(MLP_BLOCK)
     k_mlp_local(mte_params P, DevTopo T, DevState S, CellTables C, const int2 *blocks, float *feat, float *targ,
                 uint8_t *fblk) {
-    __shared__ float sd[MLP_BLOCK * MLP_NN];
-    __shared__ JT sj[MLP_BLOCK * MLP_NN];
+    using Ranges = BucketRanges<JT>;
+    // the lane's bucket ranges, bucket-major, for the bucket loop (which is
+    // not unrolled: the insertion makes its body long)
+    __shared__ typename Ranges::W srg[27 * MLP_BLOCK];
     const int2 blk = blocks[blockIdx.x];
     const int t = blk.x;
     const int base = T.tbase[t];
     const int b = base + blk.y * MLP_BLOCK + threadIdx.x;
     if (b >= base + T.tsize[t]) return;
-    float *nd = sd + threadIdx.x * MLP_NN;
-    JT *nj = sj + threadIdx.x * MLP_NN;
+    uint64_t key[MLP_NN];
+#pragma unroll
+    for (int k = 0; k < MLP_NN; k++) key[k] = ~0ull;
     const float box = T.tbox[t];
     const float3 ri = f3(S.xyz[b]);
     float A[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -124,7 +136,7 @@ __global__ void __launch_bounds__(MLP_BLOCK)
             const uint32_t h = ((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u);
             bk[o] = toff + (int)(h & mask);
         }
-        BucketRanges<JT> rg;
+        Ranges rg;
 #pragma unroll
         for (int o = 0; o < 27; o++) rg.set(o, C.range[bk[o]], base);
         // a bucket two cells hash to is visited once: a 64-bit filter on the
@@ -169,53 +181,49 @@ __global__ void __launch_bounds__(MLP_BLOCK)
                     for (int y = 0; y < 3; y++) A[3 * x + y] += w * (crr * e[x] * e[y] + (x == y ? cii : 0.f));
             }
             // 32 nearest, ordered by (r2, j)
-            const int jl = j - base;
-            if (cnt < MLP_NN || r2 < nd[cnt - 1] || (r2 == nd[cnt - 1] && jl < (int)nj[cnt - 1])) {
-                int k = cnt < MLP_NN ? cnt++ : MLP_NN - 1;
-                while (k > 0 && (nd[k - 1] > r2 || (nd[k - 1] == r2 && (int)nj[k - 1] > jl))) {
-                    nd[k] = nd[k - 1];
-                    nj[k] = nj[k - 1];
-                    k--;
-                }
-                nd[k] = r2;
-                nj[k] = (JT)jl;
-            }
-        };
-        // Buckets in order, their candidates in bucket order.  Most buckets
-        // hold no bead or one (a few per cell in a dilute box): the first
-        // candidate of each bucket is requested SCAN_CH buckets at a time, the
-        // next group's while this group is visited, so their loads overlap;
-        // a bucket's further candidates follow one load ahead.
-        constexpr int SCAN_CH = 9;
-        auto first = [&](int c0, float4 *p) {
+            const uint64_t kn = ((uint64_t)__float_as_uint(r2) << 32) | (uint32_t)(j - base);
+            if (kn < key[MLP_NN - 1]) {
+                // slots in chunks of 8, a chunk only where some lane of the
+                // wave holds a key in it (the rest are all ones and stay so):
+                // a dilute wave touches one chunk
+                constexpr int CH = 8;
+                bool used[MLP_NN / CH];
 #pragma unroll
-            for (int u = 0; u < SCAN_CH; u++) {
-                const int q0 = rg.lo(c0 + u, base), q1 = rg.hi(c0 + u, base);
-                p[u] = q0 < q1 ? C.spos[q0] : make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-        };
-        auto group = [&](int c0, const float4 *p) {
+                for (int c = 0; c < MLP_NN / CH; c++) used[c] = c == 0 || __any(cnt >= CH * c);
+                int p = 0;
 #pragma unroll
-            for (int u = 0; u < SCAN_CH; u++) {
-                const int q0 = rg.lo(c0 + u, base), q1 = rg.hi(c0 + u, base);
-                if (q0 >= q1) continue;
-                visit(p[u]);
-                float4 pn = q0 + 1 < q1 ? C.spos[q0 + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
-                for (int q = q0 + 1; q < q1; q++) {
-                    const float4 pj = pn;
-                    if (q + 1 < q1) pn = C.spos[q + 1];
-                    visit(pj);
-                }
+                for (int c = 0; c < MLP_NN / CH; c++)
+                    if (used[c]) {
+#pragma unroll
+                        for (int k = CH * c; k < CH * (c + 1); k++) p += key[k] < kn ? 1 : 0;
+                    }
+#pragma unroll
+                for (int c = MLP_NN / CH - 1; c >= 0; c--)
+                    if (used[c]) {
+#pragma unroll
+                        for (int k = CH * (c + 1) - 1; k >= CH * c; k--)
+                            key[k] = k > p ? key[k > 0 ? k - 1 : 0] : (k == p ? kn : key[k]);
+                    }
+                cnt = min(cnt + 1, MLP_NN);
             }
         };
-        static_assert(27 == 3 * SCAN_CH, "three bucket groups");
-        float4 pa[SCAN_CH], pb[SCAN_CH];
-        first(0, pa);
-        first(SCAN_CH, pb);
-        group(0, pa);
-        first(2 * SCAN_CH, pa);
-        group(SCAN_CH, pb);
-        group(2 * SCAN_CH, pa);
+#pragma unroll
+        for (int o = 0; o < 27; o++) srg[o * MLP_BLOCK + threadIdx.x] = rg.v[o];
+        // Buckets in order, their candidates in bucket order, each one load
+        // issued two candidates ahead.
+#pragma nounroll
+        for (int c = 0; c < 27; c++) {
+            const typename Ranges::W w = srg[c * MLP_BLOCK + threadIdx.x];
+            const int q0 = Ranges::lo(w, base), q1 = Ranges::hi(w, base);
+            float4 pn = q0 < q1 ? C.spos[q0] : make_float4(0.f, 0.f, 0.f, 0.f);
+            float4 pm = q0 + 1 < q1 ? C.spos[q0 + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int q = q0; q < q1; q++) {
+                const float4 pj = pn;
+                pn = pm;
+                if (q + 2 < q1) pm = C.spos[q + 2];
+                visit(pj);
+            }
+        }
     }
     if (feat) {
         // four neighbours (12 floats, three float4 stores) at a time, their
@@ -232,7 +240,7 @@ __global__ void __launch_bounds__(MLP_BLOCK)
                 const int k = k0 + u;
                 float3 d = make_float3(0.f, 0.f, 0.f);
                 if (k < cnt) {
-                    d = min_image(sub3(f3(S.xyz[base + (int)nj[k]]), ri), box);
+                    d = min_image(sub3(f3(S.xyz[base + (int)(uint32_t)key[k]]), ri), box);
                     d = make_float3(d.x / MLP_RC, d.y / MLP_RC, d.z / MLP_RC);
                 }
                 v[3 * u] = d.x;
