@@ -199,6 +199,7 @@ __global__ void __launch_bounds__(MAXT)
     uint4 s0[PT], s1[PT];
     float4 trig[PT];
     int ib[PT];   // the bead of each of this thread's lane slots (order: immobile beads last)
+    int lcnt[PT];   // its list counts, packed as in hcnt (requested with the bead's state)
 #pragma unroll
     for (int k = 0; k < PT; k++) {
         const int slot = threadIdx.x + k * blockDim.x;
@@ -206,6 +207,9 @@ __global__ void __launch_bounds__(MAXT)
         ib[k] = i;
         const int b = base + i;
         mob[k] = i < n && !T.fixed[b] && !T.extra[b];
+        lcnt[k] = 0;
+        if (mob[k])
+            lcnt[k] = pack_counts(T.harm[b] >= 0, T.lon_n[b], T.lat_n[b], P.lj_on ? T.lj_n[b] : 0);
         trig[k] = make_float4(0.f, 1.f, 0.f, 1.f);
         if (mob[k]) {
             const float4 a = S.ang[b];
@@ -265,11 +269,9 @@ __global__ void __launch_bounds__(MAXT)
 #pragma unroll
         for (int k = 0; k < PT; k++) {
             if (!mob[k]) continue;
-            const int i = ib[k], b = base + i;
-            const int h = T.harm[b] >= 0 ? 1 : 0, nlon = T.lon_n[b], nlat = T.lat_n[b];
-            const int nlj = P.lj_on ? T.lj_n[b] : 0;
-            hcnt[i] = pack_counts(h, nlon, nlat, nlj);
-            hoff[i] = h + nlon + nlat + nlj;
+            const int i = ib[k], hc = lcnt[k];
+            hcnt[i] = hc;
+            hoff[i] = (hc & 1) + ((hc >> 1) & 15) + ((hc >> 5) & 31) + (hc >> 10);
         }
         // Chunked exclusive scan: thread t owns beads [t*c, (t+1)*c); a wave
         // scans its lanes' chunk totals by shuffles, the first wave the
