@@ -128,15 +128,15 @@ struct GlobalLists {
 };
 
 // LJ repulsion over the list; the cutoff test selects the force (no branch
-// around the accumulation), and periodic and open boxes get separate loops.
-template <bool PERIODIC, class Acc, class Lists>
-__device__ __forceinline__ void lj_forces(const mte_params &P, const Acc &A, const Lists &Ls, const Own &I,
-                                          float F[6]) {
+// around the accumulation), and periodic and open boxes get separate loops,
+// as do runs with and without the regularization cap (a uniform choice: the
+// cap's reciprocal square root is not paid per pair when it is off).
+template <bool PERIODIC, bool REG, class Acc, class Lists>
+__device__ __forceinline__ void lj_sum(const mte_params &P, const Acc &A, const Lists &Ls, const Own &I, float F[6]) {
     const int nlj = Ls.nlj();
     const float s0 = P.ljscale * P.ljsigma6;
     // REGULARIZATION (src/compute_cuda.cu:482-486): the pair force s r is
     // capped at gammaR r_mon / dt, i.e. s at that over r (r_mon = 2 nm)
-    const bool reg = P.lj_regularize != 0;
     const float cap = P.gammaR * 2.f / P.dt;
     float fx = F[0], fy = F[1], fz = F[2];   // same accumulation order, scalar registers
     auto pair = [&](float4 xj) {
@@ -145,7 +145,7 @@ __device__ __forceinline__ void lj_forces(const mte_params &P, const Acc &A, con
         const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
         const float inv = rcp_fast(r2);
         float s = r2 < LJ_CUT * LJ_CUT ? s0 * (6.f * (inv * inv) * (inv * inv)) : 0.f;
-        if (reg) s = fminf(s, cap * rsq_fast(r2));
+        if (REG) s = fminf(s, cap * rsq_fast(r2));
         fx += s * d.x;
         fy += s * d.y;
         fz += s * d.z;
@@ -166,6 +166,12 @@ __device__ __forceinline__ void lj_forces(const mte_params &P, const Acc &A, con
     F[0] = fx;
     F[1] = fy;
     F[2] = fz;
+}
+template <bool PERIODIC, class Acc, class Lists>
+__device__ __forceinline__ void lj_forces(const mte_params &P, const Acc &A, const Lists &Ls, const Own &I,
+                                          float F[6]) {
+    if (P.lj_regularize) lj_sum<PERIODIC, true>(P, A, Ls, I, F);
+    else lj_sum<PERIODIC, false>(P, A, Ls, I, F);
 }
 
 // Force and torque on bead b (an accessor index, as are the list entries);
