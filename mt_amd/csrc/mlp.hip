@@ -98,8 +98,8 @@ __global__ void __launch_bounds__(MLP_BLOCK)
     k_mlp_local(mte_params P, DevTopo T, DevState S, CellTables C, const int2 *blocks, float *feat, float *targ,
                 uint8_t *fblk) {
     using Ranges = BucketRanges<JT>;
-    // the lane's bucket ranges, bucket-major, for the bucket loop (which is
-    // not unrolled: the insertion makes its body long)
+    // the lane's non-empty bucket ranges, in bucket order, for the walk over
+    // its candidates
     __shared__ typename Ranges::W srg[27 * MLP_BLOCK];
     const int2 blk = blocks[blockIdx.x];
     const int t = blk.x;
@@ -207,22 +207,38 @@ __global__ void __launch_bounds__(MLP_BLOCK)
                 cnt = min(cnt + 1, MLP_NN);
             }
         };
+        // the non-empty ranges, in bucket order, and their candidate count
+        int nne = 0, total = 0;
 #pragma unroll
-        for (int o = 0; o < 27; o++) srg[o * MLP_BLOCK + threadIdx.x] = rg.v[o];
-        // Buckets in order, their candidates in bucket order, each one load
-        // issued two candidates ahead.
-#pragma nounroll
-        for (int c = 0; c < 27; c++) {
-            const typename Ranges::W w = srg[c * MLP_BLOCK + threadIdx.x];
-            const int q0 = Ranges::lo(w, base), q1 = Ranges::hi(w, base);
-            float4 pn = q0 < q1 ? C.spos[q0] : make_float4(0.f, 0.f, 0.f, 0.f);
-            float4 pm = q0 + 1 < q1 ? C.spos[q0 + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
-            for (int q = q0; q < q1; q++) {
-                const float4 pj = pn;
-                pn = pm;
-                if (q + 2 < q1) pm = C.spos[q + 2];
-                visit(pj);
+        for (int o = 0; o < 27; o++) {
+            const int q0 = Ranges::lo(rg.v[o], base), q1 = Ranges::hi(rg.v[o], base);
+            if (q1 > q0) {
+                srg[nne * MLP_BLOCK + threadIdx.x] = rg.v[o];
+                nne++;
+                total += q1 - q0;
             }
+        }
+        // One flat walk over the lane's candidates (buckets in order, each
+        // bucket's in slot order), the next one's load issued before this
+        // one is visited.  A wave then runs as many iterations as its lane
+        // with the most candidates, not the sum over buckets of each
+        // bucket's longest lane.
+        int r = 0, q = 0, qend = 0;
+        if (nne > 0) {
+            const typename Ranges::W w = srg[threadIdx.x];
+            q = Ranges::lo(w, base);
+            qend = Ranges::hi(w, base);
+        }
+        float4 pn = total > 0 ? C.spos[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int k = 0; k < total; k++) {
+            const float4 pj = pn;
+            if (++q == qend && ++r < nne) {
+                const typename Ranges::W w = srg[r * MLP_BLOCK + threadIdx.x];
+                q = Ranges::lo(w, base);
+                qend = Ranges::hi(w, base);
+            }
+            if (k + 1 < total) pn = C.spos[q];
+            visit(pj);
         }
     }
     if (feat) {
