@@ -242,30 +242,33 @@ __global__ void __launch_bounds__(MLP_BLOCK)
         }
     }
     if (feat) {
-        // four neighbours (12 floats, three float4 stores) at a time, their
-        // positions gathered together
+        // a block of eight neighbours (24 floats, six float4 stores) at a
+        // time, their positions requested together
         float4 *f = reinterpret_cast<float4 *>(feat + (size_t)b * MLP_IN);
         const int nblk = (cnt + FEAT_BLK_NN - 1) / FEAT_BLK_NN;
         if (fblk) fblk[b] = (uint8_t)nblk;
 #pragma unroll
-        for (int k0 = 0; k0 < MLP_NN; k0 += 4) {
+        for (int k0 = 0; k0 < MLP_NN; k0 += FEAT_BLK_NN) {
             if (fblk && k0 >= FEAT_BLK_NN * nblk) break;
-            float v[12];
+            float4 xj[FEAT_BLK_NN];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int k = k0 + u;
+            for (int u = 0; u < FEAT_BLK_NN; u++)
+                xj[u] = k0 + u < cnt ? S.xyz[base + (int)(uint32_t)key[k0 + u]] : make_float4(0.f, 0.f, 0.f, 0.f);
+            float v[3 * FEAT_BLK_NN];
+#pragma unroll
+            for (int u = 0; u < FEAT_BLK_NN; u++) {
                 float3 d = make_float3(0.f, 0.f, 0.f);
-                if (k < cnt) {
-                    d = min_image(sub3(f3(S.xyz[base + (int)(uint32_t)key[k]]), ri), box);
+                if (k0 + u < cnt) {
+                    d = min_image(sub3(f3(xj[u]), ri), box);
                     d = make_float3(d.x / MLP_RC, d.y / MLP_RC, d.z / MLP_RC);
                 }
                 v[3 * u] = d.x;
                 v[3 * u + 1] = d.y;
                 v[3 * u + 2] = d.z;
             }
-            f[3 * k0 / 4] = make_float4(v[0], v[1], v[2], v[3]);
-            f[3 * k0 / 4 + 1] = make_float4(v[4], v[5], v[6], v[7]);
-            f[3 * k0 / 4 + 2] = make_float4(v[8], v[9], v[10], v[11]);
+#pragma unroll
+            for (int w = 0; w < 3 * FEAT_BLK_NN / 4; w++)
+                f[3 * k0 / 4 + w] = make_float4(v[4 * w], v[4 * w + 1], v[4 * w + 2], v[4 * w + 3]);
         }
     }
     if (LABELS) {
