@@ -59,11 +59,6 @@ __device__ __forceinline__ int3 cell_of(float3 r, float box, const CellGeom &g) 
     return make_int3((int)floorf(r.x / g.cs), (int)floorf(r.y / g.cs), (int)floorf(r.z / g.cs));
 }
 
-__global__ void __launch_bounds__(256) k_fill_int(int *p, int n, int v) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
 __global__ void __launch_bounds__(256)
     k_cell_assign(DevTopo T, DevState S, CellTables C, float side, int K) {
     const int b = blockIdx.x * 256 + threadIdx.x;
@@ -137,6 +132,7 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_apply(CellTables C) {
     for (int k = 0; k < SCAN_PER; k++) {
         const int idx = k * SCAN_T + threadIdx.x;
         cnt[scan_pad(idx)] = g0 + idx < n ? C.count[g0 + idx] : 0;
+        if (g0 + idx < n) C.count[g0 + idx] = 0;   // ready for the next build's counting atomics
     }
     __syncthreads();
     const int run = threadIdx.x * SCAN_PER;
@@ -259,7 +255,8 @@ float candidate_radius(const mte_params &P) {
 hipError_t build_cells(mte_engine *e, float side, int K) {
     const CellTables &C = e->cells;
     const int gb = (e->nb + 255) / 256;
-    hipLaunchKernelGGL(k_fill_int, dim3((C.total + 255) / 256), dim3(256), 0, e->stream, C.count, C.total, 0);
+    // the bucket counts are zero here: set at creation, and cleared by
+    // k_scan_apply once it has read them
     hipLaunchKernelGGL(k_cell_assign, dim3(gb), dim3(256), 0, e->stream, e->T, e->S[e->cur], C, side, K);
     const int nblk = (C.total + CELL_SCAN_SPAN - 1) / CELL_SCAN_SPAN;
     hipLaunchKernelGGL(k_scan_partial, dim3(nblk), dim3(SCAN_T), 0, e->stream, C.count, C.total, C.bsum);

@@ -574,7 +574,8 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
             C.toff = p_toff;
             C.tsize = p_tsz;
             C.total = total;
-            if (hipMemcpyAsync(p_toff, toff.data(), sizeof(int) * nt, hipMemcpyHostToDevice, e->stream) != hipSuccess ||
+            if (hipMemsetAsync(C.count, 0, sizeof(int) * (size_t)total, e->stream) != hipSuccess ||
+                hipMemcpyAsync(p_toff, toff.data(), sizeof(int) * nt, hipMemcpyHostToDevice, e->stream) != hipSuccess ||
                 hipMemcpyAsync(p_tsz, tsz.data(), sizeof(int) * nt, hipMemcpyHostToDevice, e->stream) != hipSuccess) {
                 e->err = "mte_create: upload failed for cell tables";
                 return bail(MTE_ERR_HIP);
@@ -589,6 +590,7 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
             dalloc(e, &M.fblk, nb) != hipSuccess ||
             dalloc(e, &M.targ, (size_t)MLP_OUT * nb) != hipSuccess ||
             dalloc(e, &M.ystep, (size_t)MLP_OUT * nb) != hipSuccess || dalloc(e, &M.err, 2 * (size_t)nt) != hipSuccess ||
+            dalloc(e, &M.err_part, 2 * (size_t)nt * MLP_ERR_PARTS) != hipSuccess ||
             dalloc(e, &M.nbad, 1) != hipSuccess || dalloc(e, &M.img, mlp_image_bytes() / sizeof(uint4)) != hipSuccess) {
             e->err = "mte_create: hipMalloc failed for MLP buffers";
             return bail(MTE_ERR_HIP);

@@ -82,6 +82,7 @@ constexpr int CELL_SCAN_SPAN = 4096;
 // MLP-HI network and work buffers (mlp.hip).
 constexpr int MLP_NN = 32, MLP_IN = 96, MLP_H = 128, MLP_OUT = 9;
 constexpr int MLP_SEARCH_BLOCK = 64;   // beads per workgroup of the neighbour search
+constexpr int MLP_ERR_PARTS = 16;      // workgroups per trajectory of the judge's error sums
 constexpr int MLP_NPARAM = MLP_IN * MLP_H + MLP_H + 2 * (MLP_H * MLP_H + MLP_H) + MLP_H * MLP_OUT + MLP_OUT;
 // mte_mlp_forward_noalloc: rows per call (the fixed staging array's size)
 // and the device ordinals whose array address is kept.
@@ -96,6 +97,7 @@ struct MlpBuffers {
     int2 *blocks = nullptr;    // neighbour-search workgroups: (trajectory, 64-bead block), round robin over trajectories
     int nblocks = 0;
     double *err = nullptr;                    // per trajectory: squared error, squared label norm
+    double *err_part = nullptr;               // their partial sums, MLP_ERR_PARTS per trajectory
     int *nbad = nullptr;                      // beads whose prediction failed the Cholesky check
     // training work space for up to `cap` samples
     int cap = 0;

@@ -695,14 +695,17 @@ __global__ void __launch_bounds__(NET_THREADS)
 }
 
 // Per-trajectory squared error and squared label norm of predictions y
-// against labels t over mobile beads (sampled-error judge), FP64 sums.
+// against labels t over mobile beads (sampled-error judge), FP64 sums: each
+// of MLP_ERR_PARTS workgroups per trajectory sums every MLP_ERR_PARTS-th
+// 256-bead slice into its partial, and k_mlp_error_sum adds the partials in
+// order.
 __global__ void __launch_bounds__(256)
-    k_mlp_error(DevTopo T, const float *y, const float *t, double *out) {
+    k_mlp_error(DevTopo T, const float *y, const float *t, double *part) {
     __shared__ double se[256], sn[256];
-    const int tr = blockIdx.x;
+    const int tr = blockIdx.x, p = blockIdx.y;
     const int base = T.tbase[tr], n = T.tsize[tr];
     double e = 0, m = 0;
-    for (int i = threadIdx.x; i < n; i += 256) {
+    for (int i = p * 256 + threadIdx.x; i < n; i += 256 * MLP_ERR_PARTS) {
         const int b = base + i;
         if (T.fixed[b] || T.extra[b]) continue;
         for (int q = 0; q < MLP_OUT; q++) {
@@ -722,9 +725,21 @@ __global__ void __launch_bounds__(256)
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        out[2 * tr] = se[0];
-        out[2 * tr + 1] = sn[0];
+        part[2 * ((size_t)tr * MLP_ERR_PARTS + p)] = se[0];
+        part[2 * ((size_t)tr * MLP_ERR_PARTS + p) + 1] = sn[0];
     }
+}
+
+__global__ void __launch_bounds__(64) k_mlp_error_sum(int ntraj, const double *part, double *out) {
+    const int tr = blockIdx.x * 64 + threadIdx.x;
+    if (tr >= ntraj) return;
+    double e = 0, m = 0;
+    for (int p = 0; p < MLP_ERR_PARTS; p++) {
+        e += part[2 * ((size_t)tr * MLP_ERR_PARTS + p)];
+        m += part[2 * ((size_t)tr * MLP_ERR_PARTS + p) + 1];
+    }
+    out[2 * tr] = e;
+    out[2 * tr + 1] = m;
 }
 
 // ----------------------------------------------------------------- training
@@ -886,7 +901,10 @@ hipError_t launch_mlp_step(mte_engine *e, const float *noise_step, float *yout) 
 }
 
 hipError_t launch_mlp_error(mte_engine *e, const float *y, const float *t) {
-    hipLaunchKernelGGL(k_mlp_error, dim3(e->ntraj), dim3(256), 0, e->stream, e->T, y, t, e->mlp.err);
+    hipLaunchKernelGGL(k_mlp_error, dim3(e->ntraj, MLP_ERR_PARTS), dim3(256), 0, e->stream, e->T, y, t,
+                       e->mlp.err_part);
+    hipLaunchKernelGGL(k_mlp_error_sum, dim3((e->ntraj + 63) / 64), dim3(64), 0, e->stream, e->ntraj,
+                       (const double *)e->mlp.err_part, e->mlp.err);
     return hipGetLastError();
 }
 

@@ -254,13 +254,22 @@ __global__ void __launch_bounds__(256)
     const int t = T.tid[b], n = T.tsize[t];
     const int h = (b - T.tbase[t]) / EPS_GROUP, g = (n + EPS_GROUP - 1) / EPS_GROUP;
     const int nb = T.nb;
-    float s[5];
+    // each of the five sums in the same order (groups h..g-1 of side 0, then
+    // 0..h of side 1), the five advancing together so their loads overlap
+    float s[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int j = h; j < g; j++) {
+        float v[5];
 #pragma unroll
-    for (int k = 0; k < 5; k++) {
-        float v = 0.f;
-        for (int j = h; j < g; j++) v += part[(((size_t)0 * ngrp + j) * 5 + k) * nb + b];
-        for (int i = 0; i <= h; i++) v += part[(((size_t)1 * ngrp + i) * 5 + k) * nb + b];
-        s[k] = v;
+        for (int k = 0; k < 5; k++) v[k] = part[(((size_t)0 * ngrp + j) * 5 + k) * nb + b];
+#pragma unroll
+        for (int k = 0; k < 5; k++) s[k] += v[k];
+    }
+    for (int i = 0; i <= h; i++) {
+        float v[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) v[k] = part[(((size_t)1 * ngrp + i) * 5 + k) * nb + b];
+#pragma unroll
+        for (int k = 0; k < 5; k++) s[k] += v[k];
     }
     ci[b] = make_float4(s[0] + s[1], s[0] + s[2], s[0] + s[3], 0.f);
     eps[b] = s[4];
