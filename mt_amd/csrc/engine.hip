@@ -515,6 +515,7 @@ int mte_create(const mte_params *params, const mte_topology *topo, const mte_opt
     UP(e->rb_order, order.data(), nb)
 #undef UP
     e->fixed_h = fixed;
+    e->any_extra = std::any_of(extra.begin(), extra.end(), [](uint8_t v) { return v != 0; });
     if (upload_resident_order(e, extra.data()) != hipSuccess) {
         e->err = "mte_create: upload failed for the resident lane order";
         return bail(MTE_ERR_HIP);
@@ -707,6 +708,7 @@ int mte_set_flags(mte_engine *e, const uint8_t *extra, const int32_t *gtp, const
             return fail(e, MTE_ERR_STATE, "mte_set_flags: no resident plan fits the new flags");
         }
         HIPCHK(hipMemcpyAsync(e->T.extra, x.data(), e->nb, hipMemcpyHostToDevice, e->stream));
+        e->any_extra = std::any_of(x.begin(), x.end(), [](uint8_t v) { return v != 0; });
         HIPCHK(upload_resident_order(e, x.data()));
         HIPCHK(hipStreamSynchronize(e->stream));
         if (plan_ok) {

@@ -146,6 +146,7 @@ struct mte_engine {
     int *res_order = nullptr;  // per trajectory: local bead at each resident lane slot (immobile beads last)
     std::vector<uint8_t> fixed_h;   // host copy of the fixed flags (the resident lane order)
     int res_mobile_max = 0;          // most mobile beads in one trajectory (the resident plan)
+    bool any_extra = false;          // some bead is flagged extra (TEA's epsilon weights are needed)
     int *mtl_counts = nullptr, *mtl_flags = nullptr;   // mte_mt_length: per trajectory, per bead
     size_t forces_lds = 0;     // > 0: forces by k_forces_lds with this much dynamic LDS
     int rb_traj_threads = 0;   // > 0: list rebuilds by k_rebuild_traj with this workgroup size

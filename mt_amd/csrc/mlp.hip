@@ -413,9 +413,11 @@ __global__ void __launch_bounds__(256) k_mlp_pack(const float *__restrict__ wi, 
 }
 
 __device__ __forceinline__ float tanh_fast(float x) {
-    // tanh|x| = 1 - 2 / (exp(2|x|) + 1): absolute error a few 1e-7, exact limit 1
-    const float e = __expf(2.f * fabsf(x));
-    return copysignf(1.f - 2.f * rcp_fast(e + 1.f), x);
+    // tanh x = 1 - 2 / (exp(2x) + 1) for either sign: absolute error a few
+    // 1e-7; exp(2x) overflows to inf for large x (1) and underflows to 0
+    // for large -x (-1)
+    const float e = __expf(2.f * x);
+    return fmaf(-2.f, rcp_fast(e + 1.f), 1.f);
 }
 
 __device__ __forceinline__ void split8(const float v[8], bf16x8 &hi, bf16x8 &lo) {

@@ -44,6 +44,11 @@ __device__ __forceinline__ float wave_rol1(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x134, 0xF, 0xF, false));
 }
 
+// wave_rol1(v) + p, the rotation a DPP operand of the addition.
+__device__ __forceinline__ float wave_rol1_add(float v, float p) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, true)) + p;
+}
+
 // The far branch of rpy_select alone (r > 2a).
 __device__ __forceinline__ RpyCoef rpy_far(float rinv, float a) {
     const float u1 = a * rinv, u2 = u1 * u1;
@@ -139,9 +144,10 @@ __global__ void __launch_bounds__(TEA_TILE)
 // The waves' j-side sums of a chunk are added in wave order in LDS.  Output
 // (deterministic, no atomics): i-side sums of a bead of group H into
 // part[0][J][.] for J >= H, j-side sums into part[1][I][.] for I <= H; the
-// pair weight is 1 for two non-extra beads and 0 otherwise.
+// pair weight is 1 for two non-extra beads and 0 otherwise (WEIGHTED: some
+// bead of the batch is extra; without, every weight is 1 and none is formed).
 constexpr int EPS_CHUNKS = EPS_GROUP / 64;
-template <bool PERIODIC>
+template <bool PERIODIC, bool WEIGHTED>
 __global__ void __launch_bounds__(EPS_GROUP)
     k_tea_eps_sym(mte_params P, DevTopo T, DevState S, const int4 *units, int ngrp, float *part) {
     __shared__ float4 jx[2 * 64];               // chunk c_J: x, weight; twice, no wrap
@@ -175,24 +181,35 @@ __global__ void __launch_bounds__(EPS_GROUP)
         }
         __syncthreads();
         float gj[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-        // one pair's five terms into the sums t (and t2 when both sides); a
-        // wave with no pair inside 2a (margin over the rounding of r / a)
-        // takes the far branch alone, which is the branch rpy_select picks
+        // one pair's five terms p: added to the sums t; with t2 (both
+        // sides), t2 = (t2 of the next lane) + p, the j-side rotation taken
+        // with the addition.  A wave with no pair inside 2a (margin over the
+        // rounding of r / a) keeps the far coefficients, which is the branch
+        // rpy_select picks for each of its pairs.  WT: the pair weights
+        // are formed (extra beads in the batch, or the padding of a
+        // trajectory's last chunk, whose slots have weight 0).
         const float near2 = (2.002f * a) * (2.002f * a);
-        auto pair = [&](float4 xj, float *t, float *t2) {
+        auto pair = [&](auto wt, float4 xj, float *t, float *t2) {
+            constexpr bool WT = decltype(wt)::value;
             float3 d = make_float3(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z);
             if (PERIODIC) d = min_image_inv(d, box, ibox);
             const float r2 = fmaxf(d.x * d.x + d.y * d.y + d.z * d.z, 1e-12f);
             const float rinv = rsq_fast(r2);
-            RpyCoef c;
-            if (__any(r2 <= near2)) c = rpy_select(r2, rinv, a, ia);
-            else c = rpy_far(rinv, a);
-            const float ww = wi * xj.w;
+            RpyCoef c = rpy_far(rinv, a);
+            if (__any(r2 <= near2)) {
+                __asm__ volatile("");   // a branch: no speculation of the near form
+                const float ra = r2 * rinv * ia;
+                const bool nr = !(ra > 2.f);
+                c.cii = nr ? fmaf(-0.28125f, ra, 1.f) : c.cii;
+                c.crr = nr ? 0.09375f * ra : c.crr;
+            }
+            const float ww = WT ? wi * xj.w : 1.f;
             const float ex = d.x * rinv, ey = d.y * rinv, ez = d.z * rinv;
-            const float cw = ww * c.cii;
-            const float k = ww * c.crr * fmaf(2.f, c.cii, c.crr);
+            const float cw = WT ? ww * c.cii : c.cii;
+            const float wcrr = WT ? ww * c.crr : c.crr;
+            const float k = wcrr * fmaf(2.f, c.cii, c.crr);
             const float se = ex + ey + ez;
-            const float tall = fmaf(ww * c.crr, se * se, 3.f * cw);
+            const float tall = fmaf(wcrr, se * se, 3.f * cw);
             const float qx = ex * ex, qy = ey * ey, qz = ez * ez;
             t[0] = fmaf(cw, c.cii, t[0]);
             t[1] = fmaf(k, qx, t[1]);
@@ -200,23 +217,26 @@ __global__ void __launch_bounds__(EPS_GROUP)
             t[3] = fmaf(k, qz, t[3]);
             t[4] += tall;
             if (t2) {
-                t2[0] = fmaf(cw, c.cii, t2[0]);
-                t2[1] = fmaf(k, qx, t2[1]);
-                t2[2] = fmaf(k, qy, t2[2]);
-                t2[3] = fmaf(k, qz, t2[3]);
-                t2[4] += tall;
+                t2[0] = fmaf(cw, c.cii, wave_rol1(t2[0]));
+                t2[1] = fmaf(k, qx, wave_rol1(t2[1]));
+                t2[2] = fmaf(k, qy, wave_rol1(t2[2]));
+                t2[3] = fmaf(k, qz, wave_rol1(t2[3]));
+                t2[4] = wave_rol1_add(t2[4], tall);
             }
         };
         if (cI < nch && cJ > cI) {
-            for (int s = 0; s < 64; s++) {
-                pair(jx[l + s], ai, gj);
-#pragma unroll
-                for (int k = 0; k < 5; k++) gj[k] = wave_rol1(gj[k]);
-            }
+            // step s: lane l meets j = (l + s) mod 64, its gj the sums of
+            // that j (rotated in from lane l + 1, which held j at step s - 1)
+            // (chunk c_I < c_J is full; c_J may be the padded last one)
+            if (WEIGHTED || (cJ + 1) * 64 > n)
+                for (int s = 0; s < 64; s++) pair(std::true_type{}, jx[l + s], ai, gj);
+            else
+                for (int s = 0; s < 64; s++) pair(std::false_type{}, jx[l + s], ai, gj);
         } else if (cI < nch && cJ == cI) {
+            // one-sided over the chunk's beads: no padding slot is met
             const int jn = min(64, n - cJ * 64);
-            for (int q = 0; q < jn; q++) pair(jx[q], ai, nullptr);
-            if (wi != 0.f) {
+            for (int q = 0; q < jn; q++) pair(std::integral_constant<bool, WEIGHTED>{}, jx[q], ai, nullptr);
+            if (!WEIGHTED || wi != 0.f) {
                 // the j == i slot entered as a coincident bead (d = 0):
                 // cii^2 into the first sum and 3 cii into the last, taken out
                 const float r2 = 1e-12f, rinv = rsq_fast(r2);
@@ -225,8 +245,9 @@ __global__ void __launch_bounds__(EPS_GROUP)
                 ai[4] = fmaf(-3.f, cii, ai[4]);
             }
         }
+        // after step 63, lane l holds the sums of j = (l + 63) mod 64
 #pragma unroll
-        for (int k = 0; k < 5; k++) jred[w][k][l] = gj[k];
+        for (int k = 0; k < 5; k++) jred[w][k][(l + 63) & 63] = gj[k];
         __syncthreads();
         if (threadIdx.x < 5 * 64) {
             const int k = threadIdx.x / 64, q = threadIdx.x % 64;
@@ -1027,12 +1048,10 @@ __global__ void __launch_bounds__(256)
 
 hipError_t launch_tea_eps(mte_engine *e) {
     if (e->tune.tea_eps == MTE_TEA_EPS_ONEPASS) {
-        if (e->periodic)
-            hipLaunchKernelGGL(k_tea_eps_sym<true>, dim3(e->tea.eps_nunits), dim3(EPS_GROUP), 0, e->stream, e->P,
-                               e->T, e->S[e->cur], e->tea.eps_units, e->tea.eps_ngrp, e->tea.eps_part);
-        else
-            hipLaunchKernelGGL(k_tea_eps_sym<false>, dim3(e->tea.eps_nunits), dim3(EPS_GROUP), 0, e->stream, e->P,
-                               e->T, e->S[e->cur], e->tea.eps_units, e->tea.eps_ngrp, e->tea.eps_part);
+        auto kern = e->periodic ? (e->any_extra ? k_tea_eps_sym<true, true> : k_tea_eps_sym<true, false>)
+                                : (e->any_extra ? k_tea_eps_sym<false, true> : k_tea_eps_sym<false, false>);
+        hipLaunchKernelGGL(kern, dim3(e->tea.eps_nunits), dim3(EPS_GROUP), 0, e->stream, e->P, e->T, e->S[e->cur],
+                           e->tea.eps_units, e->tea.eps_ngrp, e->tea.eps_part);
         hipLaunchKernelGGL(k_tea_eps_reduce, dim3((e->nb + 255) / 256), dim3(256), 0, e->stream, e->T,
                            e->tea.eps_ngrp, e->tea.eps_part, e->tea.ci, e->tea.eps);
     } else if (e->periodic)
