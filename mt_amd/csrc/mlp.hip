@@ -451,6 +451,35 @@ __device__ __forceinline__ void net_layer(const u32x4 *frag, const float *bias, 
     }
 }
 
+// The input layer over the k-slices in kuse (bit ks: features 32 ks ..
+// 32 ks + 31); a slice outside kuse is zero for every bead of the tile and
+// adds nothing.
+__device__ __forceinline__ void net_layer_in(const u32x4 *frag, const float *bias,
+                                             const bf16x8 (&hi)[KS_IN][NET_GROUPS],
+                                             const bf16x8 (&lo)[KS_IN][NET_GROUPS], f32x4 (&acc)[MT_H][NET_GROUPS],
+                                             int lane, uint32_t kuse) {
+    const int g = lane >> 4;
+#pragma unroll
+    for (int mt = 0; mt < MT_H; mt++) {
+        const f32x4 b = *reinterpret_cast<const f32x4 *>(bias + 16 * mt + 4 * g);
+#pragma unroll
+        for (int grp = 0; grp < NET_GROUPS; grp++) acc[mt][grp] = b;
+    }
+#pragma unroll
+    for (int ks = 0; ks < KS_IN; ks++) {
+        if (!((kuse >> ks) & 1u)) continue;
+#pragma unroll
+        for (int mt = 0; mt < MT_H; mt++) {
+            const bf16x8 a = __builtin_bit_cast(bf16x8, frag[(mt * KS_IN + ks) * 64 + lane]);
+#pragma unroll
+            for (int grp = 0; grp < NET_GROUPS; grp++) {
+                acc[mt][grp] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, hi[ks][grp], acc[mt][grp], 0, 0, 0);
+                acc[mt][grp] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, lo[ks][grp], acc[mt][grp], 0, 0, 0);
+            }
+        }
+    }
+}
+
 // tanh of a hidden layer, repacked as the next layer's B fragments.
 template <int KS>
 __device__ __forceinline__ void hidden_to_frags(f32x4 (&acc)[2 * KS][NET_GROUPS], bf16x8 (&hi)[KS][NET_GROUPS],
@@ -479,12 +508,20 @@ __device__ __forceinline__ void net_tile(const u32x4 *frag, const float *bias, c
     // The fragments are the same for every tile: keep the compiler from
     // hoisting all of them (368 registers) out of the caller's tile loop.
     __asm__ volatile("" ::: "memory");
+    // k-slices of the input with a feature present for some bead of the tile
+    int most = 0;
+#pragma unroll
+    for (int grp = 0; grp < NET_GROUPS; grp++) most = max(most, (int)((lim >> (8 * grp)) & 0xffu));
+    uint32_t kuse = 0;
+#pragma unroll
+    for (int ks = 0; ks < KS_IN; ks++) kuse |= __any(most > 32 * ks) ? 1u << ks : 0u;
     bf16x8 hi1[KS_IN][NET_GROUPS], lo1[KS_IN][NET_GROUPS];
 #pragma unroll
     for (int grp = 0; grp < NET_GROUPS; grp++) {
         const int bead = b0 + 16 * grp + col;
 #pragma unroll
         for (int ks = 0; ks < KS_IN; ks++) {
+            if (!((kuse >> ks) & 1u)) continue;
             float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
             if (32 * ks + 8 * g < (int)((lim >> (8 * grp)) & 0xffu)) {
                 const float4 *src = reinterpret_cast<const float4 *>(feat + (size_t)bead * MLP_IN + 32 * ks + 8 * g);
@@ -496,7 +533,7 @@ __device__ __forceinline__ void net_tile(const u32x4 *frag, const float *bias, c
         }
     }
     f32x4 acc[MT_H][NET_GROUPS];
-    net_layer<KS_IN, MT_H>(frag + FR1, bias, hi1, lo1, acc, lane);
+    net_layer_in(frag + FR1, bias, hi1, lo1, acc, lane, kuse);
     bf16x8 hi[KS_H][NET_GROUPS], lo[KS_H][NET_GROUPS];
     hidden_to_frags<KS_H>(acc, hi, lo);
     net_layer<KS_H, MT_H>(frag + FR2, bias + BI2, hi, lo, acc, lane);
