@@ -59,17 +59,44 @@ __device__ __forceinline__ int3 cell_of(float3 r, float box, const CellGeom &g) 
     return make_int3((int)floorf(r.x / g.cs), (int)floorf(r.y / g.cs), (int)floorf(r.z / g.cs));
 }
 
+// The bead's slot within its bucket comes from one atomic per bucket and
+// wave: the wave's lanes are grouped by bucket (one round per distinct
+// bucket, scalar work), each group's first lane adds the group's size, and
+// the lanes of a group take consecutive slots from the returned count.
+// Neighbouring beads of a microtubule share cells, so most of a wave's
+// lanes fall in a few buckets.  The order of the beads within a bucket is
+// not fixed (nor was it with one atomic per bead); every user of the tables
+// orders what it finds there by bead index or by distance.
 __global__ void __launch_bounds__(256)
     k_cell_assign(DevTopo T, DevState S, CellTables C, float side, int K) {
     const int b = blockIdx.x * 256 + threadIdx.x;
-    if (b >= T.nb) return;
-    const int t = T.tid[b];
-    const float box = T.tbox[t];
-    const CellGeom g = cell_geom(box, side, K);
-    const int3 c = cell_of(f3(S.xyz[b]), box, g);
-    const int bucket = C.toff[t] + (int)(cell_hash(c.x, c.y, c.z) & (uint32_t)(C.tsize[t] - 1));
-    C.cell[b] = make_int4(c.x, c.y, c.z, bucket);
-    C.rank[b] = atomicAdd(&C.count[bucket], 1);   // the bead's slot within its bucket
+    const bool valid = b < T.nb;
+    int bucket = -1;
+    if (valid) {
+        const int t = T.tid[b];
+        const float box = T.tbox[t];
+        const CellGeom g = cell_geom(box, side, K);
+        const int3 c = cell_of(f3(S.xyz[b]), box, g);
+        bucket = C.toff[t] + (int)(cell_hash(c.x, c.y, c.z) & (uint32_t)(C.tsize[t] - 1));
+        C.cell[b] = make_int4(c.x, c.y, c.z, bucket);
+    }
+    const int lane = threadIdx.x & 63;
+    uint64_t todo = __ballot(valid), group = 0;
+    int leader = lane;
+    while (todo) {
+        const int l0 = __ffsll((unsigned long long)todo) - 1;
+        const int kb = __builtin_amdgcn_readlane(bucket, l0);
+        const uint64_t m = __ballot(valid && bucket == kb);
+        if (valid && bucket == kb) {
+            leader = l0;
+            group = m;
+        }
+        todo &= ~m;
+    }
+    int first = 0;
+    if (valid && lane == leader) first = atomicAdd(&C.count[bucket], __popcll(group));
+    first = __shfl(first, leader);
+    if (valid) C.rank[b] = first + __popcll(group & ((1ull << lane) - 1ull));
 }
 
 // Exclusive scan of the bucket counts over ALL trajectories at once.  Each
