@@ -358,7 +358,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int NET_WAVES = 8;                  // waves per workgroup (2 per SIMD)
+constexpr int NET_WAVES = 12;                 // waves per workgroup (3 per SIMD)
 constexpr int NET_THREADS = 64 * NET_WAVES;
 constexpr int NET_GROUPS = 2;                 // 16-bead column groups per wave iteration
 constexpr int NET_TILE = 16 * NET_GROUPS;     // beads per wave iteration
@@ -725,11 +725,9 @@ __global__ void __launch_bounds__(NET_THREADS)
         const uint32_t lim = feat_limits(fblk, n, tile * NET_TILE, lane);
         const int b = tile * NET_TILE + lane;
         const bool integ = lane < NET_TILE && b < n;
-        MlpIn in{};
-        if (integ) in = mlp_load(A, b);
         float y[MLP_OUT];
         net_tile(net_lds, bias, feat, lim, tile * NET_TILE, lane, y);
-        if (integ) mlp_integrate(A, b, y, in);
+        if (integ) mlp_integrate(A, b, y, mlp_load(A, b));
     }
 }
 
