@@ -156,8 +156,14 @@ __global__ void __launch_bounds__(EPS_GROUP)
     const int w = threadIdx.x / 64, l = threadIdx.x % 64;
     const int base = T.tbase[u.x], n = T.tsize[u.x], nb = T.nb;
     const int nch = (n + 63) / 64;
-    const float box = T.tbox[u.x], ibox = inv_side(box);
-    const float a = P.tea_a, ia = rcp_fast(P.tea_a);
+    // Lengths in the pair loop are in box units in a periodic box (positions
+    // x / box, so the minimum image is d - rint(d)) and in nm otherwise:
+    // sc is the unit's inverse, wf 1 where the image is taken.
+    const float box = T.tbox[u.x];
+    const bool wrap = PERIODIC && box > 0.f;
+    const float sc = wrap ? inv_side(box) : 1.f, wf = wrap ? 1.f : 0.f;
+    const float a = P.tea_a * sc, ia = rcp_fast(P.tea_a) * (wrap ? box : 1.f);
+    const float r2min = 1e-12f * sc * sc;
     const int cI = u.y * EPS_CHUNKS + w;
     const int li = cI * 64 + l;
     const bool vi = li < n;
@@ -165,6 +171,7 @@ __global__ void __launch_bounds__(EPS_GROUP)
     float wi = 0.f;
     if (vi) {
         xi = f3(S.xyz[base + li]);
+        xi = make_float3(xi.x * sc, xi.y * sc, xi.z * sc);
         wi = T.extra[base + li] ? 0.f : 1.f;
     }
     float ai[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
@@ -175,6 +182,9 @@ __global__ void __launch_bounds__(EPS_GROUP)
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
             if (lj < n) {
                 v = S.xyz[base + lj];
+                v.x *= sc;
+                v.y *= sc;
+                v.z *= sc;
                 v.w = T.extra[base + lj] ? 0.f : 1.f;
             }
             jx[threadIdx.x] = v;
@@ -192,8 +202,12 @@ __global__ void __launch_bounds__(EPS_GROUP)
         auto pair = [&](auto wt, float4 xj, float *t, float *t2) {
             constexpr bool WT = decltype(wt)::value;
             float3 d = make_float3(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z);
-            if (PERIODIC) d = min_image_inv(d, box, ibox);
-            const float r2 = fmaxf(d.x * d.x + d.y * d.y + d.z * d.z, 1e-12f);
+            if (PERIODIC) {
+                d.x = fmaf(-wf, rintf(d.x), d.x);
+                d.y = fmaf(-wf, rintf(d.y), d.y);
+                d.z = fmaf(-wf, rintf(d.z), d.z);
+            }
+            const float r2 = fmaxf(d.x * d.x + d.y * d.y + d.z * d.z, r2min);
             const float rinv = rsq_fast(r2);
             RpyCoef c = rpy_far(rinv, a);
             if (__any(r2 <= near2)) {
@@ -239,7 +253,7 @@ __global__ void __launch_bounds__(EPS_GROUP)
             if (!WEIGHTED || wi != 0.f) {
                 // the j == i slot entered as a coincident bead (d = 0):
                 // cii^2 into the first sum and 3 cii into the last, taken out
-                const float r2 = 1e-12f, rinv = rsq_fast(r2);
+                const float r2 = r2min, rinv = rsq_fast(r2);
                 const float cii = rpy_select(r2, rinv, a, ia).cii;
                 ai[0] = fmaf(-cii, cii, ai[0]);
                 ai[4] = fmaf(-3.f, cii, ai[4]);
