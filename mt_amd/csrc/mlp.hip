@@ -219,26 +219,42 @@ __global__ void __launch_bounds__(MLP_BLOCK)
             }
         }
         // One flat walk over the lane's candidates (buckets in order, each
-        // bucket's in slot order), the next one's load issued before this
-        // one is visited.  A wave then runs as many iterations as its lane
-        // with the most candidates, not the sum over buckets of each
-        // bucket's longest lane.
+        // bucket's in slot order), the loads two candidates ahead of the
+        // visits (pf[k % 2] holds candidate k; the cursor q, in range r,
+        // is the next candidate to request).  A wave then runs as many
+        // iterations as its lane with the most candidates, not the sum over
+        // buckets of each bucket's longest lane.
         int r = 0, q = 0, qend = 0;
         if (nne > 0) {
             const typename Ranges::W w = srg[threadIdx.x];
             q = Ranges::lo(w, base);
             qend = Ranges::hi(w, base);
         }
-        float4 pn = total > 0 ? C.spos[q] : make_float4(0.f, 0.f, 0.f, 0.f);
-        for (int k = 0; k < total; k++) {
-            const float4 pj = pn;
+        auto advance = [&]() {
             if (++q == qend && ++r < nne) {
                 const typename Ranges::W w = srg[r * MLP_BLOCK + threadIdx.x];
                 q = Ranges::lo(w, base);
                 qend = Ranges::hi(w, base);
             }
-            if (k + 1 < total) pn = C.spos[q];
-            visit(pj);
+        };
+        float4 pf[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+            if (u < total) {
+                pf[u] = C.spos[q];
+                advance();
+            }
+        for (int k = 0; k < total; k += 2) {
+#pragma unroll
+            for (int u = 0; u < 2; u++)
+                if (k + u < total) {
+                    const float4 pj = pf[u];
+                    if (k + u + 2 < total) {
+                        pf[u] = C.spos[q];
+                        advance();
+                    }
+                    visit(pj);
+                }
         }
     }
     if (feat) {
