@@ -127,6 +127,13 @@ struct GlobalLists {
     __device__ __forceinline__ int lj(int k) const { return T.lj[k * T.nb + b] - base; }
 };
 
+// Position of a bead's k-th LJ neighbour; a list format that stores
+// something other than the neighbour's index provides its own overload.
+template <class Acc, class Lists>
+__device__ __forceinline__ float4 lj_xyz(const Acc &A, const Lists &Ls, int k) {
+    return A.xyz(Ls.lj(k));
+}
+
 // LJ repulsion over the list; the cutoff test selects the force (no branch
 // around the accumulation), and periodic and open boxes get separate loops,
 // as do runs with and without the regularization cap (a uniform choice: the
@@ -134,7 +141,7 @@ struct GlobalLists {
 template <bool PERIODIC, bool REG, class Acc, class Lists>
 __device__ __forceinline__ void lj_sum(const mte_params &P, const Acc &A, const Lists &Ls, const Own &I, float F[6]) {
     const int nlj = Ls.nlj();
-    const float s0 = P.ljscale * P.ljsigma6;
+    const float s6 = 6.f * (P.ljscale * P.ljsigma6);
     // REGULARIZATION (src/compute_cuda.cu:482-486): the pair force s r is
     // capped at gammaR r_mon / dt, i.e. s at that over r (r_mon = 2 nm)
     const float cap = P.gammaR * 2.f / P.dt;
@@ -144,7 +151,8 @@ __device__ __forceinline__ void lj_sum(const mte_params &P, const Acc &A, const 
         if (PERIODIC) d = min_image(d, I.box);
         const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
         const float inv = rcp_fast(r2);
-        float s = r2 < LJ_CUT * LJ_CUT ? s0 * (6.f * (inv * inv) * (inv * inv)) : 0.f;
+        const float inv2 = inv * inv;
+        float s = r2 < LJ_CUT * LJ_CUT ? s6 * (inv2 * inv2) : 0.f;
         if (REG) s = fminf(s, cap * rsq_fast(r2));
         fx += s * d.x;
         fy += s * d.y;
@@ -155,14 +163,14 @@ __device__ __forceinline__ void lj_sum(const mte_params &P, const Acc &A, const 
     // list order
     int k = 0;
     for (; k + 4 <= nlj; k += 4) {
-        const int j0 = Ls.lj(k), j1 = Ls.lj(k + 1), j2 = Ls.lj(k + 2), j3 = Ls.lj(k + 3);
-        const float4 x0 = A.xyz(j0), x1 = A.xyz(j1), x2 = A.xyz(j2), x3 = A.xyz(j3);
+        const float4 x0 = lj_xyz(A, Ls, k), x1 = lj_xyz(A, Ls, k + 1), x2 = lj_xyz(A, Ls, k + 2),
+                     x3 = lj_xyz(A, Ls, k + 3);
         pair(x0);
         pair(x1);
         pair(x2);
         pair(x3);
     }
-    for (; k < nlj; k++) pair(A.xyz(Ls.lj(k)));
+    for (; k < nlj; k++) pair(lj_xyz(A, Ls, k));
     F[0] = fx;
     F[1] = fy;
     F[2] = fz;

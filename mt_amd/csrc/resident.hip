@@ -4,7 +4,8 @@
 // One workgroup owns one trajectory.  LDS holds, per bead, the position and
 // the three rotated offsets with the Euler angles packed into their spare
 // lanes (64 B), plus the bead's neighbour lists compacted into a uint16 CSR
-// (local index << 1 | flag) whose places are sized for the full lists, so a
+// (bonds: local index << 1 | flag; LJ: 16 x local index, the neighbour's
+// byte offset among the positions) whose places are sized for the full lists, so a
 // distance-filtered list and its later refill share one layout.  Each
 // thread owns PT beads and keeps their two HybridTaus states in registers.
 // A launch runs every step up to the next
@@ -60,8 +61,8 @@ __device__ __forceinline__ int pack_counts(int h, int nlon, int nlat, int nlj) {
     return (h ? 1 : 0) | (nlon << 1) | (nlat << 5) | (nlj << 10);
 }
 
-// Lists compacted in LDS: packed entries with local indices, the encoding
-// bead_force reads through LdsAcc.
+// Lists compacted in LDS: packed entries with local indices (LJ entries as
+// byte offsets), the encoding bead_force reads through LdsAcc.
 struct LdsLists {
     const uint16_t *e;
     int off, cnt;
@@ -74,6 +75,11 @@ struct LdsLists {
     __device__ __forceinline__ int nlj() const { return (cnt >> 10) & 1023; }
     __device__ __forceinline__ int lj(int k) const { return e[off + h() + nlon() + nlat() + k]; }
 };
+// LJ entries in LDS hold the neighbour's byte offset in the position array
+// (16 times its local index), so reading its position takes no scaling.
+__device__ __forceinline__ float4 lj_xyz(const LdsAcc &A, const LdsLists &Ls, int k) {
+    return *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(A.x) + Ls.lj(k));
+}
 
 // Writes a bead's new state to LDS; returns (sin theta, cos theta, sin psi,
 // cos psi) for the owner's next force evaluation.
@@ -344,7 +350,7 @@ __global__ void __launch_bounds__(MAXT)
                     if (k < nbond) {
                         ent[o++] = (uint16_t)((((v[u] >> 1) - base) << 1) | (v[u] & 1));
                     } else if (k < nall && lj_keep(i, v[u] - base, rf2)) {
-                        ent[o++] = (uint16_t)(v[u] - base);
+                        ent[o++] = (uint16_t)((v[u] - base) * (int)sizeof(float4));
                         nlj++;
                     }
                 }
@@ -414,7 +420,7 @@ bool resident_plan_for(const mte_engine *e, int pt, ResidentPlan *plan) {
     if (e->P.tea_on || pt < 1 || pt > RES_MAX_PT) return false;
     int nmax = 0;
     for (int t = 0; t < e->ntraj; t++) nmax = std::max<int>(nmax, (int)(e->off[t + 1] - e->off[t]));
-    if (nmax >= 32768) return false;   // uint16 list entries
+    if (nmax > 4096) return false;   // uint16 list entries (LJ entries: 16 x the local index)
     int lds_limit = 0;
     if (hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device) != hipSuccess)
         return false;
