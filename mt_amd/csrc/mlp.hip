@@ -185,24 +185,24 @@ __global__ void __launch_bounds__(MLP_BLOCK)
             if (kn < key[MLP_NN - 1]) {
                 // slots in chunks of 8, a chunk only where some lane of the
                 // wave holds a key in it (the rest are all ones and stay so):
-                // a dilute wave touches one chunk
+                // a dilute wave touches one chunk.  Sorted keys: slot k keeps
+                // its key if that is below kn, else takes kn when the key
+                // below is below kn, else the key below; from the top down,
+                // each comparison serves two slots.
                 constexpr int CH = 8;
                 bool used[MLP_NN / CH];
 #pragma unroll
                 for (int c = 0; c < MLP_NN / CH; c++) used[c] = c == 0 || __any(cnt >= CH * c);
-                int p = 0;
-#pragma unroll
-                for (int c = 0; c < MLP_NN / CH; c++)
-                    if (used[c]) {
-#pragma unroll
-                        for (int k = CH * c; k < CH * (c + 1); k++) p += key[k] < kn ? 1 : 0;
-                    }
 #pragma unroll
                 for (int c = MLP_NN / CH - 1; c >= 0; c--)
                     if (used[c]) {
 #pragma unroll
-                        for (int k = CH * (c + 1) - 1; k >= CH * c; k--)
-                            key[k] = k > p ? key[k > 0 ? k - 1 : 0] : (k == p ? kn : key[k]);
+                        for (int k = CH * (c + 1) - 1; k >= CH * c; k--) {
+                            const bool keep = key[k] < kn;
+                            const uint64_t below = k > 0 ? key[k - 1] : kn;
+                            const bool below_lt = k > 0 && below < kn;
+                            key[k] = keep ? key[k] : (below_lt ? kn : below);
+                        }
                     }
                 cnt = min(cnt + 1, MLP_NN);
             }
