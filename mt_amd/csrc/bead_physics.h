@@ -17,7 +17,12 @@ struct Own {
     float2 uth;
     float fi, theta, psi;
     float box;   // periodic side of the bead's trajectory (0: open)
+    float ibox;  // inv_side(box)
 };
+__device__ __forceinline__ void set_box(Own &o, float L) {
+    o.box = L;
+    o.ibox = inv_side(L);
+}
 
 // trig = (sin theta, cos theta, sin psi, cos psi) of the bead.
 __device__ __forceinline__ Own make_own_trig(float4 xyz, float4 ang, float4 fn, float4 fp1, float4 fp2, float4 trig) {
@@ -31,7 +36,7 @@ __device__ __forceinline__ Own make_own_trig(float4 xyz, float4 ang, float4 fn, 
     o.psi = ang.z;
     o.uphi = make_float3(trig.w * trig.y, trig.z * trig.y, -trig.x);
     o.uth = make_float2(-trig.z, trig.w);
-    o.box = 0.f;
+    o.box = o.ibox = 0.f;
     return o;
 }
 
@@ -49,7 +54,7 @@ __device__ __forceinline__ Own make_own(float4 xyz, float4 ang, float4 fn, float
     sincos_fast(ang.z, &sp, &cp);
     o.uphi = make_float3(cp * ct, sp * ct, -st);
     o.uth = make_float2(-sp, cp);
-    o.box = 0.f;
+    o.box = o.ibox = 0.f;
     return o;
 }
 
@@ -148,7 +153,7 @@ __device__ __forceinline__ void lj_sum(const mte_params &P, const Acc &A, const 
     float fx = F[0], fy = F[1], fz = F[2];   // same accumulation order, scalar registers
     auto pair = [&](float4 xj) {
         float3 d = sub3(I.r, f3(xj));
-        if (PERIODIC) d = min_image(d, I.box);
+        if (PERIODIC) d = min_image_inv(d, I.box, I.ibox);   // called with I.box > 0
         const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
         const float inv = rcp_fast(r2);
         const float inv2 = inv * inv;
@@ -195,7 +200,7 @@ __device__ __forceinline__ void bead_force(const mte_params &P, const Acc &A, co
         const int j = ent_j(h);
         const float rm = ent_f(h) ? RMON : -RMON;
         const float3 nj = f3(A.fn(j));
-        const Grad G = gap_grad(min_image(sub3(I.r, f3(A.xyz(j))), I.box), scale3(rm, I.n), scale3(-rm, nj), I.uphi, I.uth);
+        const Grad G = gap_grad(min_image_pre(sub3(I.r, f3(A.xyz(j))), I.box, I.ibox), scale3(rm, I.n), scale3(-rm, nj), I.uphi, I.uth);
 #pragma unroll
         for (int q = 0; q < 6; q++) F[q] -= P.C * G.g[q];
         if (G.dr < ANGLE_CUT) {
@@ -210,7 +215,7 @@ __device__ __forceinline__ void bead_force(const mte_params &P, const Acc &A, co
         const int j = ent_j(e);
         const float rm = ent_f(e) ? RMON : -RMON;
         const float4 xj = A.xyz(j);
-        const Grad G = gap_grad(min_image(sub3(I.r, f3(xj)), I.box), scale3(rm, I.n), scale3(-rm, f3(A.fn(j))), I.uphi, I.uth);
+        const Grad G = gap_grad(min_image_pre(sub3(I.r, f3(xj)), I.box, I.ibox), scale3(rm, I.n), scale3(-rm, f3(A.fn(j))), I.uphi, I.uth);
         float dUdr = (G.dr == 0.f) ? 0.f : dmorse(P.D_long, P.A_long, G.dr) * G.inv;
         if (P.barrier && A.offtub(b) && A.offtub(j) && G.dr != 0.f)
             dUdr += dbarr(P.a_barr_long, P.r_barr_long, P.w_barr_long, G.dr) * G.inv;
@@ -230,7 +235,7 @@ __device__ __forceinline__ void bead_force(const mte_params &P, const Acc &A, co
         const int sw = ent_f(e);
         const float3 a = sw ? I.a1 : I.a2;
         const float3 bj = f3(sw ? A.fp2(j) : A.fp1(j));
-        const Grad G = gap_grad(min_image(sub3(I.r, f3(A.xyz(j))), I.box), a, bj, I.uphi, I.uth);
+        const Grad G = gap_grad(min_image_pre(sub3(I.r, f3(A.xyz(j))), I.box, I.ibox), a, bj, I.uphi, I.uth);
         // seam bonds (monomer types differ) are weakened by seam_coeff (:447-452)
         const float Dl = A.mon(b) != A.mon(j) ? P.D_lat / P.seam_coeff : P.D_lat;
         float dUdr = (G.dr == 0.f) ? 0.f : dmorse(Dl, P.A_lat, G.dr) * G.inv;
@@ -271,7 +276,7 @@ __device__ __forceinline__ void bead_energy(const mte_params &P, const DevTopo &
     if (h >= 0) {
         const int j = ent_j(h);
         const float rm = ent_f(h) ? RMON : -RMON;
-        const Grad G = gap_grad(min_image(sub3(I.r, f3(A.xyz(j))), I.box), scale3(rm, I.n), scale3(-rm, f3(A.fn(j))), I.uphi, I.uth);
+        const Grad G = gap_grad(min_image_pre(sub3(I.r, f3(A.xyz(j))), I.box, I.ibox), scale3(rm, I.n), scale3(-rm, f3(A.fn(j))), I.uphi, I.uth);
         U[0] += (P.C / 2.f) * G.dr * G.dr;
         if (G.dr < ANGLE_CUT) {
             const float4 aj = A.ang(j);
@@ -287,7 +292,7 @@ __device__ __forceinline__ void bead_energy(const mte_params &P, const DevTopo &
         const int j = ent_j(e);
         const float rm = ent_f(e) ? RMON : -RMON;
         const float4 xj = A.xyz(j);
-        const Grad G = gap_grad(min_image(sub3(I.r, f3(xj)), I.box), scale3(rm, I.n), scale3(-rm, f3(A.fn(j))), I.uphi, I.uth);
+        const Grad G = gap_grad(min_image_pre(sub3(I.r, f3(xj)), I.box, I.ibox), scale3(rm, I.n), scale3(-rm, f3(A.fn(j))), I.uphi, I.uth);
         U[1] += morse_en(P.D_long, P.A_long, G.dr);
         if (P.barrier && A.offtub(b) && A.offtub(j))
             U[1] += barr(P.a_barr_long, P.r_barr_long, P.w_barr_long, G.dr);
@@ -306,7 +311,7 @@ __device__ __forceinline__ void bead_energy(const mte_params &P, const DevTopo &
         const int e = T.lat[k * nb + b];
         const int j = ent_j(e);
         const int sw = ent_f(e);
-        const Grad G = gap_grad(min_image(sub3(I.r, f3(A.xyz(j))), I.box), sw ? I.a1 : I.a2, f3(sw ? A.fp2(j) : A.fp1(j)),
+        const Grad G = gap_grad(min_image_pre(sub3(I.r, f3(A.xyz(j))), I.box, I.ibox), sw ? I.a1 : I.a2, f3(sw ? A.fp2(j) : A.fp1(j)),
                                 I.uphi, I.uth);
         if (A.mon(b) != A.mon(j)) U[2] += morse_en(P.D_lat / P.seam_coeff, P.A_lat, G.dr);
         else U[2] += morse_en(P.D_lat, P.A_lat, G.dr);
@@ -317,7 +322,7 @@ __device__ __forceinline__ void bead_energy(const mte_params &P, const DevTopo &
         const int nlj = T.lj_n[b];
         for (int k = 0; k < nlj; k++) {
             const int j = T.lj[k * nb + b];
-            const float3 d = min_image(sub3(I.r, f3(A.xyz(j))), I.box);
+            const float3 d = min_image_pre(sub3(I.r, f3(A.xyz(j))), I.box, I.ibox);
             const float dx = d.x, dy = d.y, dz = d.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             if (sqrtf(r2) < LJ_CUT) {

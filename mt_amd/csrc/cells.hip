@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(256) k_cell_scatter(DevTopo T, DevState S, Cel
 __device__ int gather_candidates(const DevTopo &T, const DevState &S, const CellTables &C, int b, float radius,
                                  int K, int *cand) {
     const int t = T.tid[b], base = T.tbase[t];
-    const float box = T.tbox[t];
+    const float box = T.tbox[t], ibox = inv_side(box);
     const CellGeom g = cell_geom(box, radius / K, K);
     const int4 me = C.cell[b];
     const float3 ri = f3(S.xyz[b]);
@@ -234,7 +234,7 @@ __device__ int gather_candidates(const DevTopo &T, const DevState &S, const Cell
                     if (j == b) continue;
                     const int4 cj = C.cell[j];
                     if (cj.x != cx || cj.y != cy || cj.z != cz) continue;
-                    const float3 d = min_image(sub3(ri, f3(S.xyz[j])), box);
+                    const float3 d = min_image_pre(sub3(ri, f3(S.xyz[j])), box, ibox);
                     if (d.x * d.x + d.y * d.y + d.z * d.z >= r2max) continue;
                     if (n >= CAND_CAP) return -1;
                     // insertion keeps the candidates sorted by index

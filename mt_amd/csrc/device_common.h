@@ -87,6 +87,12 @@ __device__ __forceinline__ float3 min_image_inv(float3 d, float L, float iL) {
     return d;
 }
 __device__ __forceinline__ float inv_side(float L) { return L > 0.f ? 1.f / L : 0.f; }
+// min_image with iL = inv_side(L) taken by the caller (outside its loops):
+// the same values, without a division per call
+__device__ __forceinline__ float3 min_image_pre(float3 d, float L, float iL) {
+    if (L > 0.f) d = min_image_inv(d, L, iL);
+    return d;
+}
 __device__ __forceinline__ float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
 
 // Periodic cell index: i in [-dims, 2 dims) wrapped into [0, dims) (a

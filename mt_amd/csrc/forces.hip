@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(BS)
         return;
     }
     Own I = make_own(x, a, Sin.fn[b], Sin.fp1[b], Sin.fp2[b]);
-    I.box = T.tbox[T.tid[b]];
+    set_box(I, T.tbox[T.tid[b]]);
     float F[6];
     bead_force(P, GlobalAcc{Sin, T}, GlobalLists{T, b, 0}, b, P.is_wall ? T.zs[T.tid[b]] : 0.f, I, F);
     float4 rx, ra;
@@ -113,7 +113,7 @@ __global__ void __launch_bounds__(BS)
     float F[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (!T.extra[b]) {
         Own I = make_own(S.xyz[b], S.ang[b], S.fn[b], S.fp1[b], S.fp2[b]);
-        I.box = T.tbox[T.tid[b]];
+        set_box(I, T.tbox[T.tid[b]]);
         bead_force(P, GlobalAcc{S, T}, GlobalLists{T, b, 0}, b, P.is_wall ? T.zs[T.tid[b]] : 0.f, I, F);
     }
     fxyz[b] = make_float4(F[0], F[1], F[2], 0.f);
@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(FORCE_TILE)
     if (!T.extra[b]) {
         const float4 fn = ln[i], f1 = l1[i], f2 = l2[i];
         Own I = make_own(lx[i], make_float4(fn.w, f1.w, f2.w, 0.f), fn, f1, f2);
-        I.box = T.tbox[t];
+        set_box(I, T.tbox[t]);
         bead_force(P, LdsAcc{lx, ln, l1, l2}, GlobalLists{T, b, base}, i, P.is_wall ? T.zs[t] : 0.f, I, F);
     }
     fxyz[b] = make_float4(F[0], F[1], F[2], 0.f);
@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(BS)
     float U[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (!T.extra[b]) {
         Own I = make_own(S.xyz[b], S.ang[b], S.fn[b], S.fp1[b], S.fp2[b]);
-        I.box = T.tbox[T.tid[b]];
+        set_box(I, T.tbox[T.tid[b]]);
         bead_energy(P, T, GlobalAcc{S, T}, b, I, U);
     }
 #pragma unroll

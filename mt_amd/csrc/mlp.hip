@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(MLP_BLOCK)
     uint64_t key[MLP_NN];
 #pragma unroll
     for (int k = 0; k < MLP_NN; k++) key[k] = ~0ull;
-    const float box = T.tbox[t];
+    const float box = T.tbox[t], ibox = inv_side(box);
     const float3 ri = f3(S.xyz[b]);
     float A[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     int cnt = 0;
@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(MLP_BLOCK)
         auto visit = [&](const float4 pj) {
             const int jw = __float_as_int(pj.w);
             const int j = jw & 0x7fffffff;
-            const float3 d = min_image(make_float3(pj.x - ri.x, pj.y - ri.y, pj.z - ri.z), box);
+            const float3 d = min_image_pre(make_float3(pj.x - ri.x, pj.y - ri.y, pj.z - ri.z), box, ibox);
             const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
             // extra beads, this bead and beads beyond the radius drop out
             if (jw < 0 || j == b || r2 >= MLP_RC * MLP_RC) return;
@@ -278,8 +278,9 @@ __global__ void __launch_bounds__(MLP_BLOCK)
             for (int u = 0; u < FEAT_BLK_NN; u++) {
                 float3 d = make_float3(0.f, 0.f, 0.f);
                 if (k0 + u < cnt) {
-                    d = min_image(sub3(f3(xj[u]), ri), box);
-                    d = make_float3(d.x / MLP_RC, d.y / MLP_RC, d.z / MLP_RC);
+                    d = min_image_pre(sub3(f3(xj[u]), ri), box, ibox);
+                    constexpr float irc = 1.f / MLP_RC;   // features in units of the radius
+                    d = make_float3(d.x * irc, d.y * irc, d.z * irc);
                 }
                 v[3 * u] = d.x;
                 v[3 * u + 1] = d.y;

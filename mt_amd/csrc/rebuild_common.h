@@ -41,6 +41,7 @@ struct RebuildOwn {
     float3 ri, ani, a1, a2;   // position, R_MON n_i, R p1, R p2
     float rp;                  // R_MON used by pairs_kernel
     float box;                 // periodic side (0: open)
+    float ibox;                // inv_side(box)
     int mti, hraw, hpart, li, b, base;
     int nlon, nlat, nlj, ovf;
 };
@@ -49,6 +50,7 @@ __device__ __forceinline__ RebuildOwn rebuild_own(float4 x, float4 n, float4 p1,
                                                   int b, int base, float box) {
     RebuildOwn o;
     o.box = box;
+    o.ibox = inv_side(box);
     o.ri = f3(x);
     // pairs_kernel takes R_MON and the excluded partner from the first
     // harmonic slot, including the -1 fill of partnerless beads (:548-551, :565).
@@ -71,7 +73,7 @@ __device__ __forceinline__ void rebuild_pair(const DevTopo &T, RebuildOwn &o, in
                                              float4 p1j, float4 p2j, int mtj, float ljcut, bool do_lj,
                                              bool do_bonded) {
     const int nb = T.nb;
-    const float3 dij = min_image(sub3(o.ri, f3(xj)), o.box);
+    const float3 dij = min_image_pre(sub3(o.ri, f3(xj)), o.box, o.ibox);
     const float d2 = dij.x * dij.x + dij.y * dij.y + dij.z * dij.z;
     if (do_lj && below_cut(d2, ljcut) && o.li != j) {
         if (o.nlj < T.max_lj) T.lj[o.nlj * nb + o.b] = o.base + j;

@@ -133,7 +133,7 @@ __device__ __forceinline__ int run_steps(const mte_params &P, const DevTopo &T, 
             float4 x = lx[i];
             float4 a = make_float4(ln[i].w, l1[i].w, l2[i].w, 0.f);
             Own I = make_own_trig(x, a, ln[i], l1[i], l2[i], trig[k]);
-            I.box = box;
+            set_box(I, box);
             float F[6];
             const Lists Ls = make_lists(i, b);
             bead_force(P, A, Ls, i, wall_z, I, F);
@@ -256,12 +256,12 @@ __global__ void __launch_bounds__(MAXT)
     }
     __syncthreads();
     const LdsAcc A{lx, ln, l1, l2};
-    const float box = T.tbox[t];
+    const float box = T.tbox[t], ibox = inv_side(box);
     // LJ entries of bead i kept in LDS: all of them (rf2 == 0), or those
     // within sqrt(rf2) of it now.
-    auto lj_keep = [lx, box](int i, int j, float rf2) {
+    auto lj_keep = [lx, box, ibox](int i, int j, float rf2) {
         if (rf2 == 0.f) return true;
-        const float3 d = min_image(sub3(f3(lx[i]), f3(lx[j])), box);
+        const float3 d = min_image_pre(sub3(f3(lx[i]), f3(lx[j])), box, ibox);
         return d.x * d.x + d.y * d.y + d.z * d.z < rf2;
     };
     // Offsets of this trajectory's full lists in LDS, for the mobile beads
