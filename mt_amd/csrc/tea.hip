@@ -134,24 +134,28 @@ __global__ void __launch_bounds__(TEA_TILE)
 // Epsilon partials and ci with each pair evaluated once.  The five sums of
 // k_tea_eps are even in e (cii^2, crr (2 cii + crr) e_a^2, 3 cii + crr
 // (ex + ey + ez)^2), so a pair adds the same terms to both beads.  Work unit:
-// (trajectory, 1024-bead group I, group J >= I), one workgroup of 16 waves;
-// wave w takes chunk c_I = 16 I + w (64 beads) against each 64-bead chunk c_J
-// of group J in turn, staged in LDS:
-//   c_J > c_I  both sides, lane l meeting j = (l + s) mod 64 at step s, the
-//              j-side sums carried along by a one-lane rotation (wave_rol1),
-//   c_J = c_I  one-sided over the chunk, the j == i term taken out after,
-//   c_J < c_I  nothing (that pair of chunks is the wave c_J's).
+// (trajectory, 1024-bead group I, group J >= I), one workgroup of 8 waves;
+// wave w takes chunks c = 16 I + w and 16 I + w + 8 (64 beads each, one bead
+// of each per lane) against each 64-bead chunk c_J of group J in turn,
+// staged in LDS:
+//   c_J > c    both sides, lane l meeting j = (l + s) mod 64 at step s, the
+//              j-side sums carried along by a one-lane rotation (wave_rol1);
+//              when c_J lies above both chunks, the lane's two beads meet
+//              the same j together and share the rotation,
+//   c_J = c    one-sided over the chunk, the j == i term taken out after,
+//   c_J < c    nothing (that pair of chunks is the wave c_J's).
 // The waves' j-side sums of a chunk are added in wave order in LDS.  Output
 // (deterministic, no atomics): i-side sums of a bead of group H into
 // part[0][J][.] for J >= H, j-side sums into part[1][I][.] for I <= H; the
 // pair weight is 1 for two non-extra beads and 0 otherwise (WEIGHTED: some
 // bead of the batch is extra; without, every weight is 1 and none is formed).
-constexpr int EPS_CHUNKS = EPS_GROUP / 64;
+constexpr int EPS_CHUNKS = EPS_GROUP / 64;   // 64-bead chunks of a group
+constexpr int EPS_WAVES = EPS_CHUNKS / 2;    // waves of a workgroup, two chunks each
 template <bool PERIODIC, bool WEIGHTED>
-__global__ void __launch_bounds__(EPS_GROUP)
+__global__ void __launch_bounds__(64 * EPS_WAVES)
     k_tea_eps_sym(mte_params P, DevTopo T, DevState S, const int4 *units, int ngrp, float *part) {
-    __shared__ float4 jx[2 * 64];               // chunk c_J: x, weight; twice, no wrap
-    __shared__ float jred[EPS_CHUNKS][5][64];   // j-side sums of each wave
+    __shared__ float4 jx[2 * 64];              // chunk c_J: x, weight; twice, no wrap
+    __shared__ float jred[EPS_WAVES][5][64];   // j-side sums of each wave
     const int4 u = units[blockIdx.x];
     const int w = threadIdx.x / 64, l = threadIdx.x % 64;
     const int base = T.tbase[u.x], n = T.tsize[u.x], nb = T.nb;
@@ -164,18 +168,76 @@ __global__ void __launch_bounds__(EPS_GROUP)
     const float sc = wrap ? inv_side(box) : 1.f, wf = wrap ? 1.f : 0.f;
     const float a = P.tea_a * sc, ia = rcp_fast(P.tea_a) * (wrap ? box : 1.f);
     const float r2min = 1e-12f * sc * sc;
-    const int cI = u.y * EPS_CHUNKS + w;
-    const int li = cI * 64 + l;
-    const bool vi = li < n;
-    float3 xi = make_float3(0.f, 0.f, 0.f);
-    float wi = 0.f;
-    if (vi) {
-        xi = f3(S.xyz[base + li]);
-        xi = make_float3(xi.x * sc, xi.y * sc, xi.z * sc);
-        wi = T.extra[base + li] ? 0.f : 1.f;
+    // the lane's bead in each of the wave's two chunks
+    int cI[2], li[2];
+    bool vi[2];
+    float3 xi[2];
+    float wi[2], ai[2][5];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        cI[h] = u.y * EPS_CHUNKS + w + h * EPS_WAVES;
+        li[h] = cI[h] * 64 + l;
+        vi[h] = li[h] < n;
+        xi[h] = make_float3(0.f, 0.f, 0.f);
+        wi[h] = 0.f;
+        if (vi[h]) {
+            const float3 x = f3(S.xyz[base + li[h]]);
+            xi[h] = make_float3(x.x * sc, x.y * sc, x.z * sc);
+            wi[h] = T.extra[base + li[h]] ? 0.f : 1.f;
+        }
+#pragma unroll
+        for (int k = 0; k < 5; k++) ai[h][k] = 0.f;
     }
-    float ai[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
     const int c0 = u.z * EPS_CHUNKS, c1 = min(c0 + EPS_CHUNKS, nch);
+    // one pair's five terms p: added to the sums t; with t2 (both sides),
+    // t2 += p, after t2 takes the next lane's t2 when ROT (the j-side
+    // rotation, done with the first of the lane's beads to meet this j).  A
+    // wave with no pair inside 2a (margin over the rounding of r / a) keeps
+    // the far coefficients, which is the branch rpy_select picks for each of
+    // its pairs.  WT: the pair weights are formed (extra beads in the batch,
+    // or the padding of a trajectory's last chunk, whose slots have weight 0).
+    const float near2 = (2.002f * a) * (2.002f * a);
+    auto pair = [&](auto wt, auto rot, const float3 &xa, float wa, float4 xj, float *t, float *t2) {
+        constexpr bool WT = decltype(wt)::value, ROT = decltype(rot)::value;
+        float3 d = make_float3(xj.x - xa.x, xj.y - xa.y, xj.z - xa.z);
+        if (PERIODIC) {
+            d.x = fmaf(-wf, rintf(d.x), d.x);
+            d.y = fmaf(-wf, rintf(d.y), d.y);
+            d.z = fmaf(-wf, rintf(d.z), d.z);
+        }
+        const float r2 = fmaxf(d.x * d.x + d.y * d.y + d.z * d.z, r2min);
+        const float rinv = rsq_fast(r2);
+        RpyCoef c = rpy_far(rinv, a);
+        if (__any(r2 <= near2)) {
+            __asm__ volatile("");   // a branch: no speculation of the near form
+            const float ra = r2 * rinv * ia;
+            const bool nr = !(ra > 2.f);
+            c.cii = nr ? fmaf(-0.28125f, ra, 1.f) : c.cii;
+            c.crr = nr ? 0.09375f * ra : c.crr;
+        }
+        const float ww = WT ? wa * xj.w : 1.f;
+        const float ex = d.x * rinv, ey = d.y * rinv, ez = d.z * rinv;
+        const float cw = WT ? ww * c.cii : c.cii;
+        const float wcrr = WT ? ww * c.crr : c.crr;
+        const float k = wcrr * fmaf(2.f, c.cii, c.crr);
+        const float se = ex + ey + ez;
+        const float tall = fmaf(wcrr, se * se, 3.f * cw);
+        const float qx = ex * ex, qy = ey * ey, qz = ez * ez;
+        t[0] = fmaf(cw, c.cii, t[0]);
+        t[1] = fmaf(k, qx, t[1]);
+        t[2] = fmaf(k, qy, t[2]);
+        t[3] = fmaf(k, qz, t[3]);
+        t[4] += tall;
+        if (t2) {
+            t2[0] = fmaf(cw, c.cii, ROT ? wave_rol1(t2[0]) : t2[0]);
+            t2[1] = fmaf(k, qx, ROT ? wave_rol1(t2[1]) : t2[1]);
+            t2[2] = fmaf(k, qy, ROT ? wave_rol1(t2[2]) : t2[2]);
+            t2[3] = fmaf(k, qz, ROT ? wave_rol1(t2[3]) : t2[3]);
+            t2[4] = ROT ? wave_rol1_add(t2[4], tall) : t2[4] + tall;
+        }
+    };
+    const std::true_type yes{};
+    const std::false_type no{};
     for (int cJ = c0; cJ < c1; cJ++) {
         if (threadIdx.x < 128) {
             const int lj = cJ * 64 + (threadIdx.x % 64);
@@ -191,75 +253,53 @@ __global__ void __launch_bounds__(EPS_GROUP)
         }
         __syncthreads();
         float gj[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-        // one pair's five terms p: added to the sums t; with t2 (both
-        // sides), t2 = (t2 of the next lane) + p, the j-side rotation taken
-        // with the addition.  A wave with no pair inside 2a (margin over the
-        // rounding of r / a) keeps the far coefficients, which is the branch
-        // rpy_select picks for each of its pairs.  WT: the pair weights
-        // are formed (extra beads in the batch, or the padding of a
-        // trajectory's last chunk, whose slots have weight 0).
-        const float near2 = (2.002f * a) * (2.002f * a);
-        auto pair = [&](auto wt, float4 xj, float *t, float *t2) {
-            constexpr bool WT = decltype(wt)::value;
-            float3 d = make_float3(xj.x - xi.x, xj.y - xi.y, xj.z - xi.z);
-            if (PERIODIC) {
-                d.x = fmaf(-wf, rintf(d.x), d.x);
-                d.y = fmaf(-wf, rintf(d.y), d.y);
-                d.z = fmaf(-wf, rintf(d.z), d.z);
-            }
-            const float r2 = fmaxf(d.x * d.x + d.y * d.y + d.z * d.z, r2min);
-            const float rinv = rsq_fast(r2);
-            RpyCoef c = rpy_far(rinv, a);
-            if (__any(r2 <= near2)) {
-                __asm__ volatile("");   // a branch: no speculation of the near form
-                const float ra = r2 * rinv * ia;
-                const bool nr = !(ra > 2.f);
-                c.cii = nr ? fmaf(-0.28125f, ra, 1.f) : c.cii;
-                c.crr = nr ? 0.09375f * ra : c.crr;
-            }
-            const float ww = WT ? wi * xj.w : 1.f;
-            const float ex = d.x * rinv, ey = d.y * rinv, ez = d.z * rinv;
-            const float cw = WT ? ww * c.cii : c.cii;
-            const float wcrr = WT ? ww * c.crr : c.crr;
-            const float k = wcrr * fmaf(2.f, c.cii, c.crr);
-            const float se = ex + ey + ez;
-            const float tall = fmaf(wcrr, se * se, 3.f * cw);
-            const float qx = ex * ex, qy = ey * ey, qz = ez * ez;
-            t[0] = fmaf(cw, c.cii, t[0]);
-            t[1] = fmaf(k, qx, t[1]);
-            t[2] = fmaf(k, qy, t[2]);
-            t[3] = fmaf(k, qz, t[3]);
-            t[4] += tall;
-            if (t2) {
-                t2[0] = fmaf(cw, c.cii, wave_rol1(t2[0]));
-                t2[1] = fmaf(k, qx, wave_rol1(t2[1]));
-                t2[2] = fmaf(k, qy, wave_rol1(t2[2]));
-                t2[3] = fmaf(k, qz, wave_rol1(t2[3]));
-                t2[4] = wave_rol1_add(t2[4], tall);
-            }
-        };
-        if (cI < nch && cJ > cI) {
-            // step s: lane l meets j = (l + s) mod 64, its gj the sums of
-            // that j (rotated in from lane l + 1, which held j at step s - 1)
-            // (chunk c_I < c_J is full; c_J may be the padded last one)
-            if (WEIGHTED || (cJ + 1) * 64 > n)
-                for (int s = 0; s < 64; s++) pair(std::true_type{}, jx[l + s], ai, gj);
+        // step s of a both-sided sweep: lane l meets j = (l + s) mod 64, its
+        // gj the sums of that j (rotated in from lane l + 1, which held j at
+        // the sweep's step s - 1; sweeps in a row continue the rotation).
+        // A chunk below c_J is full; c_J may be the padded last one.
+        const bool two0 = cI[0] < nch && cJ > cI[0], two1 = cI[1] < nch && cJ > cI[1];
+        const bool wtj = WEIGHTED || (cJ + 1) * 64 > n;
+        if (two0 && two1) {
+            if (wtj)
+                for (int s = 0; s < 64; s++) {
+                    const float4 xj = jx[l + s];
+                    pair(yes, yes, xi[0], wi[0], xj, ai[0], gj);
+                    pair(yes, no, xi[1], wi[1], xj, ai[1], gj);
+                }
             else
-                for (int s = 0; s < 64; s++) pair(std::false_type{}, jx[l + s], ai, gj);
-        } else if (cI < nch && cJ == cI) {
-            // one-sided over the chunk's beads: no padding slot is met
-            const int jn = min(64, n - cJ * 64);
-            for (int q = 0; q < jn; q++) pair(std::integral_constant<bool, WEIGHTED>{}, jx[q], ai, nullptr);
-            if (!WEIGHTED || wi != 0.f) {
-                // the j == i slot entered as a coincident bead (d = 0):
-                // cii^2 into the first sum and 3 cii into the last, taken out
-                const float r2 = r2min, rinv = rsq_fast(r2);
-                const float cii = rpy_select(r2, rinv, a, ia).cii;
-                ai[0] = fmaf(-cii, cii, ai[0]);
-                ai[4] = fmaf(-3.f, cii, ai[4]);
-            }
+                for (int s = 0; s < 64; s++) {
+                    const float4 xj = jx[l + s];
+                    pair(no, yes, xi[0], wi[0], xj, ai[0], gj);
+                    pair(no, no, xi[1], wi[1], xj, ai[1], gj);
+                }
+        } else {
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+                if (h == 0 ? two0 : two1) {
+                    if (wtj)
+                        for (int s = 0; s < 64; s++) pair(yes, yes, xi[h], wi[h], jx[l + s], ai[h], gj);
+                    else
+                        for (int s = 0; s < 64; s++) pair(no, yes, xi[h], wi[h], jx[l + s], ai[h], gj);
+                }
         }
-        // after step 63, lane l holds the sums of j = (l + 63) mod 64
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+            if (cI[h] < nch && cJ == cI[h]) {
+                // one-sided over the chunk's beads: no padding slot is met
+                const int jn = min(64, n - cJ * 64);
+                for (int q = 0; q < jn; q++)
+                    pair(std::integral_constant<bool, WEIGHTED>{}, no, xi[h], wi[h], jx[q], ai[h], nullptr);
+                if (!WEIGHTED || wi[h] != 0.f) {
+                    // the j == i slot entered as a coincident bead (d = 0):
+                    // cii^2 into the first sum and 3 cii into the last, taken out
+                    const float r2 = r2min, rinv = rsq_fast(r2);
+                    const float cii = rpy_select(r2, rinv, a, ia).cii;
+                    ai[h][0] = fmaf(-cii, cii, ai[h][0]);
+                    ai[h][4] = fmaf(-3.f, cii, ai[h][4]);
+                }
+            }
+        // after 64 (or 128) rotations lane l holds the sums of j = (l + 63)
+        // mod 64; with none, gj is zero
 #pragma unroll
         for (int k = 0; k < 5; k++) jred[w][k][(l + 63) & 63] = gj[k];
         __syncthreads();
@@ -268,16 +308,18 @@ __global__ void __launch_bounds__(EPS_GROUP)
             const int lj = cJ * 64 + q;
             if (lj < n) {
                 float sum = 0.f;
-                for (int v = 0; v < EPS_CHUNKS; v++) sum += jred[v][k][q];
+                for (int v = 0; v < EPS_WAVES; v++) sum += jred[v][k][q];
                 part[(((size_t)1 * ngrp + u.y) * 5 + k) * nb + base + lj] = sum;
             }
         }
         __syncthreads();
     }
-    if (vi) {
 #pragma unroll
-        for (int k = 0; k < 5; k++) part[(((size_t)0 * ngrp + u.z) * 5 + k) * nb + base + li] = ai[k];
-    }
+    for (int h = 0; h < 2; h++)
+        if (vi[h]) {
+#pragma unroll
+            for (int k = 0; k < 5; k++) part[(((size_t)0 * ngrp + u.z) * 5 + k) * nb + base + li[h]] = ai[h][k];
+        }
 }
 
 // Per bead: the group sums of k_tea_eps_sym added in fixed order into ci and
@@ -1064,7 +1106,7 @@ hipError_t launch_tea_eps(mte_engine *e) {
     if (e->tune.tea_eps == MTE_TEA_EPS_ONEPASS) {
         auto kern = e->periodic ? (e->any_extra ? k_tea_eps_sym<true, true> : k_tea_eps_sym<true, false>)
                                 : (e->any_extra ? k_tea_eps_sym<false, true> : k_tea_eps_sym<false, false>);
-        hipLaunchKernelGGL(kern, dim3(e->tea.eps_nunits), dim3(EPS_GROUP), 0, e->stream, e->P, e->T, e->S[e->cur],
+        hipLaunchKernelGGL(kern, dim3(e->tea.eps_nunits), dim3(64 * EPS_WAVES), 0, e->stream, e->P, e->T, e->S[e->cur],
                            e->tea.eps_units, e->tea.eps_ngrp, e->tea.eps_part);
         hipLaunchKernelGGL(k_tea_eps_reduce, dim3((e->nb + 255) / 256), dim3(256), 0, e->stream, e->T,
                            e->tea.eps_ngrp, e->tea.eps_part, e->tea.ci, e->tea.eps);
