@@ -134,14 +134,14 @@ __global__ void __launch_bounds__(TEA_TILE)
 // Epsilon partials and ci with each pair evaluated once.  The five sums of
 // k_tea_eps are even in e (cii^2, crr (2 cii + crr) e_a^2, 3 cii + crr
 // (ex + ey + ez)^2), so a pair adds the same terms to both beads.  Work unit:
-// (trajectory, 1024-bead group I, group J >= I), one workgroup of 8 waves;
-// wave w takes chunks c = 16 I + w and 16 I + w + 8 (64 beads each, one bead
-// of each per lane) against each 64-bead chunk c_J of group J in turn,
-// staged in LDS:
+// (trajectory, 1024-bead group I, group J >= I), one workgroup of
+// EPS_WAVES waves; wave w takes chunks c = 16 I + w + h EPS_WAVES, h <
+// EPS_PER (64 beads each, one bead of each per lane) against each 64-bead
+// chunk c_J of group J in turn, staged in LDS:
 //   c_J > c    both sides, lane l meeting j = (l + s) mod 64 at step s, the
 //              j-side sums carried along by a one-lane rotation (wave_rol1);
-//              when c_J lies above both chunks, the lane's two beads meet
-//              the same j together and share the rotation,
+//              when c_J lies above all of the wave's chunks, the lane's
+//              beads meet the same j together and share the rotation,
 //   c_J = c    one-sided over the chunk, the j == i term taken out after,
 //   c_J < c    nothing (that pair of chunks is the wave c_J's).
 // The waves' j-side sums of a chunk are added in wave order in LDS.  Output
@@ -149,8 +149,10 @@ __global__ void __launch_bounds__(TEA_TILE)
 // part[0][J][.] for J >= H, j-side sums into part[1][I][.] for I <= H; the
 // pair weight is 1 for two non-extra beads and 0 otherwise (WEIGHTED: some
 // bead of the batch is extra; without, every weight is 1 and none is formed).
-constexpr int EPS_CHUNKS = EPS_GROUP / 64;   // 64-bead chunks of a group
-constexpr int EPS_WAVES = EPS_CHUNKS / 2;    // waves of a workgroup, two chunks each
+constexpr int EPS_CHUNKS = EPS_GROUP / 64;        // 64-bead chunks of a group
+constexpr int EPS_PER = 4;                        // chunks of a wave
+constexpr int EPS_WAVES = EPS_CHUNKS / EPS_PER;   // waves of a workgroup
+static_assert(EPS_CHUNKS % EPS_PER == 0 && EPS_WAVES >= 2, "chunks split over the waves; 128 threads stage c_J");
 template <bool PERIODIC, bool WEIGHTED>
 __global__ void __launch_bounds__(64 * EPS_WAVES)
     k_tea_eps_sym(mte_params P, DevTopo T, DevState S, const int4 *units, int ngrp, float *part) {
@@ -168,13 +170,13 @@ __global__ void __launch_bounds__(64 * EPS_WAVES)
     const float sc = wrap ? inv_side(box) : 1.f, wf = wrap ? 1.f : 0.f;
     const float a = P.tea_a * sc, ia = rcp_fast(P.tea_a) * (wrap ? box : 1.f);
     const float r2min = 1e-12f * sc * sc;
-    // the lane's bead in each of the wave's two chunks
-    int cI[2], li[2];
-    bool vi[2];
-    float3 xi[2];
-    float wi[2], ai[2][5];
+    // the lane's bead in each of the wave's chunks
+    int cI[EPS_PER], li[EPS_PER];
+    bool vi[EPS_PER];
+    float3 xi[EPS_PER];
+    float wi[EPS_PER], ai[EPS_PER][5];
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
+    for (int h = 0; h < EPS_PER; h++) {
         cI[h] = u.y * EPS_CHUNKS + w + h * EPS_WAVES;
         li[h] = cI[h] * 64 + l;
         vi[h] = li[h] < n;
@@ -257,25 +259,32 @@ __global__ void __launch_bounds__(64 * EPS_WAVES)
         // gj the sums of that j (rotated in from lane l + 1, which held j at
         // the sweep's step s - 1; sweeps in a row continue the rotation).
         // A chunk below c_J is full; c_J may be the padded last one.
-        const bool two0 = cI[0] < nch && cJ > cI[0], two1 = cI[1] < nch && cJ > cI[1];
+        bool two[EPS_PER], all = true;
+#pragma unroll
+        for (int h = 0; h < EPS_PER; h++) {
+            two[h] = cI[h] < nch && cJ > cI[h];
+            all = all && two[h];
+        }
         const bool wtj = WEIGHTED || (cJ + 1) * 64 > n;
-        if (two0 && two1) {
+        if (all) {
             if (wtj)
                 for (int s = 0; s < 64; s++) {
                     const float4 xj = jx[l + s];
                     pair(yes, yes, xi[0], wi[0], xj, ai[0], gj);
-                    pair(yes, no, xi[1], wi[1], xj, ai[1], gj);
+#pragma unroll
+                    for (int h = 1; h < EPS_PER; h++) pair(yes, no, xi[h], wi[h], xj, ai[h], gj);
                 }
             else
                 for (int s = 0; s < 64; s++) {
                     const float4 xj = jx[l + s];
                     pair(no, yes, xi[0], wi[0], xj, ai[0], gj);
-                    pair(no, no, xi[1], wi[1], xj, ai[1], gj);
+#pragma unroll
+                    for (int h = 1; h < EPS_PER; h++) pair(no, no, xi[h], wi[h], xj, ai[h], gj);
                 }
         } else {
 #pragma unroll
-            for (int h = 0; h < 2; h++)
-                if (h == 0 ? two0 : two1) {
+            for (int h = 0; h < EPS_PER; h++)
+                if (two[h]) {
                     if (wtj)
                         for (int s = 0; s < 64; s++) pair(yes, yes, xi[h], wi[h], jx[l + s], ai[h], gj);
                     else
@@ -283,7 +292,7 @@ __global__ void __launch_bounds__(64 * EPS_WAVES)
                 }
         }
 #pragma unroll
-        for (int h = 0; h < 2; h++)
+        for (int h = 0; h < EPS_PER; h++)
             if (cI[h] < nch && cJ == cI[h]) {
                 // one-sided over the chunk's beads: no padding slot is met
                 const int jn = min(64, n - cJ * 64);
@@ -298,13 +307,13 @@ __global__ void __launch_bounds__(64 * EPS_WAVES)
                     ai[h][4] = fmaf(-3.f, cii, ai[h][4]);
                 }
             }
-        // after 64 (or 128) rotations lane l holds the sums of j = (l + 63)
-        // mod 64; with none, gj is zero
+        // after a multiple of 64 rotations lane l holds the sums of j =
+        // (l + 63) mod 64; with none, gj is zero
 #pragma unroll
         for (int k = 0; k < 5; k++) jred[w][k][(l + 63) & 63] = gj[k];
         __syncthreads();
-        if (threadIdx.x < 5 * 64) {
-            const int k = threadIdx.x / 64, q = threadIdx.x % 64;
+        for (int e = threadIdx.x; e < 5 * 64; e += 64 * EPS_WAVES) {
+            const int k = e / 64, q = e % 64;
             const int lj = cJ * 64 + q;
             if (lj < n) {
                 float sum = 0.f;
@@ -315,7 +324,7 @@ __global__ void __launch_bounds__(64 * EPS_WAVES)
         __syncthreads();
     }
 #pragma unroll
-    for (int h = 0; h < 2; h++)
+    for (int h = 0; h < EPS_PER; h++)
         if (vi[h]) {
 #pragma unroll
             for (int k = 0; k < 5; k++) part[(((size_t)0 * ngrp + u.z) * 5 + k) * nb + base + li[h]] = ai[h][k];
