@@ -48,7 +48,8 @@ static void release(mte_engine *e, void *p) {
 // Work units (trajectory, 256-bead i group, j chunk) of the TEA pair kernel
 // and its per-chunk partial buffer, for j-chunks of `chunk` beads; for
 // MTE_TEA_SYMMETRIC and MTE_TEA_MFMA, units (trajectory, 64-bead chunk I,
-// chunk J >= I), and for MTE_TEA_MFMA the tile tables as well.
+// chunk J >= I) (for MTE_TEA_SYMMETRIC, one or two I chunks below J), and
+// for MTE_TEA_MFMA the tile tables as well.
 hipError_t mte::tea_build_units(mte_engine *e, int chunk) {
     release(e, e->tea.units);
     release(e, e->tea.nchunks);
@@ -73,7 +74,14 @@ hipError_t mte::tea_build_units(mte_engine *e, int chunk) {
     for (int t = 0; t < e->ntraj; t++) {
         const int n = (int)(e->off[t + 1] - e->off[t]);
         int c = 0;
-        if (chunk == MTE_TEA_SYMMETRIC || chunk == MTE_TEA_MFMA) {
+        if (chunk == MTE_TEA_SYMMETRIC) {
+            // (trajectory, I, J, number of I chunks): the diagonal chunk
+            // alone, the chunks below J two at a time (I and I + 64)
+            for (int j0 = 0; j0 < n; j0 += 64, c++) {
+                units.push_back(make_int4(t, j0, j0, 1));
+                for (int i0 = 0; i0 < j0; i0 += 128) units.push_back(make_int4(t, i0, j0, i0 + 64 < j0 ? 2 : 1));
+            }
+        } else if (chunk == MTE_TEA_MFMA) {
             for (int i0 = 0; i0 < n; i0 += 64, c++)
                 for (int j0 = i0; j0 < n; j0 += 64) units.push_back(make_int4(t, i0, j0, c));
         } else {
@@ -88,7 +96,7 @@ hipError_t mte::tea_build_units(mte_engine *e, int chunk) {
         // diagonal blocks (one-sided, one step per bead of the chunk)
         auto cost = [&](const int4 &u) {
             const int n = (int)(e->off[u.x + 1] - e->off[u.x]);
-            return u.y == u.z ? std::min(64, n - u.z) : 128;
+            return u.y == u.z ? std::min(64, n - u.z) : (chunk == MTE_TEA_SYMMETRIC ? 128 * u.w : 128);
         };
         std::stable_sort(units.begin(), units.end(),
                          [&](const int4 &a, const int4 &b) { return cost(a) > cost(b); });

@@ -485,7 +485,9 @@ __global__ void __launch_bounds__(TEA_TILE)
 //   i-side  D_hat_ij (F_j + beta C_i . R_j)  into partial slot J of bead i,
 //   j-side  D_hat_ij (F_i + beta C_j . R_i)  into partial slot I of bead j,
 // and slot c of a bead holds the sum over chunk c, as k_tea_pairs<64> writes
-// it.  Off the diagonal, lane l takes j = (l + s) mod 64 at step s: every
+// it (a unit may take two I chunks below J, I and I + 64: its lanes meet j
+// with both of their beads before g moves on, slot I of a j bead holds the
+// sum over both chunks and slot I + 1 zero).  Off the diagonal, lane l takes j = (l + s) mod 64 at step s: every
 // lane meets every j of the chunk once and no two lanes the same j in a
 // step.  The j-side sum of j travels with j in registers: after each step
 // the wave rotates it one lane down (DPP wave_rol:1, lane l takes lane l+1's
@@ -510,19 +512,28 @@ __global__ void __launch_bounds__(SYM_CHUNK)
     const int base = T.tbase[u.x], n = T.tsize[u.x];
     const float bt = beta[u.x], w2 = bt * bt;
     const float box = T.tbox[u.x], ibox = inv_side(box);
-    // bead of this lane in chunk I ...
-    const int li = u.y + l, bi = base + li;
-    const bool vi = has && li < n;
-    const bool ei = vi && T.extra[bi];
-    float3 xi = make_float3(0.f, 0.f, 0.f), Fi = xi, Ri = xi, Ci = xi;
-    if (vi) {
-        xi = f3(S.xyz[bi]);
-        if (!ei) {
-            Fi = f3(fxyz[bi]);
-            Ri = f3(R[bi]);
-            const float4 cc = ci[bi];
-            Ci = make_float3(bt * rsq_fast(1.f + w2 * cc.x), bt * rsq_fast(1.f + w2 * cc.y),
-                             bt * rsq_fast(1.f + w2 * cc.z));
+    // beads of this lane in chunk I and, in a two-chunk unit (u.w = 2, both
+    // chunks below J), in chunk I + 64 ...
+    const int ni = u.w == 2 ? 2 : 1;
+    int bi[2];
+    bool vi[2], ei[2];
+    float3 xi[2], Fi[2], Ri[2], Ci[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int li = u.y + 64 * h + l;
+        bi[h] = base + li;
+        vi[h] = has && h < ni && li < n;
+        ei[h] = vi[h] && T.extra[bi[h]];
+        xi[h] = Fi[h] = Ri[h] = Ci[h] = make_float3(0.f, 0.f, 0.f);
+        if (vi[h]) {
+            xi[h] = f3(S.xyz[bi[h]]);
+            if (!ei[h]) {
+                Fi[h] = f3(fxyz[bi[h]]);
+                Ri[h] = f3(R[bi[h]]);
+                const float4 cc = ci[bi[h]];
+                Ci[h] = make_float3(bt * rsq_fast(1.f + w2 * cc.x), bt * rsq_fast(1.f + w2 * cc.y),
+                                    bt * rsq_fast(1.f + w2 * cc.z));
+            }
         }
     }
     // ... and in chunk J, staged for the wave
@@ -552,67 +563,79 @@ __global__ void __launch_bounds__(SYM_CHUNK)
     }
     __syncthreads();
     const float a = P.tea_a, ia = rcp_fast(P.tea_a);
-    float3 acc = make_float3(0.f, 0.f, 0.f), g = acc;
+    float3 acc[2] = {make_float3(0.f, 0.f, 0.f), make_float3(0.f, 0.f, 0.f)}, g = acc[0];
     const bool diag = u.y == u.z;
     if (has && diag) {
-        if (vi && !ei) {
+        if (vi[0] && !ei[0]) {
             const int jn = min(SYM_CHUNK, n - u.z);
             for (int q = 0; q < jn; q++) {
                 const float4 A = jd[0][q], B = jd[1][q], Cq = jd[2][q];
-                float3 d = make_float3(A.x - xi.x, A.y - xi.y, A.z - xi.z);
+                float3 d = make_float3(A.x - xi[0].x, A.y - xi[0].y, A.z - xi[0].z);
                 if (PERIODIC) d = min_image_inv(d, box, ibox);
                 const float r2 = fmaxf(d.x * d.x + d.y * d.y + d.z * d.z, 1e-12f);
                 const float rinv = rsq_fast(r2);
                 const RpyCoef c = rpy_select(r2, rinv, a, ia);
-                const float vx = fmaf(Ci.x, B.z, A.w), vy = fmaf(Ci.y, B.w, B.x), vz = fmaf(Ci.z, Cq.x, B.y);
+                const float vx = fmaf(Ci[0].x, B.z, A.w), vy = fmaf(Ci[0].y, B.w, B.x), vz = fmaf(Ci[0].z, Cq.x, B.y);
                 const float sv = (c.crr * rinv * rinv) * (d.x * vx + d.y * vy + d.z * vz);
-                acc.x = fmaf(sv, d.x, fmaf(c.cii, vx, acc.x));
-                acc.y = fmaf(sv, d.y, fmaf(c.cii, vy, acc.y));
-                acc.z = fmaf(sv, d.z, fmaf(c.cii, vz, acc.z));
+                acc[0].x = fmaf(sv, d.x, fmaf(c.cii, vx, acc[0].x));
+                acc[0].y = fmaf(sv, d.y, fmaf(c.cii, vy, acc[0].y));
+                acc[0].z = fmaf(sv, d.z, fmaf(c.cii, vz, acc[0].z));
             }
             // the j == i slot (d = 0) left its cii v_i term: taken out again
             const float r2 = 1e-12f, rinv = rsq_fast(r2);
             const float cii = rpy_select(r2, rinv, a, ia).cii;
-            acc.x = fmaf(-cii, fmaf(Ci.x, Ri.x, Fi.x), acc.x);
-            acc.y = fmaf(-cii, fmaf(Ci.y, Ri.y, Fi.y), acc.y);
-            acc.z = fmaf(-cii, fmaf(Ci.z, Ri.z, Fi.z), acc.z);
+            acc[0].x = fmaf(-cii, fmaf(Ci[0].x, Ri[0].x, Fi[0].x), acc[0].x);
+            acc[0].y = fmaf(-cii, fmaf(Ci[0].y, Ri[0].y, Fi[0].y), acc[0].y);
+            acc[0].z = fmaf(-cii, fmaf(Ci[0].z, Ri[0].z, Fi[0].z), acc[0].z);
         }
     } else if (has) {
         const float4 *j0 = &jd[0][l], *j1 = &jd[1][l], *j2 = &jd[2][l];
-        auto sweep = [&](auto far) {
+        // bead h of the lane against j: i-side into acc[h], j-side into g
+        auto pair = [&](auto far, int h, const float4 &A, const float4 &B, const float4 &Cq) {
+            float3 d = make_float3(A.x - xi[h].x, A.y - xi[h].y, A.z - xi[h].z);
+            if (PERIODIC) d = min_image_inv(d, box, ibox);
+            // (the clamp is the identity beyond 2a)
+            const float r2s = d.x * d.x + d.y * d.y + d.z * d.z;
+            const float r2 = decltype(far)::value ? r2s : fmaxf(r2s, 1e-12f);
+            const float rinv = rsq_fast(r2);
+            const RpyCoef c = decltype(far)::value ? rpy_far(rinv, a) : rpy_select(r2, rinv, a, ia);
+            const float cr = c.crr * rinv * rinv;
+            // i-side: F_j + beta C_i . R_j
+            const float vx = fmaf(Ci[h].x, B.z, A.w), vy = fmaf(Ci[h].y, B.w, B.x), vz = fmaf(Ci[h].z, Cq.x, B.y);
+            const float sv = cr * (d.x * vx + d.y * vy + d.z * vz);
+            acc[h].x = fmaf(sv, d.x, fmaf(c.cii, vx, acc[h].x));
+            acc[h].y = fmaf(sv, d.y, fmaf(c.cii, vy, acc[h].y));
+            acc[h].z = fmaf(sv, d.z, fmaf(c.cii, vz, acc[h].z));
+            // j-side: F_i + beta C_j . R_i (e e^T is even in e)
+            const float wx = fmaf(Cq.y, Ri[h].x, Fi[h].x), wy = fmaf(Cq.z, Ri[h].y, Fi[h].y),
+                        wz = fmaf(Cq.w, Ri[h].z, Fi[h].z);
+            const float sw = cr * (d.x * wx + d.y * wy + d.z * wz);
+            g.x = fmaf(sw, d.x, fmaf(c.cii, wx, g.x));
+            g.y = fmaf(sw, d.y, fmaf(c.cii, wy, g.y));
+            g.z = fmaf(sw, d.z, fmaf(c.cii, wz, g.z));
+        };
+        // each step: the lane's beads meet j together, then g moves on
+        auto sweep = [&](auto far, auto two) {
             for (int s = 0; s < SYM_CHUNK; s++) {
                 const float4 A = j0[s], B = j1[s], Cq = j2[s];
-                float3 d = make_float3(A.x - xi.x, A.y - xi.y, A.z - xi.z);
-                if (PERIODIC) d = min_image_inv(d, box, ibox);
-                // (the clamp is the identity beyond 2a)
-                const float r2s = d.x * d.x + d.y * d.y + d.z * d.z;
-                const float r2 = decltype(far)::value ? r2s : fmaxf(r2s, 1e-12f);
-                const float rinv = rsq_fast(r2);
-                const RpyCoef c = decltype(far)::value ? rpy_far(rinv, a) : rpy_select(r2, rinv, a, ia);
-                const float cr = c.crr * rinv * rinv;
-                // i-side: F_j + beta C_i . R_j
-                const float vx = fmaf(Ci.x, B.z, A.w), vy = fmaf(Ci.y, B.w, B.x), vz = fmaf(Ci.z, Cq.x, B.y);
-                const float sv = cr * (d.x * vx + d.y * vy + d.z * vz);
-                acc.x = fmaf(sv, d.x, fmaf(c.cii, vx, acc.x));
-                acc.y = fmaf(sv, d.y, fmaf(c.cii, vy, acc.y));
-                acc.z = fmaf(sv, d.z, fmaf(c.cii, vz, acc.z));
-                // j-side: F_i + beta C_j . R_i (e e^T is even in e)
-                const float wx = fmaf(Cq.y, Ri.x, Fi.x), wy = fmaf(Cq.z, Ri.y, Fi.y), wz = fmaf(Cq.w, Ri.z, Fi.z);
-                const float sw = cr * (d.x * wx + d.y * wy + d.z * wz);
-                g.x = wave_rol1(fmaf(sw, d.x, fmaf(c.cii, wx, g.x)));
-                g.y = wave_rol1(fmaf(sw, d.y, fmaf(c.cii, wy, g.y)));
-                g.z = wave_rol1(fmaf(sw, d.z, fmaf(c.cii, wz, g.z)));
+                pair(far, 0, A, B, Cq);
+                if (decltype(two)::value) pair(far, 1, A, B, Cq);
+                g.x = wave_rol1(g.x);
+                g.y = wave_rol1(g.y);
+                g.z = wave_rol1(g.z);
             }
         };
-        // Open boundaries: when the bounding boxes of the two chunks lie more
-        // than 2a apart (with a margin over the rounding of r / a), every
-        // pair takes the far branch and the selection is skipped (same
-        // values).  Slots past the trajectory's end stand in as the chunk's
-        // first bead.
+        // Open boundaries: when the bounding boxes of the I beads and of
+        // chunk J lie more than 2a apart (with a margin over the rounding of
+        // r / a), every pair takes the far branch and the selection is
+        // skipped (same values).  Slots past the trajectory's end stand in
+        // as the chunk's first bead; a one-chunk unit counts its bead twice.
         bool far = false;
         if (!PERIODIC) {
             const float4 xj = jd[0][vj ? l : 0];
-            float lo[6] = {xi.x, xi.y, xi.z, xj.x, xj.y, xj.z}, hi[6] = {xi.x, xi.y, xi.z, xj.x, xj.y, xj.z};
+            const float3 xb = ni == 2 ? xi[1] : xi[0];
+            float lo[6] = {fminf(xi[0].x, xb.x), fminf(xi[0].y, xb.y), fminf(xi[0].z, xb.z), xj.x, xj.y, xj.z};
+            float hi[6] = {fmaxf(xi[0].x, xb.x), fmaxf(xi[0].y, xb.y), fmaxf(xi[0].z, xb.z), xj.x, xj.y, xj.z};
 #pragma unroll
             for (int m = 1; m < SYM_CHUNK; m <<= 1)
 #pragma unroll
@@ -626,14 +649,26 @@ __global__ void __launch_bounds__(SYM_CHUNK)
             const float lim = 2.002f * a;
             far = __builtin_amdgcn_readfirstlane((int)(gx * gx + gy * gy + gz * gz > lim * lim)) != 0;
         }
-        if (far) sweep(std::true_type{});
-        else sweep(std::false_type{});
+        if (ni == 2) {
+            if (far) sweep(std::true_type{}, std::true_type{});
+            else sweep(std::false_type{}, std::true_type{});
+        } else {
+            if (far) sweep(std::true_type{}, std::false_type{});
+            else sweep(std::false_type{}, std::false_type{});
+        }
     }
-    if (vi) partial[(size_t)(u.z / SYM_CHUNK) * chunk_stride + bi] = ei ? make_float4(0.f, 0.f, 0.f, 0.f)
-                                                                         : make_float4(acc.x, acc.y, acc.z, 0.f);
-    if (vj && !diag)
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+        if (vi[h])
+            partial[(size_t)(u.z / SYM_CHUNK) * chunk_stride + bi[h]] =
+                ei[h] ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(acc[h].x, acc[h].y, acc[h].z, 0.f);
+    if (vj && !diag) {
+        // a two-chunk unit's j-side sum covers both chunks: slot I holds
+        // it, slot I + 1 zero
         partial[(size_t)(u.y / SYM_CHUNK) * chunk_stride + bj] = ej ? make_float4(0.f, 0.f, 0.f, 0.f)
                                                                      : make_float4(g.x, g.y, g.z, 0.f);
+        if (ni == 2) partial[(size_t)(u.y / SYM_CHUNK + 1) * chunk_stride + bj] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
 }
 
 // Self term, chunk partials in fixed order, and the update of one bead
