@@ -237,7 +237,7 @@ __global__ void __launch_bounds__(MLP_BLOCK)
                 qend = Ranges::hi(w, base);
             }
         };
-        constexpr int PF = 2;   // 4 measured no faster
+        constexpr int PF = 2;   // 3 and 4 measured no faster
         float4 pf[PF];
 #pragma unroll
         for (int u = 0; u < PF; u++) pf[u] = make_float4(0.f, 0.f, 0.f, 0.f);
